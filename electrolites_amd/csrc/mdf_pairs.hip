@@ -1,0 +1,142 @@
+// Pair-vector gather/scatter around the J/K contractions.
+//
+// mdf_pack_dm restates electrolites/fastdf.py:674-686 (_sparse_dm):
+//     d[n, p] = 2 D[i, j]            (hermi = 1)
+//             = D[i, j] + D[j, i]    (hermi = 0), diagonal pairs halved.
+// mdf_unpack_pairs restates the vj scatter of fastdf.py:1014-1015 and
+// :736-738: vj[:, i, j] = vj[:, j, i] = vj_pairs, zero where no pair is
+// stored.
+// Pairs are given by an address list (addr = i*nao + j, i >= j) or, when the
+// list is NULL, are the full lower triangle p = i(i+1)/2 + j.
+#include <cmath>
+
+#include "mdf_common.h"
+#include "../../include/mi355df.h"
+
+namespace {
+
+__device__ inline void pair_ij(long p, const int *addr, int nao, int &i, int &j) {
+    if (addr) {
+        int a = addr[p];
+        i = a / nao;
+        j = a % nao;
+    } else {
+        long r = (long)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+        while (r * (r + 1) / 2 > p) r--;
+        while ((r + 1) * (r + 2) / 2 <= p) r++;
+        i = (int)r;
+        j = (int)(p - r * (r + 1) / 2);
+    }
+}
+
+__global__ __launch_bounds__(256) void pack_dm_kernel(const double *__restrict__ dms, int ndm, int nao,
+                                                      long ldm, const int *__restrict__ addr, long npairs,
+                                                      int hermi, double *__restrict__ d, long ldd) {
+    long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= npairs) return;
+    int i, j;
+    pair_ij(p, addr, nao, i, j);
+    for (int n = 0; n < ndm; n++) {
+        const double *D = dms + n * ldm;
+        double v = hermi ? 2. * D[(long)i * nao + j] : D[(long)i * nao + j] + D[(long)j * nao + i];
+        if (i == j) v *= .5;
+        d[n * ldd + p] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(double *__restrict__ x, long n) {
+    long t = (long)blockIdx.x * 256 + threadIdx.x;
+    long stride = (long)gridDim.x * 256;
+    for (; t < n; t += stride) x[t] = 0.;
+}
+
+__global__ __launch_bounds__(256) void unpack_pairs_kernel(const double *__restrict__ vp, int ndm, long ldv,
+                                                           const int *__restrict__ addr, long npairs, int nao,
+                                                           double *__restrict__ out, long ldo) {
+    long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= npairs) return;
+    int i, j;
+    pair_ij(p, addr, nao, i, j);
+    for (int n = 0; n < ndm; n++) {
+        double v = vp[n * ldv + p];
+        out[n * ldo + (long)i * nao + j] = v;
+        out[n * ldo + (long)j * nao + i] = v;
+    }
+}
+
+}  // namespace
+
+extern "C" int mdf_pack_dm(const double *dms, int n_dm, int nao, long ldm, const int *pair_addr,
+                           long npairs, int hermi, double *dm_pairs, long ldd, void *stream) {
+    MDF_CHECK_ARG(dms && dm_pairs && n_dm >= 1 && nao >= 0 && ldm >= (long)nao * nao && ldd >= npairs,
+                  "mdf_pack_dm: bad arguments");
+    MDF_CHECK_ARG(pair_addr || npairs == (long)nao * (nao + 1) / 2,
+                  "mdf_pack_dm: without pair addresses npairs must be nao(nao+1)/2");
+    if (npairs == 0) return MDF_OK;
+    hipLaunchKernelGGL(pack_dm_kernel, dim3((unsigned)mdf::cdiv(npairs, 256L)), dim3(256), 0,
+                       mdf::as_stream(stream), dms, n_dm, nao, ldm, pair_addr, npairs, hermi, dm_pairs, ldd);
+    MDF_LAUNCHED("pack_dm_kernel");
+    return MDF_OK;
+}
+
+extern "C" int mdf_unpack_pairs(const double *vj_pairs, int n_dm, long ldv, const int *pair_addr,
+                                long npairs, int nao, double *out, long ldo, void *stream) {
+    MDF_CHECK_ARG(vj_pairs && out && n_dm >= 1 && ldo >= (long)nao * nao && ldv >= npairs,
+                  "mdf_unpack_pairs: bad arguments");
+    MDF_CHECK_ARG(pair_addr || npairs == (long)nao * (nao + 1) / 2,
+                  "mdf_unpack_pairs: without pair addresses npairs must be nao(nao+1)/2");
+    hipStream_t st = mdf::as_stream(stream);
+    if (pair_addr) {
+        // screened pairs leave holes that must read as zero
+        long n = (long)n_dm * ldo;
+        unsigned g = (unsigned)std::min(mdf::cdiv(n, 256L), 65536L);
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(g), dim3(256), 0, st, out, n);
+        MDF_LAUNCHED("fill_zero_kernel");
+    }
+    if (npairs == 0) return MDF_OK;
+    hipLaunchKernelGGL(unpack_pairs_kernel, dim3((unsigned)mdf::cdiv(npairs, 256L)), dim3(256), 0, st,
+                       vj_pairs, n_dm, ldv, pair_addr, npairs, nao, out, ldo);
+    MDF_LAUNCHED("unpack_pairs_kernel");
+    return MDF_OK;
+}
+
+extern "C" int mdf_fill_zero(double *x, long n, void *stream) {
+    MDF_CHECK_ARG(x || n == 0, "mdf_fill_zero: null pointer");
+    if (n == 0) return MDF_OK;
+    unsigned g = (unsigned)std::min(mdf::cdiv(n, 256L), 65536L);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(g), dim3(256), 0, mdf::as_stream(stream), x, n);
+    MDF_LAUNCHED("fill_zero_kernel");
+    return MDF_OK;
+}
+
+namespace {
+__global__ __launch_bounds__(256) void fill_int_kernel(int *__restrict__ x, long n, int v) {
+    long t = (long)blockIdx.x * 256 + threadIdx.x;
+    long stride = (long)gridDim.x * 256;
+    for (; t < n; t += stride) x[t] = v;
+}
+
+__global__ __launch_bounds__(256) void pair_map_kernel(const int *__restrict__ addr, long npairs, int nao,
+                                                       int *__restrict__ map) {
+    long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= npairs) return;
+    int a = addr[p], i = a / nao, j = a % nao;
+    map[(long)i * nao + j] = (int)p;
+    map[(long)j * nao + i] = (int)p;
+}
+}  // namespace
+
+// map[i*nao + j] = pair index of (i, j) (either order), -1 where not stored.
+extern "C" int mdf_build_pair_map(const int *pair_addr, long npairs, int nao, int *map, void *stream) {
+    MDF_CHECK_ARG(pair_addr && map && nao > 0 && npairs < (1L << 31), "mdf_build_pair_map: bad arguments");
+    hipStream_t st = mdf::as_stream(stream);
+    long n = (long)nao * nao;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)std::min(mdf::cdiv(n, 256L), 65536L)), dim3(256), 0, st,
+                       map, n, -1);
+    MDF_LAUNCHED("fill_int_kernel");
+    if (npairs == 0) return MDF_OK;
+    hipLaunchKernelGGL(pair_map_kernel, dim3((unsigned)mdf::cdiv(npairs, 256L)), dim3(256), 0, st,
+                       pair_addr, npairs, nao, map);
+    MDF_LAUNCHED("pair_map_kernel");
+    return MDF_OK;
+}

@@ -1,0 +1,320 @@
+"""Density-fitted J/K on MI355X: the drop-in for ``df_jk.get_jk``.
+
+Mirrors electrolites/fastdf.py (``get_jk`` :754, ``_get_jk`` :805,
+``_get_j`` :688, ``_sparse_dm`` :674, declines and fall-through :406-441,
+``STATS``/``stats()`` :271-285) with the arithmetic on hand-written gfx950
+HIP kernels behind libmi355df's C ABI:
+
+* J: ``mdf_pack_dm`` (pair gather with the symmetry factors), ``mdf_get_j``
+  (two streaming passes over the packed cderi), ``mdf_unpack_pairs``;
+* K: ``mdf_get_k`` (packed-triangle unpack fused into an FP64-MFMA step-1
+  transform, then a symmetric rank-k step 2).
+
+``dfobj`` is anything with the reference DF object's fields: ``_cderi``
+(per-device mapping to a ``(naux, npairs)`` float64 device array),
+``_cderi_idx = (pair_addresses, diag_idx)``, ``build()``.  This package's own
+:class:`electrolites_amd.df.DF` is one; a GPU4PySCF ``DF`` object is another
+when :func:`electrolites_amd.patch` has installed this function.
+
+Every case this module does not cover (a complex density, a range-separated
+``omega``, more than one device, a non-float64 tensor) is *declined*: counted
+in ``stats()`` and handed to the function this one replaced, or, when there is
+none (standalone use), raised as :class:`Declined`.
+"""
+import os
+
+import numpy as np
+
+from . import _dev
+from . import _lib
+
+STATS = {'calls': 0, 'declined': 0, 'j_only': 0, 'mode0': 0, 'mode1': 0,
+         'k_calls': 0}
+DECLINED = {}
+
+#: the function this one replaced (GPU4PySCF's) once patched; None standalone
+_orig_get_jk = None
+#: disable switch, read once at import like the reference's FASTDF_OFF
+_OFF = os.environ.get('FASTDF_OFF', '0') not in ('', '0')
+#: bytes of device scratch the K build may use for its transformed block
+K_WORKSPACE_BYTES = int(float(os.environ.get('FASTDF_K_WORKSPACE', 8 << 30)))
+#: relative cutoff below which density eigenvalues are dropped (untagged dm)
+FACTOR_CUTOFF = 1e-15
+
+
+class Declined(RuntimeError):
+    """This call is outside what the HIP path covers and nothing to fall
+    through to exists (standalone use)."""
+
+
+def stats():
+    """What this module did in this process."""
+    out = dict(STATS)
+    out['declined_reasons'] = dict(DECLINED)
+    return out
+
+
+def _decline(reason):
+    STATS['declined'] += 1
+    DECLINED[reason] = DECLINED.get(reason, 0) + 1
+    return reason
+
+
+def _fall_through(dfobj, dms, hermi, with_j, with_k, omega, reason):
+    _decline(reason)
+    cderi = _cderi_of(dfobj, missing_ok=True)
+    if cderi is not None and str(getattr(cderi, 'dtype', '')).endswith('float32'):
+        # fastdf.py:434-440: a narrow tensor must not reach a float64 reader
+        raise RuntimeError(f'the cderi is float32 and this call was declined '
+                           f'({reason}); rebuild it in float64')
+    if _orig_get_jk is None:
+        raise Declined(reason)
+    return _orig_get_jk(dfobj, dms, hermi, with_j, with_k, omega=omega)
+
+
+def _cderi_of(dfobj, missing_ok=False):
+    store = getattr(dfobj, '_cderi', None)
+    if store is None:
+        if missing_ok:
+            return None
+        raise ValueError('dfobj has no cderi; call build() first')
+    t = _dev.torch()
+    dev = t.cuda.current_device()
+    try:
+        return store[dev]
+    except (KeyError, IndexError, TypeError):
+        if missing_ok:
+            return None
+        raise
+
+
+# --------------------------------------------------------------- pair layout
+class _PairLayout:
+    """Device-side pair tables of one cderi, cached on the DF object."""
+
+    def __init__(self, pair_addr, nao, npairs):
+        t = _dev.torch()
+        self.nao, self.npairs = nao, npairs
+        full = npairs == nao * (nao + 1) // 2
+        addr = None
+        if pair_addr is not None:
+            addr = _dev.to_device(pair_addr, dtype=t.int32)
+            if full:
+                rows, cols = np.tril_indices(nao)
+                ref = t.from_numpy((rows * nao + cols).astype(np.int32)).to(addr.device)
+                full = bool(t.equal(addr, ref))
+        self.full = full
+        self.addr = None if full else addr
+        self.map = None
+        if not full:
+            if addr is None:
+                raise ValueError('a screened cderi needs its pair addresses')
+            self.map = t.empty(nao * nao, dtype=t.int32, device=_dev.device())
+            _lib.call('mdf_build_pair_map', _dev.ptr(addr), npairs, nao,
+                      _dev.ptr(self.map), _dev.stream_ptr())
+
+
+def _layout(dfobj, cderi, nao):
+    key = (id(cderi), nao, cderi.shape[1])
+    cached = getattr(dfobj, '_mdf_layout', None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    idx = getattr(dfobj, '_cderi_idx', None)
+    pair_addr = None if idx is None else idx[0]
+    lay = _PairLayout(pair_addr, nao, cderi.shape[1])
+    try:
+        dfobj._mdf_layout = (key, lay)
+    except AttributeError:
+        pass
+    return lay
+
+
+# ------------------------------------------------------------ density factors
+def factorize_dm(dms, hermi):
+    """``(fl, fr)`` with ``D = fl fr^T`` per density; ``fr is None`` when the
+    density is tagged with ``mo_coeff``/``mo_occ`` (mode 0, ``D = F F^T``).
+
+    Restates what GPU4PySCF's ``DFJK.factorize_dm`` provides to the reference
+    (call at fastdf.py:829; modes at :837-845): tagged -> ``C_occ sqrt(occ)``;
+    untagged -> two factors from ``eigh`` (hermi 1) or an SVD (hermi 0),
+    dropping values below ``FACTOR_CUTOFF`` of the largest.
+    Returns device tensors shaped ``(n, nao, nocc)``.
+    """
+    t = _dev.torch()
+    nao = dms.shape[-1]
+    mo_coeff = getattr(dms, 'mo_coeff', None)
+    mo_occ = getattr(dms, 'mo_occ', None)
+    if mo_coeff is not None and mo_occ is not None:
+        c = _dev.to_device(mo_coeff).reshape(-1, nao, np.shape(mo_coeff)[-1])
+        occ = _dev.to_device(mo_occ).reshape(c.shape[0], -1)
+        facs = []
+        nocc = int(max(int((occ[i] > 0).sum()) for i in range(c.shape[0])))
+        out = t.zeros((c.shape[0], nao, nocc), dtype=t.float64, device=c.device)
+        for i in range(c.shape[0]):
+            sel = occ[i] > 0
+            f = c[i][:, sel] * t.sqrt(occ[i][sel])
+            out[i, :, :f.shape[1]] = f
+        return out.contiguous(), None
+    d = _dev.to_device(dms).reshape(-1, nao, nao)
+    fls, frs = [], []
+    for i in range(d.shape[0]):
+        if hermi == 1:
+            w, v = t.linalg.eigh(.5 * (d[i] + d[i].T))
+            keep = w.abs() > FACTOR_CUTOFF * max(float(w.abs().max()), 1e-300)
+            w, v = w[keep], v[:, keep]
+            s = w.abs().sqrt()
+            fls.append(v * s)
+            frs.append(v * (s * t.sign(w)))
+        else:
+            u, s, vh = t.linalg.svd(d[i])
+            keep = s > FACTOR_CUTOFF * max(float(s.max()), 1e-300)
+            s = s[keep].sqrt()
+            fls.append(u[:, keep] * s)
+            frs.append(vh[keep].T * s)
+    nocc = max(f.shape[1] for f in fls)
+    fl = t.zeros((len(fls), nao, nocc), dtype=t.float64, device=d.device)
+    fr = t.zeros_like(fl)
+    for i, (a, b) in enumerate(zip(fls, frs)):
+        fl[i, :, :a.shape[1]] = a
+        fr[i, :, :b.shape[1]] = b
+    return fl.contiguous(), fr.contiguous()
+
+
+# ----------------------------------------------------------------- J and K
+def _sparse_dm(dms3, lay, hermi):
+    """The density on the stored pairs (fastdf.py:674-686), ``(n, npairs)``."""
+    t = _dev.torch()
+    n, nao = dms3.shape[0], dms3.shape[-1]
+    d = t.empty((n, lay.npairs), dtype=t.float64, device=dms3.device)
+    _lib.call('mdf_pack_dm', _dev.ptr(dms3), n, nao, nao * nao,
+              _dev.ptr(lay.addr), lay.npairs, 1 if hermi == 1 else 0,
+              _dev.ptr(d), lay.npairs, _dev.stream_ptr())
+    return d
+
+
+def _vj(cderi, lay, dms3, hermi):
+    t = _dev.torch()
+    n, nao = dms3.shape[0], dms3.shape[-1]
+    naux, npairs = cderi.shape
+    d = _sparse_dm(dms3, lay, hermi)
+    rho = t.empty((n, naux), dtype=t.float64, device=dms3.device)
+    vjp = t.empty((n, npairs), dtype=t.float64, device=dms3.device)
+    nbytes = _lib.query('mdf_get_j_workspace', naux, npairs, n)
+    work = _dev.WORKSPACE.get(nbytes)
+    _lib.call('mdf_get_j', _dev.ptr(cderi), naux, npairs, cderi.stride(0),
+              _dev.ptr(d), n, npairs, _dev.ptr(rho), naux, _dev.ptr(vjp),
+              npairs, 0, _dev.ptr(work), work.numel(), _dev.stream_ptr())
+    vj = t.empty((n, nao, nao), dtype=t.float64, device=dms3.device)
+    _lib.call('mdf_unpack_pairs', _dev.ptr(vjp), n, npairs, _dev.ptr(lay.addr),
+              npairs, nao, _dev.ptr(vj), nao * nao, _dev.stream_ptr())
+    return vj
+
+
+def _vk(cderi, lay, fl, fr):
+    """``vk[n] = sum_L (A_L fl[n])(A_L fr[n])^T`` on the HIP path."""
+    t = _dev.torch()
+    n, nao, nocc = fl.shape
+    naux, npairs = cderi.shape
+    vk = t.empty((n, nao, nao), dtype=t.float64, device=fl.device)
+    two = fr is not None
+    per_row = nao * nocc * 8 * (2 if two else 1)
+    rows = max(1, min(naux, K_WORKSPACE_BYTES // max(per_row, 1)))
+    nbytes = _lib.query('mdf_get_k_workspace', naux, nao, nocc, int(two), rows)
+    work = _dev.WORKSPACE.get(nbytes)
+    for i in range(n):
+        STATS['k_calls'] += 1
+        _lib.call('mdf_get_k', _dev.ptr(cderi), naux, npairs, cderi.stride(0),
+                  _dev.ptr(lay.map), nao, _dev.ptr(fl[i]),
+                  _dev.ptr(fr[i]) if two else None, nocc, nocc,
+                  _dev.ptr(vk[i]), nao, 0, _dev.ptr(work), work.numel(),
+                  _dev.stream_ptr())
+    return vk
+
+
+def _finish(x, dms, out_device):
+    x = x.reshape(tuple(np.shape(dms)))
+    if out_device:
+        return x
+    return x.cpu().numpy()
+
+
+def get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
+    """``(vj, vk)`` of ``dms`` over ``dfobj``'s cderi (df_jk.get_jk's
+    signature; fastdf.py:754).  numpy in gives numpy out; a device tensor in
+    gives device tensors out."""
+    return _get_jk(dfobj, dms, hermi, with_j, with_k, omega)
+
+
+def _get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
+    if _OFF:
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'FASTDF_OFF')
+    t = _dev.torch()
+    if not t.cuda.is_available():
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'no GPU')
+    if t.cuda.device_count() != 1 and os.environ.get('MDF_ALLOW_MULTI', '0') == '0':
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'more than one device')
+    dt = getattr(dms, 'dtype', None)
+    if dt not in (np.float64, t.float64):
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'complex density' if dt is not None and
+                             'complex' in str(dt) else 'non-float64 density')
+    if omega not in (None, 0, 0.):
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'range-separated omega')
+    if getattr(dfobj, '_cderi', None) is None:
+        dfobj.build()
+    cderi = _cderi_of(dfobj)
+    if cderi.dtype != t.float64:
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'non-float64 cderi')
+    if cderi.shape[0] == 0:
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'empty cderi slice')
+    if cderi.dim() != 2 or cderi.stride(1) != 1:
+        # the kernels stream rows of a pair-contiguous tensor; an
+        # aux-contiguous one (GPU4PySCF's own device layout) is not covered
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'aux-contiguous cderi layout')
+    STATS['calls'] += 1
+    out_device = _dev.is_device_array(dms)
+    nao = dms.shape[-1]
+    lay = _layout(dfobj, cderi, nao)
+    dms3 = _dev.to_device(dms).reshape(-1, nao, nao)
+
+    if not with_k:
+        STATS['j_only'] += 1
+        vj = _vj(cderi, lay, dms3, hermi) if with_j else None
+        return (None if vj is None else _finish(vj, dms, out_device)), None
+
+    fl, fr = factorize_dm(dms, hermi)
+    if fl.shape[-1] == 0:
+        return dms, dms                       # fastdf.py:850-851
+    STATS['mode1' if fr is not None else 'mode0'] += 1
+    if fl.shape[0] != dms3.shape[0]:
+        raise ValueError(f'{fl.shape[0]} density factors for '
+                         f'{dms3.shape[0]} densities')
+    vj = _vj(cderi, lay, dms3, hermi) if with_j else None
+    vk = _vk(cderi, lay, fl, fr)
+    return ((None if vj is None else _finish(vj, dms, out_device)),
+            _finish(vk, dms, out_device))
+
+
+def tag_array(a, **kwargs):
+    """``pyscf.lib.tag_array``: an ndarray view (or the tensor itself) carrying
+    ``mo_coeff``/``mo_occ`` attributes, which select the one-factor K."""
+    t = _dev.torch()
+    if isinstance(a, t.Tensor):
+        for k, v in kwargs.items():
+            setattr(a, k, v)
+        return a
+    out = np.asarray(a).view(_Tagged)
+    out.__dict__.update(kwargs)
+    return out
+
+
+class _Tagged(np.ndarray):
+    """ndarray subclass whose instance dict holds the tags; views and results
+    of arithmetic do not inherit them."""

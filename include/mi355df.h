@@ -1,0 +1,89 @@
+/*
+ * mi355df -- C ABI of the MI355X (gfx950) density-fitted J/K library.
+ *
+ * Conventions for every entry point:
+ *   - plain C, extern "C"; returns 0 on success, otherwise an MDF_E* code and
+ *     a message readable with mdf_last_error() (thread-local);
+ *   - every array argument is a DEVICE pointer unless said otherwise; row-major
+ *     with an explicit leading dimension (in elements);
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream); calls
+ *     are asynchronous on it;
+ *   - scratch memory is a caller-provided workspace (`work`, `work_bytes`),
+ *     sized by the matching *_workspace() query; hot calls allocate nothing.
+ *
+ * The cderi is the reference's packed three-index tensor: (naux, npairs),
+ * pair index contiguous, pair p <-> (i >= j) with address i*nao + j
+ * (electrolites/fastdfcderi.py:446-449, 690).  Without a pair map the pairs
+ * are the full lower triangle in row-major order, p = i(i+1)/2 + j.
+ */
+#ifndef MI355DF_H
+#define MI355DF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDF_ABI_VERSION 1
+
+/* ---- library / device ------------------------------------------------- */
+int mdf_abi_version(void);
+const char *mdf_last_error(void);
+/* cu_count, HBM bytes and gfx arch name of the current device (host pointers). */
+int mdf_device_info(int *cu_count, long long *hbm_bytes, char *arch, int arch_len);
+
+/* ---- J: Coulomb over the packed cderi -----------------------------------
+ * Replaces electrolites/fastdf.py:348-386 (_vj_pass: rho = L.d, vj = L^T.rho)
+ * as used by _get_jk (:936-946) and the J-only branch _get_j (:688-745).
+ *   cderi     (naux, npairs), row stride ld
+ *   dm_pairs  (n_dm, npairs), row stride ldd: the density gathered onto the
+ *             pairs with the symmetry factors of fastdf.py:674-686
+ *   rho       (n_dm, naux) out, row stride ldrho (the fitted density)
+ *   vj_pairs  (n_dm, npairs) out, row stride ldvj; accumulate != 0 adds.
+ */
+size_t mdf_get_j_workspace(long naux, long npairs, int n_dm);
+int mdf_get_j(const double *cderi, long naux, long npairs, long ld,
+              const double *dm_pairs, int n_dm, long ldd,
+              double *rho, long ldrho, double *vj_pairs, long ldvj, int accumulate,
+              void *work, size_t work_bytes, void *stream);
+
+/* ---- K: exchange over the packed cderi (FP64 MFMA) ----------------------
+ * Replaces electrolites/fastdf.py:974-1005 (fill_symmetric unpack, step 1
+ * contract('Lij,njk->nikL'), step 2 _syrk / contract('nikL,njkL->nij')) and
+ * _fill_from_triangle (:389-402), for one density:
+ *   vk = sum_L (A_L fl)(A_L fr)^T,   fr == NULL: fr = fl (symmetric, mode 0)
+ *   pair_map  (nao*nao) int32 pair index of (i,j) or -1 when screened; NULL
+ *             when the cderi holds the full packed triangle
+ *   fl, fr    (nao, nocc) density factors, row stride ldf
+ *   vk        (nao, nao) out, row stride ldvk; accumulate != 0 adds.
+ * max_block_rows bounds the aux rows transformed per pass (workspace size).
+ */
+size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_factors, long max_block_rows);
+int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *pair_map,
+              int nao, const double *fl, const double *fr, int nocc, long ldf,
+              double *vk, long ldvk, int accumulate, void *work, size_t work_bytes,
+              void *stream);
+
+/* ---- pair-vector helpers ----------------------------------------------
+ * mdf_pack_dm replaces electrolites/fastdf.py:674-686 (_sparse_dm): the
+ * density gathered onto the stored pairs, 2 D_ij (hermi 1) or D_ij + D_ji
+ * (hermi 0), diagonal halved.  dms is (n_dm, nao, nao) with matrix stride ldm.
+ * mdf_unpack_pairs replaces fastdf.py:1014-1015: out[n, i, j] = out[n, j, i]
+ * = vj_pairs[n, p], zero where no pair is stored; matrix stride ldo.
+ * pair_addr (int32, i*nao + j) may be NULL for the full lower triangle.
+ * mdf_build_pair_map: map[i*nao + j] = p (both orders), -1 if not stored;
+ * the table mdf_get_k reads for a screened cderi.
+ */
+int mdf_pack_dm(const double *dms, int n_dm, int nao, long ldm, const int *pair_addr,
+                long npairs, int hermi, double *dm_pairs, long ldd, void *stream);
+int mdf_unpack_pairs(const double *vj_pairs, int n_dm, long ldv, const int *pair_addr,
+                     long npairs, int nao, double *out, long ldo, void *stream);
+int mdf_build_pair_map(const int *pair_addr, long npairs, int nao, int *map, void *stream);
+int mdf_fill_zero(double *x, long n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MI355DF_H */

@@ -163,12 +163,13 @@ def cholesky_eri(j3c, j2c, pair_addr=None):
     b = j3c[rows, cols, :].T.copy()                 # (naux, npairs)
     try:
         low = scipy.linalg.cholesky(j2c, lower=True)
-        return scipy.linalg.solve_triangular(low, b, lower=True)
+        return np.ascontiguousarray(
+            scipy.linalg.solve_triangular(low, b, lower=True))
     except np.linalg.LinAlgError:
         w, v = scipy.linalg.eigh(j2c)
         keep = w > LINEAR_DEP_THR
         v = v[:, keep] / np.sqrt(w[keep])
-        return v.T @ b
+        return np.ascontiguousarray(v.T @ b)
 
 
 # ------------------------------------------------------------------- J/K
