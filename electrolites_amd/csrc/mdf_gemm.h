@@ -1,0 +1,173 @@
+// FP64 MFMA tile engine shared by the K contraction and the metric solve.
+//
+// One workgroup (256 threads = 4 wavefronts as 2 x 2) computes a BM x BN
+// tile of C = sum_k A[m, k] B[k, n] over nk K-steps of BK = 16.  Operands are
+// staged through double-buffered LDS (A as As[k][m], B as Bs[k][n]) with the
+// next K-step's global loads held in registers while the current step's MFMAs
+// run, so there is one barrier per K-step.  Loaders decide where a tile comes
+// from (packed triangle, row-major matrix, transformed block ...):
+//
+//   struct Loader {
+//     static constexpr int N;                 // elements per thread per K-step
+//     __device__ void load(int kt, double (&r)[N]);    // global -> registers
+//     __device__ void store(const double (&r)[N], double *lds, int pitch);
+//   };
+//
+// v_mfma_f64_16x16x4_f64 operand layout on gfx950: A lane = m + 16 k,
+// B lane = n + 16 k, D element v of lane l is row l/16 + 4 v, column l % 16.
+#pragma once
+#include "mdf_common.h"
+
+namespace mdf {
+
+constexpr int G_THREADS = 256;
+constexpr int G_BK = 16;
+
+__device__ inline double4_t mfma16(double a, double b, double4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int BM, int BN>
+struct TileShape {
+    static constexpr int WM = BM / 2, WN = BN / 2;       // per-wave tile
+    static constexpr int MI = WM / 16, NI = WN / 16;     // MFMA tiles per wave
+    static constexpr int PA = BM + 16, PB = BN + 16;     // LDS pitches (doubles)
+    static constexpr int LDS_DOUBLES = 2 * G_BK * (PA + PB);
+};
+
+// Accumulator tile of one wavefront.
+template <int MI, int NI>
+struct Acc {
+    double4_t v[MI][NI];
+    __device__ void zero() {
+#pragma unroll
+        for (int a = 0; a < MI; a++)
+#pragma unroll
+            for (int b = 0; b < NI; b++) v[a][b] = double4_t{0., 0., 0., 0.};
+    }
+};
+
+// Runs the K loop.  `lds` must hold TileShape<BM,BN>::LDS_DOUBLES doubles.
+template <int BM, int BN, class LA, class LB>
+__device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, double *lds,
+                                     Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI> &acc) {
+    using S = TileShape<BM, BN>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    double *As[2] = {lds, lds + G_BK * S::PA};
+    double *Bs[2] = {lds + 2 * G_BK * S::PA, lds + 2 * G_BK * S::PA + G_BK * S::PB};
+    double ra[LA::N], rb[LB::N];
+    if (nk <= 0) return;
+    la.load(0, ra);
+    lb.load(0, rb);
+    la.store(ra, As[0], S::PA);
+    lb.store(rb, Bs[0], S::PB);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            la.load(kt + 1, ra);
+            lb.load(kt + 1, rb);
+        }
+        const double *A = As[cur] + wm * S::WM + (lane & 15);
+        const double *B = Bs[cur] + wn * S::WN + (lane & 15);
+#pragma unroll
+        for (int kq = 0; kq < G_BK / 4; kq++) {
+            const int kk = kq * 4 + (lane >> 4);
+            double a[S::MI], b[S::NI];
+#pragma unroll
+            for (int mi = 0; mi < S::MI; mi++) a[mi] = A[kk * S::PA + mi * 16];
+#pragma unroll
+            for (int ni = 0; ni < S::NI; ni++) b[ni] = B[kk * S::PB + ni * 16];
+#pragma unroll
+            for (int mi = 0; mi < S::MI; mi++)
+#pragma unroll
+                for (int ni = 0; ni < S::NI; ni++) acc.v[mi][ni] = mfma16(a[mi], b[ni], acc.v[mi][ni]);
+        }
+        if (more) {
+            la.store(ra, As[cur ^ 1], S::PA);
+            lb.store(rb, Bs[cur ^ 1], S::PB);
+        }
+        __syncthreads();
+    }
+}
+
+// Visits every accumulator element with its tile-local (row, col).
+template <int BM, int BN, class F>
+__device__ inline void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI> &acc, F &&f) {
+    using S = TileShape<BM, BN>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+#pragma unroll
+    for (int mi = 0; mi < S::MI; mi++)
+#pragma unroll
+        for (int ni = 0; ni < S::NI; ni++) {
+            const int c = wn * S::WN + ni * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = wm * S::WM + mi * 16 + (lane >> 4) + 4 * v;
+                f(r, c, acc.v[mi][ni][v]);
+            }
+        }
+}
+
+// ---------------------------------------------------------------- loaders
+// Row-major matrix M[row][k] (k contiguous) as the A or B operand: tile rows
+// r0 .. r0+R-1, K-step kt covers k = kt*BK .. +BK-1.  Out-of-range reads 0.
+template <int R>
+struct RowsLoader {
+    static constexpr int N = R * G_BK / G_THREADS;
+    static_assert(N >= 1 && (G_THREADS / (G_BK / N)) == R, "RowsLoader shape");
+    const double *base;   // &M[0][0]
+    long ld;              // row stride
+    int r0, nrows, k0, kmax;
+    __device__ void load(int kt, double (&r)[N]) const {
+        const int t = threadIdx.x;
+        const int row = r0 + t / (G_BK / N);
+        const int kb = k0 + kt * G_BK + (t % (G_BK / N)) * N;
+        const double *src = base + (long)row * ld + kb;
+#pragma unroll
+        for (int v = 0; v < N; v++) r[v] = (row < nrows && kb + v < kmax) ? src[v] : 0.;
+    }
+    __device__ void store(const double (&r)[N], double *lds, int pitch) const {
+        const int t = threadIdx.x;
+        const int row = t / (G_BK / N);
+        const int kb = (t % (G_BK / N)) * N;
+#pragma unroll
+        for (int v = 0; v < N; v++) lds[(kb + v) * pitch + row] = r[v];
+    }
+};
+
+// Row-major matrix M[k][col] (col contiguous) as the B operand: tile columns
+// c0 .. c0+C-1.
+template <int C>
+struct ColsLoader {
+    static constexpr int N = C * G_BK / G_THREADS;
+    const double *base;
+    long ld;
+    int c0, ncols, k0, kmax;
+    __device__ void load(int kt, double (&r)[N]) const {
+        const int t = threadIdx.x;
+        // each thread takes N consecutive columns of one k row
+        constexpr int TPR = C / N;              // threads per k row
+        const int kk = t / TPR;
+        const int cb = c0 + (t % TPR) * N;
+        const int k = k0 + kt * G_BK + kk;
+        const double *src = base + (long)k * ld + cb;
+#pragma unroll
+        for (int v = 0; v < N; v++) r[v] = (kk < G_BK && k < kmax && cb + v < ncols) ? src[v] : 0.;
+    }
+    __device__ void store(const double (&r)[N], double *lds, int pitch) const {
+        const int t = threadIdx.x;
+        constexpr int TPR = C / N;
+        const int kk = t / TPR;
+        const int cb = (t % TPR) * N;
+        if (kk < G_BK) {
+#pragma unroll
+            for (int v = 0; v < N; v++) lds[kk * pitch + cb + v] = r[v];
+        }
+    }
+};
+
+}  // namespace mdf
