@@ -31,6 +31,8 @@ _SIGS = {
     'mdf_unpack_pairs': (I, [P, I, L, P, L, I, P, L, P]),
     'mdf_build_pair_map': (I, [P, L, I, P, P]),
     'mdf_fill_zero': (I, [P, L, P]),
+    'mdf_int3c2e': (I, [P, P, P, P, P, P, L, P, L, P]),
+    'mdf_int2c2e': (I, [P, P, P, P, L, P]),
 }
 
 

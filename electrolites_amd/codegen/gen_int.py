@@ -1,0 +1,257 @@
+"""Generate the Rys-quadrature integral kernels (csrc/mdf_int_gen.hip).
+
+One kernel per angular class, fully unrolled for that class:
+
+* ``int3c_<la><lb><lc>``: three-centre ``(ij|P)`` over the shell pairs of class
+  ``(la, lb)`` and the auxiliary shells of angular momentum ``lc``.  One thread
+  per (shell pair, aux shell, aux spherical component m); ``m`` is the slowest
+  task index, so a wavefront normally shares it and the ``switch (m)`` below
+  does not diverge.  Each thread accumulates the Cartesian bra block of its m
+  over the primitive triples, transforms the bra to spherical, and writes its
+  values into the (naux, pair-chunk) output at pair ``i(i+1)/2 + j`` (i >= j).
+* ``int2c_<la><lc>``: the metric ``(P|Q)`` with the same machinery (bra = one
+  aux shell and a unit s function of exponent 0).
+
+Rys recurrences (u = t^2 a root, p/q bra/ket exponents, P/Q their centres):
+    B00 = u / (2(p+q)),  B10 = (1 - q u / (p+q)) / (2p),  B01 = (1 - p u/(p+q)) / (2q)
+    C00 = (P - A) - q u (P - Q) / (p+q),   C00' = (Q - C) + p u (P - Q) / (p+q)
+    G(e+1, f) = C00 G(e, f) + e B10 G(e-1, f) + f B00 G(e, f-1)
+    G(e, f+1) = C00' G(e, f) + f B01 G(e, f-1) + e B00 G(e-1, f)
+and the horizontal transfer I(a, b+1) = I(a+1, b) + (A - B) I(a, b).  The
+prefactor 2 pi^(5/2) / (p q sqrt(p+q)) exp(-ab/p |A-B|^2) times the weight is
+folded into G_z(0, 0).  (Standard Rys scheme; the class-per-kernel layout and
+the generator-driven unrolling follow electrolites/codegen/gen_j3c.py:158-372,
+which does the same for McMurchie-Davidson.)
+
+Run: ``python -m electrolites_amd.codegen.gen_int``.
+"""
+import os
+
+import numpy as np
+
+from electrolites_amd.gto.mole import cart2sph, cart_powers, ncart
+
+LMAX_AO = 2
+LMAX_AUX = 4
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, '..', 'csrc', 'mdf_int_gen.hip')
+THREADS = 128
+
+
+def fmt(c):
+    return repr(float(c))
+
+
+def gen_class(la, lb, lc, two_c):
+    name = f'int2c_{la}{lc}' if two_c else f'int3c_{la}{lb}{lc}'
+    lab = la + lb
+    nroots = (la + lb + lc) // 2 + 1
+    ca, cb, cc = cart_powers(la), cart_powers(lb), cart_powers(lc)
+    nca, ncb = len(ca), len(cb)
+    Ca, Cb, Cc = cart2sph(la), cart2sph(lb), cart2sph(lc)
+    L = []
+    w = L.append
+    if two_c:
+        w(f'__global__ __launch_bounds__({THREADS}) void {name}(const BasisDev aux, '
+          'const int *__restrict__ pl, int np, const int *__restrict__ ql, int nq, '
+          'double *__restrict__ out, long ldo) {')
+        w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
+        w(f'    if (tid >= (long)np * nq * {2 * lc + 1}) return;')
+        w('    const int ip = (int)(tid % np);')
+        w('    const long rest = tid / np;')
+        w('    const int kq = (int)(rest % nq), m = (int)(rest / nq);')
+        w('    const int ish = pl[ip], ksh = ql[kq];')
+        w('    const BasisDev &ao = aux;')
+        w('    const int jsh = ish;')
+    else:
+        w(f'__global__ __launch_bounds__({THREADS}) void {name}(const BasisDev ao, const BasisDev aux, '
+          'const int *__restrict__ pairs, int np, const int *__restrict__ kl, int nk, '
+          'double *__restrict__ out, long ldo, long p0) {')
+        w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
+        w(f'    if (tid >= (long)np * nk * {2 * lc + 1}) return;')
+        w('    const int ip = (int)(tid % np);')
+        w('    const long rest = tid / np;')
+        w('    const int kq = (int)(rest % nk), m = (int)(rest / nk);')
+        w('    const int ish = pairs[2 * ip], jsh = pairs[2 * ip + 1], ksh = kl[kq];')
+    w('    const double Ax = ao.xyz[3 * ish], Ay = ao.xyz[3 * ish + 1], Az = ao.xyz[3 * ish + 2];')
+    if not two_c:
+        w('    const double Bx = ao.xyz[3 * jsh], By = ao.xyz[3 * jsh + 1], Bz = ao.xyz[3 * jsh + 2];')
+        w('    const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;')
+        w('    const double rab2 = ABx * ABx + ABy * ABy + ABz * ABz;')
+    w('    const double Cx = aux.xyz[3 * ksh], Cy = aux.xyz[3 * ksh + 1], Cz = aux.xyz[3 * ksh + 2];')
+    w(f'    double acc[{nca * ncb}];')
+    w('#pragma unroll')
+    w(f'    for (int t = 0; t < {nca * ncb}; t++) acc[t] = 0.;')
+    w('    const int npa = ao.nprim[ish], oa = ao.poff[ish];')
+    if not two_c:
+        w('    const int npb = ao.nprim[jsh], ob = ao.poff[jsh];')
+    w('    const int npc = aux.nprim[ksh], oc = aux.poff[ksh];')
+    w('    for (int ia = 0; ia < npa; ia++) {')
+    w('        const double ea = ao.exps[oa + ia], da = ao.coefs[oa + ia];')
+    if two_c:
+        w('        {')
+        w('        const double p = ea, fab = da;')
+        w('        const double Px = Ax, Py = Ay, Pz = Az;')
+    else:
+        w('        for (int ib = 0; ib < npb; ib++) {')
+        w('            const double eb = ao.exps[ob + ib];')
+        w('            const double p = ea + eb, ip_ = 1.0 / p;')
+        w('            const double fab = da * ao.coefs[ob + ib] * exp(-ea * eb * ip_ * rab2);')
+        w('            if (fabs(fab) < PRIM_SCREEN) continue;')
+        w('            const double Px = (ea * Ax + eb * Bx) * ip_, Py = (ea * Ay + eb * By) * ip_, '
+          'Pz = (ea * Az + eb * Bz) * ip_;')
+    w('            const double PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az;')
+    w('            const double PCx = Px - Cx, PCy = Py - Cy, PCz = Pz - Cz;')
+    w('            const double rpc2 = PCx * PCx + PCy * PCy + PCz * PCz;')
+    w('            for (int ic = 0; ic < npc; ic++) {')
+    w('                const double q = aux.exps[oc + ic];')
+    w('                const double pq = p + q, ipq = 1.0 / pq;')
+    w('                const double x = p * q * ipq * rpc2;')
+    w('                const double pref = TWO_PI_2P5 / (p * q * sqrt(pq)) * fab * aux.coefs[oc + ic];')
+    w(f'                double ru[{nroots}], rw[{nroots}];')
+    w(f'                rys_roots<{nroots}>(x, ru, rw);')
+    w('#pragma unroll')
+    w(f'                for (int r = 0; r < {nroots}; r++) {{')
+    w('                    const double u = ru[r];')
+    w('                    const double B00 = 0.5 * u * ipq;')
+    if lab >= 2:
+        w('                    const double B10 = 0.5 / p - 0.5 * q * u * ipq / p;')
+    if lc >= 2:
+        w('                    const double B01 = 0.5 / q - 0.5 * p * u * ipq / q;')
+    w('                    const double qf = q * u * ipq, pf = p * u * ipq;')
+    for d in 'xyz':
+        w(f'                    const double C00{d} = PA{d} - qf * PC{d}, C0p{d} = pf * PC{d};')
+    # VRR
+    for d in 'xyz':
+        init = 'rw[r] * pref' if d == 'z' else '1.0'
+        w(f'                    const double g{d}_0_0 = {init};')
+        for e in range(0, lab + 1):
+            for f in range(0, lc + 1):
+                if e == 0 and f == 0:
+                    continue
+                if f == 0:
+                    terms = [f'C00{d} * g{d}_{e-1}_0']
+                    if e - 1 >= 1:
+                        terms.append(f'{e-1}.0 * B10 * g{d}_{e-2}_0')
+                else:
+                    terms = [f'C0p{d} * g{d}_{e}_{f-1}']
+                    if f - 1 >= 1:
+                        terms.append(f'{f-1}.0 * B01 * g{d}_{e}_{f-2}')
+                    if e >= 1:
+                        terms.append(f'{e}.0 * B00 * g{d}_{e-1}_{f-1}')
+                w(f'                    const double g{d}_{e}_{f} = ' + ' + '.join(terms) + ';')
+    # HRR: I{d}_{a}_{b}_{f}
+    needed = {}
+    for d in 'xyz':
+        for f in range(lc + 1):
+            for a in range(lab + 1):
+                w(f'                    const double I{d}_{a}_0_{f} = g{d}_{a}_{f};')
+            for b in range(1, lb + 1):
+                for a in range(0, lab - b + 1):
+                    w(f'                    const double I{d}_{a}_{b}_{f} = I{d}_{a+1}_{b-1}_{f} + '
+                      f'AB{d} * I{d}_{a}_{b-1}_{f};')
+    # products per m
+    w('                    switch (m) {')
+    for mi in range(2 * lc + 1):
+        w(f'                    case {mi}: {{')
+        sup = [(k, Cc[mi, k]) for k in range(len(cc)) if abs(Cc[mi, k]) > 1e-15]
+        for a_i, (ax, ay, az) in enumerate(ca):
+            for b_i, (bx, by, bz) in enumerate(cb):
+                terms = []
+                for k, coef in sup:
+                    kx, ky, kz = cc[k]
+                    terms.append(f'{fmt(coef)} * Ix_{ax}_{bx}_{kx} * Iy_{ay}_{by}_{ky} * Iz_{az}_{bz}_{kz}')
+                w(f'                        acc[{a_i * ncb + b_i}] += ' + ' + '.join(terms) + ';')
+        w('                        break;')
+        w('                    }')
+    w('                    }')
+    w('                }')          # roots
+    w('            }')              # ic
+    w('        }')                  # ib / block
+    w('    }')                      # ia
+    # bra to spherical and store
+    nsa, nsb = 2 * la + 1, 2 * lb + 1
+    if two_c:
+        w('    const int prow0 = aux.aooff[ish], qcol = aux.aooff[ksh] + m;')
+        for s in range(nsa):
+            terms = [f'{fmt(Ca[s, a])} * acc[{a}]' for a in range(nca) if abs(Ca[s, a]) > 1e-15]
+            w(f'    out[(long)(prow0 + {s}) * ldo + qcol] = ' + ' + '.join(terms) + ';')
+    else:
+        w('    const int i0 = ao.aooff[ish], j0 = ao.aooff[jsh];')
+        w('    double *orow = out + (long)(aux.aooff[ksh] + m) * ldo - p0;')
+        w('    const bool diag = ish == jsh;')
+        for s in range(nsa):
+            for t in range(nsb):
+                terms = []
+                for a in range(nca):
+                    if abs(Ca[s, a]) < 1e-15:
+                        continue
+                    for b in range(ncb):
+                        if abs(Cb[t, b]) < 1e-15:
+                            continue
+                        terms.append(f'{fmt(Ca[s, a] * Cb[t, b])} * acc[{a * ncb + b}]')
+                cond = f'if (!diag || {s} >= {t}) ' if la == lb else ''
+                w(f'    {cond}{{ const long i = i0 + {s}; orow[i * (i + 1) / 2 + j0 + {t}] = '
+                  + ' + '.join(terms) + '; }')
+    w('}')
+    return name, '\n'.join(L)
+
+
+def main():
+    parts = ['// Generated by electrolites_amd/codegen/gen_int.py; do not edit.',
+             '// Rys-quadrature int3c2e / int2c2e kernels, one per angular class.',
+             '#include "mdf_int.h"', '',
+             'namespace mdf {', '']
+    names3, names2 = {}, {}
+    for la in range(LMAX_AO + 1):
+        for lb in range(LMAX_AO + 1):
+            for lc in range(LMAX_AUX + 1):
+                n, src = gen_class(la, lb, lc, False)
+                names3[(la, lb, lc)] = n
+                parts.append(src)
+                parts.append('')
+    for la in range(LMAX_AUX + 1):
+        for lc in range(LMAX_AUX + 1):
+            n, src = gen_class(la, 0, lc, True)
+            names2[(la, lc)] = n
+            parts.append(src)
+            parts.append('')
+    # dispatch
+    parts.append('int launch_int3c(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, '
+                 'const int *pairs, int np, const int *kl, int nk, double *out, long ldo, long p0, '
+                 'hipStream_t st) {')
+    parts.append('    const long n = (long)np * nk * (2 * lc + 1);')
+    parts.append('    if (n == 0) return MDF_OK;')
+    parts.append(f'    const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})), b({THREADS});')
+    parts.append(f'    switch ((la * {LMAX_AO + 1} + lb) * {LMAX_AUX + 1} + lc) {{')
+    for (la, lb, lc), n in names3.items():
+        key = (la * (LMAX_AO + 1) + lb) * (LMAX_AUX + 1) + lc
+        parts.append(f'    case {key}: hipLaunchKernelGGL({n}, g, b, 0, st, ao, aux, pairs, np, kl, nk, out, ldo, p0); break;')
+    parts.append('    default: set_error("int3c2e: class (%d %d|%d) not generated", la, lb, lc); return MDF_EINVAL;')
+    parts.append('    }')
+    parts.append('    MDF_LAUNCHED("int3c");')
+    parts.append('    return MDF_OK;')
+    parts.append('}')
+    parts.append('')
+    parts.append('int launch_int2c(int la, int lc, const BasisDev &aux, const int *pl, int np, const int *ql, '
+                 'int nq, double *out, long ldo, hipStream_t st) {')
+    parts.append('    const long n = (long)np * nq * (2 * lc + 1);')
+    parts.append('    if (n == 0) return MDF_OK;')
+    parts.append(f'    const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})), b({THREADS});')
+    parts.append(f'    switch (la * {LMAX_AUX + 1} + lc) {{')
+    for (la, lc), n in names2.items():
+        parts.append(f'    case {la * (LMAX_AUX + 1) + lc}: hipLaunchKernelGGL({n}, g, b, 0, st, aux, pl, np, ql, nq, out, ldo); break;')
+    parts.append('    default: set_error("int2c2e: class (%d|%d) not generated", la, lc); return MDF_EINVAL;')
+    parts.append('    }')
+    parts.append('    MDF_LAUNCHED("int2c");')
+    parts.append('    return MDF_OK;')
+    parts.append('}')
+    parts.append('')
+    parts.append('}  // namespace mdf')
+    with open(OUT, 'w') as f:
+        f.write('\n'.join(parts) + '\n')
+    print('wrote', OUT, len(names3), '3c and', len(names2), '2c kernels')
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,25 @@
+// Shared declarations of the Rys integral kernels (hand-written entry points
+// in mdf_int.hip, generated kernels in mdf_int_gen.hip).
+#pragma once
+#include "mdf_common.h"
+#include "mdf_rys.h"
+
+namespace mdf {
+
+constexpr double TWO_PI_2P5 = 34.986836655249725;   // 2 pi^(5/2)
+constexpr double PRIM_SCREEN = 1e-20;               // |c_a c_b exp(-ab/p R^2)| cutoff
+constexpr int LMAX_AO = 2;
+constexpr int LMAX_AUX = 4;
+
+// Device view of one basis (all pointers are device memory).
+struct BasisDev {
+    const int *nprim, *poff, *aooff;
+    const double *xyz, *exps, *coefs;
+};
+
+int launch_int3c(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, const int *pairs, int np,
+                 const int *kl, int nk, double *out, long ldo, long p0, hipStream_t st);
+int launch_int2c(int la, int lc, const BasisDev &aux, const int *pl, int np, const int *ql, int nq,
+                 double *out, long ldo, hipStream_t st);
+
+}  // namespace mdf

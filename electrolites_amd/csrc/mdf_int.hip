@@ -1,0 +1,71 @@
+// C-ABI entry points of the Rys integral kernels.
+//
+// mdf_int3c2e fills one pair-chunk of the three-centre tensor in the layout
+// the metric solve consumes: out[P][p - p0], P over spherical aux functions,
+// p = i(i+1)/2 + j (i >= j) over the chunk's packed pairs.  It plays the part
+// of GPU4PySCF's int3c2e evaluator in the reference's cderi build
+// (electrolites/fastdfcderi.py:404-409, called at :604 and :667); the aux
+// functions come out spherical and contracted, so no separate cart->sph /
+// recontraction pass (electrolites/_dfaux.py:75-119) is needed.
+// mdf_int2c2e fills the metric (P|Q) (electrolites/_dfj2c.py:234-262).
+#include "mdf_int.h"
+#include "../../include/mi355df.h"
+
+using namespace mdf;
+
+static BasisDev to_dev(const mdf_basis *b) {
+    BasisDev d;
+    d.nprim = b->nprim;
+    d.poff = b->poff;
+    d.aooff = b->aooff;
+    d.xyz = b->xyz;
+    d.exps = b->exps;
+    d.coefs = b->coefs;
+    return d;
+}
+
+extern "C" int mdf_int3c2e(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
+                           const long *pair_class_off, const int *aux_sorted, const long *aux_l_off,
+                           long p0, double *out, long ldo, void *stream) {
+    MDF_CHECK_ARG(ao && aux && pair_class_off && aux_l_off && out, "mdf_int3c2e: null argument");
+    MDF_CHECK_ARG(ao->lmax <= LMAX_AO && aux->lmax <= LMAX_AUX,
+                  "mdf_int3c2e: angular momentum beyond the generated classes (ao %d > %d or aux %d > %d)",
+                  ao->lmax, LMAX_AO, aux->lmax, LMAX_AUX);
+    hipStream_t st = as_stream(stream);
+    const BasisDev a = to_dev(ao), x = to_dev(aux);
+    for (int la = 0; la <= LMAX_AO; la++)
+        for (int lb = 0; lb <= LMAX_AO; lb++) {
+            const int cls = la * (LMAX_AO + 1) + lb;
+            const long q0 = pair_class_off[cls], q1 = pair_class_off[cls + 1];
+            if (q1 <= q0) continue;
+            MDF_CHECK_ARG(q1 - q0 < (1L << 31), "mdf_int3c2e: too many pairs in one class");
+            for (int lc = 0; lc <= LMAX_AUX; lc++) {
+                const long k0 = aux_l_off[lc], k1 = aux_l_off[lc + 1];
+                if (k1 <= k0) continue;
+                int rc = launch_int3c(la, lb, lc, a, x, pairs + 2 * q0, (int)(q1 - q0), aux_sorted + k0,
+                                      (int)(k1 - k0), out, ldo, p0, st);
+                if (rc) return rc;
+            }
+        }
+    return MDF_OK;
+}
+
+extern "C" int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_off, double *out,
+                           long ldo, void *stream) {
+    MDF_CHECK_ARG(aux && aux_l_off && out, "mdf_int2c2e: null argument");
+    MDF_CHECK_ARG(aux->lmax <= LMAX_AUX, "mdf_int2c2e: aux angular momentum %d > %d", aux->lmax, LMAX_AUX);
+    hipStream_t st = as_stream(stream);
+    const BasisDev x = to_dev(aux);
+    for (int la = 0; la <= LMAX_AUX; la++) {
+        const long p0 = aux_l_off[la], p1 = aux_l_off[la + 1];
+        if (p1 <= p0) continue;
+        for (int lc = 0; lc <= LMAX_AUX; lc++) {
+            const long k0 = aux_l_off[lc], k1 = aux_l_off[lc + 1];
+            if (k1 <= k0) continue;
+            int rc = launch_int2c(la, lc, x, aux_sorted + p0, (int)(p1 - p0), aux_sorted + k0, (int)(k1 - k0),
+                                  out, ldo, st);
+            if (rc) return rc;
+        }
+    }
+    return MDF_OK;
+}

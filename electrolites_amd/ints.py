@@ -1,0 +1,130 @@
+"""Device-side basis sets and the Rys integral entry points.
+
+``DeviceBasis`` uploads a :class:`~electrolites_amd.gto.ShellTable` once and
+holds the ``mdf_basis`` descriptor libmi355df reads.  ``int3c2e_chunk`` and
+``int2c2e`` call the generated Rys kernels; shell-pair screening and the
+class-sorted task lists are prepared here on the host.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _dev
+from . import _lib
+
+LMAX_AO = 2
+LMAX_AUX = 4
+#: a shell pair is skipped when every primitive pair has
+#: |c_a c_b| exp(-ab/(a+b) R^2) (pi/(a+b))^(3/2) below this
+PAIR_SCREEN = 1e-18
+
+
+class MdfBasis(ctypes.Structure):
+    _fields_ = [('nbas', ctypes.c_int), ('lmax', ctypes.c_int),
+                ('nprim', ctypes.c_void_p), ('poff', ctypes.c_void_p),
+                ('aooff', ctypes.c_void_p), ('xyz', ctypes.c_void_p),
+                ('exps', ctypes.c_void_p), ('coefs', ctypes.c_void_p)]
+
+
+class DeviceBasis:
+    """A shell table resident on the current device."""
+
+    def __init__(self, shells):
+        t = _dev.torch()
+        self.shells = shells
+        self.nbas = shells.nbas
+        self.nao = shells.nao
+        self.l = np.asarray(shells.shl_l)
+        i32 = lambda a: _dev.to_device(np.asarray(a, dtype=np.int32), dtype=t.int32)
+        f64 = lambda a: _dev.to_device(np.asarray(a, dtype=np.float64))
+        self._nprim = i32(shells.shl_nprim)
+        self._poff = i32(shells.shl_prim_off)
+        self._aooff = i32(shells.shl_ao_off)
+        self._xyz = f64(shells.shl_xyz.reshape(-1))
+        self._exps = f64(shells.exps)
+        self._coefs = f64(shells.coefs)
+        self.desc = MdfBasis(self.nbas, int(self.l.max()) if self.nbas else 0,
+                             self._nprim.data_ptr(), self._poff.data_ptr(),
+                             self._aooff.data_ptr(), self._xyz.data_ptr(),
+                             self._exps.data_ptr(), self._coefs.data_ptr())
+        # shells sorted by l (for the aux role)
+        order = np.argsort(self.l, kind='stable').astype(np.int32)
+        self.sorted = i32(order)
+        cnt = np.bincount(self.l, minlength=LMAX_AUX + 1)[:LMAX_AUX + 1]
+        self.l_off = np.zeros(LMAX_AUX + 2, dtype=np.int64)
+        self.l_off[1:] = np.cumsum(cnt)
+
+
+def pair_bound(shells):
+    """Per shell pair (I >= J): max over primitive pairs of
+    |c_a c_b| exp(-ab/(a+b) |A-B|^2) (pi/(a+b))^(3/2)."""
+    n = shells.nbas
+    xyz = shells.shl_xyz
+    out = np.zeros((n, n))
+    for I in range(n):
+        ea = shells.exps[shells.shl_prim_off[I]:shells.shl_prim_off[I] + shells.shl_nprim[I]]
+        ca = shells.coefs[shells.shl_prim_off[I]:shells.shl_prim_off[I] + shells.shl_nprim[I]]
+        r2 = np.sum((xyz[:I + 1] - xyz[I]) ** 2, axis=1)
+        for J in range(I + 1):
+            o = shells.shl_prim_off[J]
+            eb = shells.exps[o:o + shells.shl_nprim[J]]
+            cb = shells.coefs[o:o + shells.shl_nprim[J]]
+            p = ea[:, None] + eb[None, :]
+            v = np.abs(ca[:, None] * cb[None, :]) * np.exp(-ea[:, None] * eb[None, :] / p * r2[J]) \
+                * (np.pi / p) ** 1.5
+            out[I, J] = v.max()
+    return out
+
+
+class PairTasks:
+    """Screened shell pairs of the row-shell range [I0, I1), class-sorted."""
+
+    def __init__(self, shells, I0, I1, bound=None, thr=PAIR_SCREEN):
+        l = np.asarray(shells.shl_l)
+        pairs = []
+        for I in range(I0, I1):
+            for J in range(I + 1):
+                if bound is not None and bound[I, J] < thr:
+                    continue
+                pairs.append((I, J))
+        pairs = np.array(pairs, dtype=np.int32).reshape(-1, 2)
+        cls = l[pairs[:, 0]] * (LMAX_AO + 1) + l[pairs[:, 1]] if len(pairs) else np.zeros(0, int)
+        order = np.argsort(cls, kind='stable')
+        self.pairs = pairs[order]
+        cnt = np.bincount(cls, minlength=(LMAX_AO + 1) ** 2)
+        self.class_off = np.zeros((LMAX_AO + 1) ** 2 + 1, dtype=np.int64)
+        self.class_off[1:] = np.cumsum(cnt)
+        ao_loc = shells.ao_loc
+        i0, i1 = int(ao_loc[I0]), int(ao_loc[I1])
+        self.p0 = i0 * (i0 + 1) // 2
+        self.p1 = i1 * (i1 + 1) // 2
+        self.npairs_kept = len(pairs)
+        t = _dev.torch()
+        self.dev = _dev.to_device(self.pairs.reshape(-1), dtype=t.int32) if len(pairs) else \
+            t.zeros(2, dtype=t.int32, device=_dev.device())
+
+
+def _lp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+
+
+def int3c2e_chunk(ao, aux, tasks, out, ldo):
+    """Fill ``out[P, p - p0]`` for the chunk's pairs (out pre-zeroed)."""
+    if ao.desc.lmax > LMAX_AO or aux.desc.lmax > LMAX_AUX:
+        raise ValueError(f'angular momentum beyond the generated classes '
+                         f'(orbital {ao.desc.lmax} > {LMAX_AO} or auxiliary '
+                         f'{aux.desc.lmax} > {LMAX_AUX})')
+    _lib.call('mdf_int3c2e', ctypes.byref(ao.desc), ctypes.byref(aux.desc),
+              _dev.ptr(tasks.dev), _lp(tasks.class_off), _dev.ptr(aux.sorted),
+              _lp(aux.l_off), tasks.p0, _dev.ptr(out), ldo, _dev.stream_ptr())
+
+
+def int2c2e(aux, out=None):
+    """The metric ``(P|Q)`` as a ``(naux, naux)`` device tensor."""
+    t = _dev.torch()
+    n = aux.nao
+    if out is None:
+        out = t.empty((n, n), dtype=t.float64, device=_dev.device())
+    _lib.call('mdf_int2c2e', ctypes.byref(aux.desc), _dev.ptr(aux.sorted),
+              _lp(aux.l_off), _dev.ptr(out), out.stride(0), _dev.stream_ptr())
+    return out

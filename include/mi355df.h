@@ -82,6 +82,44 @@ int mdf_unpack_pairs(const double *vj_pairs, int n_dm, long ldv, const int *pair
 int mdf_build_pair_map(const int *pair_addr, long npairs, int nao, int *map, void *stream);
 int mdf_fill_zero(double *x, long n, void *stream);
 
+/* ---- Gaussian integrals (Rys quadrature, one kernel per angular class) --
+ * A basis is a set of contracted shells; all arrays are DEVICE memory:
+ *   nprim[s], poff[s] (offset into exps/coefs), aooff[s] (first spherical
+ *   function), xyz[3 s] (bohr), exps[k], coefs[k] (normalised: gto_norm and
+ *   the contraction normalisation folded in, PySCF convention).
+ * Spherical functions follow PySCF (m = -l..l; p as x, y, z).
+ */
+typedef struct {
+    int nbas;
+    int lmax;                 /* largest l in the basis */
+    const int *nprim;
+    const int *poff;
+    const int *aooff;
+    const double *xyz;
+    const double *exps;
+    const double *coefs;
+} mdf_basis;
+
+/* Three-centre (ij|P) into one pair chunk: out[P * ldo + (p - p0)] with
+ * p = i(i+1)/2 + j, i >= j.  Replaces GPU4PySCF's int3c2e evaluator in the
+ * reference's cderi build (electrolites/fastdfcderi.py:404-409, :604, :667).
+ *   pairs          device int32 (ish, jsh) with ish >= jsh, sorted by class
+ *                  (l_i, l_j); pair_class_off (HOST, 10 entries) bounds the
+ *                  class la*3 + lb.  Every function of the listed pairs must
+ *                  fall in [p0, p0 + ldo); unlisted (screened) pairs are left
+ *                  untouched (zero-fill out first).
+ *   aux_sorted     device int32 aux shells sorted by l; aux_l_off (HOST,
+ *                  6 entries) bounds l = 0..4.
+ * Supported: orbital l <= 2, auxiliary l <= 4. */
+int mdf_int3c2e(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
+                const long *pair_class_off, const int *aux_sorted, const long *aux_l_off,
+                long p0, double *out, long ldo, void *stream);
+
+/* The Coulomb metric (P|Q), all naux x naux entries, out[P * ldo + Q].
+ * Replaces electrolites/_dfj2c.py:234-262 (int2c2e). */
+int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_off,
+                double *out, long ldo, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
