@@ -33,6 +33,12 @@ _SIGS = {
     'mdf_fill_zero': (I, [P, L, P]),
     'mdf_int3c2e': (I, [P, P, P, P, P, P, L, P, L, P]),
     'mdf_int2c2e': (I, [P, P, P, P, L, P]),
+    'mdf_gemm': (I, [I, I, I, L, L, L, D, P, L, P, L, D, P, L, P]),
+    'mdf_potrf_workspace': (SZ, [L]),
+    'mdf_potrf': (I, [P, L, L, P, P, SZ, P]),
+    'mdf_trtri_workspace': (SZ, [L]),
+    'mdf_trtri': (I, [P, L, L, P, P, L, P, SZ, P]),
+    'mdf_trmm': (I, [P, L, L, P, L, L, P, L, P]),
 }
 
 

@@ -1,0 +1,225 @@
+// FP64 dense linear algebra of the cderi build on the MFMA tile engine.
+//
+// The reference decomposes the metric and applies its inverse factor with
+// cuSOLVER/cuBLAS (electrolites/fastdfcderi.py:173-215 _decompose_j2c,
+// :479-481 explicit L^-1, :237-261 _trmm_right_lowerT).  Here:
+//   mdf_gemm        C = alpha op(A) op(B) + beta C (row-major; B may be
+//                   transposed; A may be lower triangular; C may be
+//                   restricted to its lower 128x128 tiles)
+//   mdf_potrf       blocked right-looking Cholesky A = L L^T (lower, in
+//                   place), 128-wide panels: one-workgroup diagonal block
+//                   factorisation + inverse, panel = A21 inv(L11)^T, trailing
+//                   SYRK on the engine.  A non-positive pivot sets *info.
+//   mdf_trtri       X = L^-1 by block rows from the diagonal inverses
+//   mdf_trmm        C = L^-1 B with the triangular A bound folded into the
+//                   K loop (the cderi = L^-1 (P|pq) product).
+#include <cmath>
+
+#include "mdf_common.h"
+#include "mdf_gemm.h"
+#include "../../include/mi355df.h"
+
+using namespace mdf;
+
+namespace {
+
+constexpr int LB = 128;   // panel / tile width
+
+template <int BN, bool BT, bool TRI_A, bool LOWER_C>
+__global__ __launch_bounds__(G_THREADS, 2) void gemm_f64_kernel(
+    int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
+    const double *__restrict__ B, long ldb, double beta, double *C, long ldc) {
+    using S = TileShape<128, BN>;
+    __shared__ double lds[S::LDS_DOUBLES];
+    int ti, tj;
+    if (LOWER_C) {
+        const int x = blockIdx.x;
+        ti = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+        while (ti * (ti + 1) / 2 > x) ti--;
+        while ((ti + 1) * (ti + 2) / 2 <= x) ti++;
+        tj = x - ti * (ti + 1) / 2;
+    } else {
+        ti = blockIdx.y;
+        tj = blockIdx.x;
+    }
+    const int m0 = ti * 128, n0 = tj * BN;
+    const int kmax = TRI_A ? min(K, m0 + 128) : K;
+    RowsLoader<128> la{A, lda, m0, M, 0, kmax};
+    Acc<S::MI, S::NI> acc;
+    acc.zero();
+    const int nk = (kmax + G_BK - 1) / G_BK;
+    if (BT) {
+        RowsLoader<BN> lb{B, ldb, n0, N, 0, kmax};
+        gemm_mainloop<128, BN>(la, lb, nk, lds, acc);
+    } else {
+        ColsLoader<BN> lb{B, ldb, n0, N, 0, kmax};
+        gemm_mainloop<128, BN>(la, lb, nk, lds, acc);
+    }
+    gemm_epilogue<128, BN>(acc, [&](int r, int c, double v) {
+        const int m = m0 + r, n = n0 + c;
+        if (m < M && n < N) {
+            double *dst = C + (long)m * ldc + n;
+            *dst = beta == 0. ? alpha * v : alpha * v + beta * *dst;
+        }
+    });
+}
+
+template <int BN, bool BT, bool TRI_A, bool LOWER_C>
+int launch_gemm(int M, int N, int K, double alpha, const double *A, long lda, const double *B, long ldb,
+                double beta, double *C, long ldc, hipStream_t st) {
+    if (M <= 0 || N <= 0) return MDF_OK;
+    dim3 g;
+    if (LOWER_C) {
+        const int t = (M + 127) / 128;
+        g = dim3((unsigned)(t * (t + 1) / 2));
+    } else {
+        g = dim3((unsigned)((N + BN - 1) / BN), (unsigned)((M + 127) / 128));
+    }
+    hipLaunchKernelGGL((gemm_f64_kernel<BN, BT, TRI_A, LOWER_C>), g, dim3(G_THREADS), 0, st, M, N, K, alpha,
+                       A, lda, B, ldb, beta, C, ldc);
+    MDF_LAUNCHED("gemm_f64_kernel");
+    return MDF_OK;
+}
+
+// Cholesky of one diagonal block (n <= 128, lower, in place) by one
+// workgroup, then its inverse into D (ldd = 128).  A non-positive pivot
+// records (global index + 1) in *info and continues with pivot 1.
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, long lda, int n, int k0, double *D,
+                                                         int *info) {
+    const int t = threadIdx.x;
+    for (int k = 0; k < n; k++) {
+        __syncthreads();
+        const double d = A[(long)k * lda + k];
+        const bool bad = !(d > 0.);
+        const double piv = bad ? 1. : sqrt(d);
+        __syncthreads();
+        if (t == 0) {
+            A[(long)k * lda + k] = piv;
+            if (bad && info) atomicCAS(info, 0, k0 + k + 1);
+        }
+        for (int i = k + 1 + t; i < n; i += 256) A[(long)i * lda + k] /= piv;
+        __syncthreads();
+        const int m = n - k - 1;
+        for (long e = t; e < (long)m * m; e += 256) {
+            const int i = k + 1 + (int)(e / m), j = k + 1 + (int)(e % m);
+            if (j <= i) A[(long)i * lda + j] -= A[(long)i * lda + k] * A[(long)j * lda + k];
+        }
+    }
+    __syncthreads();
+    // zero the strict upper triangle of the block, so the factor is clean
+    for (long e = t; e < (long)n * n; e += 256) {
+        const int i = (int)(e / n), j = (int)(e % n);
+        if (j > i) A[(long)i * lda + j] = 0.;
+    }
+    // inverse, one column per thread: X[j][j] = 1/L[j][j], forward substitution
+    for (int j = t; j < LB; j += 256) {
+        for (int i = 0; i < LB; i++) {
+            double x = 0.;
+            if (j < n && i < n && i >= j) {
+                double s = i == j ? 1. : 0.;
+                for (int k = j; k < i; k++) s -= A[(long)i * lda + k] * D[(long)k * LB + j];
+                x = s / A[(long)i * lda + i];
+            }
+            D[(long)i * LB + j] = x;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int mdf_gemm(int transb, int tri_a, int lower_c, long M, long N, long K, double alpha,
+                        const double *A, long lda, const double *B, long ldb, double beta, double *C, long ldc,
+                        void *stream) {
+    MDF_CHECK_ARG(M >= 0 && N >= 0 && K >= 0 && M < (1L << 31) && N < (1L << 31) && K < (1L << 31),
+                  "mdf_gemm: bad sizes");
+    MDF_CHECK_ARG(A && B && C, "mdf_gemm: null pointer");
+    MDF_CHECK_ARG(!lower_c || M == N, "mdf_gemm: lower_c needs a square C");
+    MDF_CHECK_ARG(lda >= K && ldc >= N && ldb >= (transb ? K : N), "mdf_gemm: leading dimension too small");
+    hipStream_t st = as_stream(stream);
+    const int m = (int)M, n = (int)N, k = (int)K;
+    if (lower_c) {
+        if (transb) return tri_a ? launch_gemm<128, true, true, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st)
+                                 : launch_gemm<128, true, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+        return tri_a ? launch_gemm<128, false, true, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st)
+                     : launch_gemm<128, false, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    }
+    if (transb) return tri_a ? launch_gemm<128, true, true, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st)
+                             : launch_gemm<128, true, false, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    return tri_a ? launch_gemm<128, false, true, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st)
+                 : launch_gemm<128, false, false, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+}
+
+// workspace: the diagonal-block inverses D (nblocks x 128 x 128, kept for
+// mdf_trtri) followed by an n x 128 panel buffer
+extern "C" size_t mdf_potrf_workspace(long n) {
+    return ((size_t)((n + LB - 1) / LB) * LB * LB + (size_t)n * LB) * sizeof(double) + 256;
+}
+
+extern "C" int mdf_potrf(double *A, long n, long lda, int *info, void *work, size_t work_bytes, void *stream) {
+    MDF_CHECK_ARG(A && n >= 0 && lda >= n && n < (1L << 31), "mdf_potrf: bad arguments");
+    MDF_CHECK_ARG(work && work_bytes >= mdf_potrf_workspace(n), "mdf_potrf: workspace too small");
+    hipStream_t st = as_stream(stream);
+    double *D = (double *)work;
+    double *P = D + (size_t)((n + LB - 1) / LB) * LB * LB;
+    for (long k0 = 0; k0 < n; k0 += LB) {
+        const int kb = (int)(n - k0 < LB ? n - k0 : LB);
+        double *Akk = A + k0 * lda + k0;
+        double *Dk = D + (k0 / LB) * LB * LB;
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, Akk, lda, kb, (int)k0, Dk, info);
+        MDF_LAUNCHED("potrf_diag_kernel");
+        const long m2 = n - k0 - kb;
+        if (m2 <= 0) break;
+        double *A21 = A + (k0 + kb) * lda + k0;
+        // A21 <- A21 inv(L11)^T, through the panel buffer
+        int rc = launch_gemm<128, true, false, false>((int)m2, kb, kb, 1., A21, lda, Dk, LB, 0., P, LB, st);
+        if (rc) return rc;
+        MDF_HIP(hipMemcpy2DAsync(A21, lda * sizeof(double), P, LB * sizeof(double), kb * sizeof(double), m2,
+                                 hipMemcpyDeviceToDevice, st));
+        // A22 <- A22 - A21 A21^T (lower tiles)
+        double *A22 = A + (k0 + kb) * lda + k0 + kb;
+        rc = launch_gemm<128, true, false, true>((int)m2, (int)m2, kb, -1., A21, lda, A21, lda, 1., A22, lda, st);
+        if (rc) return rc;
+    }
+    return MDF_OK;
+}
+
+extern "C" size_t mdf_trtri_workspace(long n) { return (size_t)LB * n * sizeof(double) + 256; }
+
+// X = L^-1 for the lower factor L left by mdf_potrf; D holds its diagonal
+// block inverses (mdf_potrf's workspace).  X must not alias L.
+extern "C" int mdf_trtri(const double *L, long n, long ldl, const double *D, double *X, long ldx, void *work,
+                         size_t work_bytes, void *stream) {
+    MDF_CHECK_ARG(L && D && X && n >= 0 && ldl >= n && ldx >= n, "mdf_trtri: bad arguments");
+    MDF_CHECK_ARG(work && work_bytes >= mdf_trtri_workspace(n), "mdf_trtri: workspace too small");
+    hipStream_t st = as_stream(stream);
+    double *T = (double *)work;      // LB x n
+    // zero X, then place the diagonal inverses
+    {
+        const long tot = n * ldx;
+        MDF_HIP(hipMemsetAsync(X, 0, tot * sizeof(double), st));
+    }
+    for (long b0 = 0; b0 < n; b0 += LB) {
+        const int kb = (int)(n - b0 < LB ? n - b0 : LB);
+        const double *Db = D + (b0 / LB) * LB * LB;
+        MDF_HIP(hipMemcpy2DAsync(X + b0 * ldx + b0, ldx * sizeof(double), Db, LB * sizeof(double),
+                                 kb * sizeof(double), kb, hipMemcpyDeviceToDevice, st));
+        if (b0 == 0) continue;
+        // T = L[b, 0:b0] X[0:b0, 0:b0];  X[b, 0:b0] = -D_b T
+        int rc = launch_gemm<128, false, false, false>(kb, (int)b0, (int)b0, 1., L + b0 * ldl, ldl, X, ldx, 0.,
+                                                       T, b0, st);
+        if (rc) return rc;
+        rc = launch_gemm<128, false, false, false>(kb, (int)b0, kb, -1., Db, LB, T, b0, 0., X + b0 * ldx, ldx, st);
+        if (rc) return rc;
+    }
+    return MDF_OK;
+}
+
+// C[0:n, 0:ncols] = Linv[0:n, 0:n] B[0:n, 0:ncols], Linv lower triangular.
+extern "C" int mdf_trmm(const double *Linv, long n, long ldl, const double *B, long ldb, long ncols, double *C,
+                        long ldc, void *stream) {
+    MDF_CHECK_ARG(Linv && B && C && n >= 0 && ncols >= 0 && ldl >= n && ldb >= ncols && ldc >= ncols,
+                  "mdf_trmm: bad arguments");
+    MDF_CHECK_ARG(n < (1L << 31) && ncols < (1L << 31), "mdf_trmm: sizes beyond int range");
+    return launch_gemm<128, false, true, false>((int)n, (int)ncols, (int)n, 1., Linv, ldl, B, ldb, 0., C, ldc,
+                                                as_stream(stream));
+}

@@ -1,0 +1,148 @@
+"""The cderi build on MI355X: the drop-in for ``df._cholesky_eri``.
+
+Mirrors electrolites/fastdfcderi.py ``_cholesky_eri`` (:352-693) and
+``_decompose_j2c`` (:173-215) with every arithmetic step a libmi355df kernel:
+
+1. ``j2c = (P|Q)``                          mdf_int2c2e   (Rys, per class)
+2. ``j2c = L L^T``                           mdf_potrf     (blocked, MFMA)
+   a non-positive pivot switches to the eigen filter of :203-210
+   (``v / sqrt(w)`` for ``w > LINEAR_DEP_THR``; the aux count shrinks)
+3. ``Linv = L^-1``                           mdf_trtri     (:479-481)
+4. per chunk of orbital row shells:
+   ``B = (pq|P)`` for the chunk's pairs      mdf_int3c2e   (:604, :667)
+   ``cderi[:, chunk] = Linv B``              mdf_trmm      (:237-261)
+
+The result is ``(naux, npairs)`` FP64, pair-contiguous, over the full packed
+lower triangle (``p = i(i+1)/2 + j``), with ``cderi_idx = (pair_addr = i*nao + j
+(int32), diag_idx)`` as the reference returns (:446-449, :693).  Shell pairs
+whose primitive overlap bound is below ``ints.PAIR_SCREEN`` are not computed
+and hold zeros.
+"""
+import os
+
+import numpy as np
+
+from . import _dev
+from . import _lib
+from . import ints
+
+STATS = {'builds': 0, 'eigen_fallback': 0, 'chunks': 0, 'pairs_screened': 0,
+         'pairs_kept': 0}
+LINEAR_DEP_THR = 1e-9
+#: device bytes for one chunk of three-centre integrals
+CHUNK_BYTES = int(float(os.environ.get('FASTDFCDERI_CHUNK', 4 << 30)))
+
+
+def stats():
+    return dict(STATS)
+
+
+def _basis(mol, which):
+    key = f'_mdf_dev_{which}'
+    b = getattr(mol, key, None)
+    if b is None:
+        b = ints.DeviceBasis(mol.ao if which == 'ao' else mol.aux)
+        setattr(mol, key, b)
+    return b
+
+
+def decompose_j2c(aux):
+    """``(Linv, None)`` from the Cholesky route, or ``(None, coef)`` with
+    ``coef = (v / sqrt(w))^T`` from the eigen filter (non-PD metric)."""
+    t = _dev.torch()
+    n = aux.nao
+    j2c = ints.int2c2e(aux)
+    info = t.zeros(1, dtype=t.int32, device=_dev.device())
+    work = t.empty(_lib.query('mdf_potrf_workspace', n), dtype=t.uint8, device=_dev.device())
+    low = j2c.clone()
+    _lib.call('mdf_potrf', _dev.ptr(low), n, n, _dev.ptr(info), _dev.ptr(work),
+              work.numel(), _dev.stream_ptr())
+    if int(info.item()) == 0:
+        linv = t.empty((n, n), dtype=t.float64, device=_dev.device())
+        w2 = t.empty(_lib.query('mdf_trtri_workspace', n), dtype=t.uint8, device=_dev.device())
+        _lib.call('mdf_trtri', _dev.ptr(low), n, n, _dev.ptr(work), _dev.ptr(linv), n,
+                  _dev.ptr(w2), w2.numel(), _dev.stream_ptr())
+        return linv, None
+    STATS['eigen_fallback'] += 1
+    w, v = t.linalg.eigh(j2c)
+    keep = w > LINEAR_DEP_THR
+    coef = (v[:, keep] / t.sqrt(w[keep])).T.contiguous()
+    return None, coef
+
+
+def _chunks(shells, naux, budget):
+    """Row-shell ranges [I0, I1) whose packed pair count fits the budget."""
+    ao_loc = shells.ao_loc
+    out = []
+    I0 = 0
+    n = shells.nbas
+    while I0 < n:
+        I1 = I0 + 1
+        while I1 < n:
+            i0, i1 = int(ao_loc[I0]), int(ao_loc[I1 + 1])
+            if (i1 * (i1 + 1) // 2 - i0 * (i0 + 1) // 2) * naux * 8 > budget:
+                break
+            I1 += 1
+        out.append((I0, I1))
+        I0 = I1
+    return out
+
+
+def cholesky_eri(mol, chunk_bytes=None):
+    """Build the cderi of ``mol`` (orbital basis ``mol.ao``, fitting basis
+    ``mol.aux``) on the current device.  Returns ``(cderi, (pair_addr,
+    diag_idx))``."""
+    _dev.require_gpu()
+    t = _dev.torch()
+    ao, aux = _basis(mol, 'ao'), _basis(mol, 'aux')
+    nao, naux = ao.nao, aux.nao
+    npairs = nao * (nao + 1) // 2
+    linv, coef = decompose_j2c(aux)
+    nrow = naux if coef is None else coef.shape[0]
+    cderi = t.empty((nrow, npairs), dtype=t.float64, device=_dev.device())
+    bound = getattr(mol, '_mdf_pair_bound', None)
+    if bound is None:
+        bound = ints.pair_bound(mol.ao)
+        mol._mdf_pair_bound = bound
+    budget = chunk_bytes or CHUNK_BYTES
+    chunks = _chunks(mol.ao, naux, budget)
+    widest = max(int(mol.ao.ao_loc[I1]) * (int(mol.ao.ao_loc[I1]) + 1) // 2 -
+                 int(mol.ao.ao_loc[I0]) * (int(mol.ao.ao_loc[I0]) + 1) // 2
+                 for I0, I1 in chunks)
+    buf = t.empty(naux * widest, dtype=t.float64, device=_dev.device())
+    for I0, I1 in chunks:
+        tasks = ints.PairTasks(mol.ao, I0, I1, bound)
+        STATS['pairs_kept'] += tasks.npairs_kept
+        STATS['pairs_screened'] += (I1 - I0) * (I0 + I1 + 1) // 2 - tasks.npairs_kept
+        width = tasks.p1 - tasks.p0
+        blk = buf[:naux * width]
+        _lib.call('mdf_fill_zero', _dev.ptr(blk), naux * width, _dev.stream_ptr())
+        ints.int3c2e_chunk(ao, aux, tasks, blk, width)
+        dst = cderi[:, tasks.p0:]
+        if coef is None:
+            _lib.call('mdf_trmm', _dev.ptr(linv), naux, naux, _dev.ptr(blk), width, width,
+                      _dev.ptr(dst), npairs, _dev.stream_ptr())
+        else:
+            _lib.call('mdf_gemm', 0, 0, 0, nrow, width, naux, 1., _dev.ptr(coef), naux,
+                      _dev.ptr(blk), width, 0., _dev.ptr(dst), npairs, _dev.stream_ptr())
+        STATS['chunks'] += 1
+    STATS['builds'] += 1
+    rows, cols = np.tril_indices(nao)
+    pair_addr = (rows * nao + cols).astype(np.int32)
+    diag = np.where(rows == cols)[0]
+    return cderi, (pair_addr, diag)
+
+
+def _cholesky_eri(intopt, omega=None, use_gpu_memory=None):
+    """``df._cholesky_eri``'s signature.  ``intopt`` is this package's
+    :class:`~electrolites_amd.df.DF` (or anything with a ``mol`` carrying
+    ``ao``/``aux`` shell tables).  Range separation is not covered."""
+    if omega not in (None, 0, 0.):
+        raise NotImplementedError('range-separated cderi (omega != 0) is not '
+                                  'covered by the MI355X build')
+    if use_gpu_memory is False:
+        raise NotImplementedError('host-resident cderi is not covered: the '
+                                  'tensor stays in HBM')
+    mol = getattr(intopt, 'mol', intopt)
+    cderi, idx = cholesky_eri(mol)
+    return [cderi], idx

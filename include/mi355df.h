@@ -120,6 +120,31 @@ int mdf_int3c2e(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
 int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_off,
                 double *out, long ldo, void *stream);
 
+/* ---- FP64 dense linear algebra of the metric (MFMA tile engine) --------
+ * Replace the cuSOLVER/cuBLAS calls of electrolites/fastdfcderi.py:
+ * potrf (:196), explicit L^-1 by solve_triangular (:479-481) and the
+ * triangular multiply of _trmm_right_lowerT (:237-261).  Row-major.
+ * mdf_gemm: C = alpha A op(B) + beta C; transb: B is (N, K); tri_a: A is
+ *   lower triangular (K loop bounded per row tile); lower_c: only the lower
+ *   128x128 tiles of a square C are written.
+ * mdf_potrf: A = L L^T in place (lower; the strict upper part of each
+ *   diagonal 128-block is zeroed).  *info (device int, zero it first) gets
+ *   the 1-based index of the first non-positive pivot.  The workspace keeps
+ *   the diagonal-block inverses that mdf_trtri reads.
+ * mdf_trtri: X = L^-1 (X must not alias L), D = mdf_potrf's workspace.
+ * mdf_trmm: C = Linv B for lower-triangular Linv (n x n), B (n x ncols).
+ */
+int mdf_gemm(int transb, int tri_a, int lower_c, long M, long N, long K, double alpha,
+             const double *A, long lda, const double *B, long ldb, double beta, double *C, long ldc,
+             void *stream);
+size_t mdf_potrf_workspace(long n);
+int mdf_potrf(double *A, long n, long lda, int *info, void *work, size_t work_bytes, void *stream);
+size_t mdf_trtri_workspace(long n);
+int mdf_trtri(const double *L, long n, long ldl, const double *D, double *X, long ldx, void *work,
+              size_t work_bytes, void *stream);
+int mdf_trmm(const double *Linv, long n, long ldl, const double *B, long ldb, long ncols, double *C,
+             long ldc, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
