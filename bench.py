@@ -4,7 +4,10 @@ Workload (one "step" = one FP64 ``get_jk`` call, J and K, over a resident
 cderi): a 64-water cluster at 1.0 g/cm^3 (seed 0), def2-SVP orbital basis
 (nao 1536) with the 109-function-per-water fitting set (naux 6976), the full
 packed triangle (npairs 1,180,416, 65.9 GB in FP64), and a mo-tagged random
-PSD density of rank nocc = 320.
+PSD density of rank nocc = 320.  The cderi is built on the GPU first (Rys
+int2c2e/int3c2e, Cholesky, L^-1, triangular multiply) and its build is timed
+per stage; ``--synthetic-cderi`` replaces it with seeded N(0,1) values of the
+same shape for quick kernel timing.
 
 Prints ONE JSON line.  ``roofline`` is the K build (the dominant kernel
 group) against FP64 MFMA peak; ``roofline_j`` is the J build against HBM
@@ -35,6 +38,7 @@ def parse():
     ap.add_argument('--nwater', type=int, default=64)
     ap.add_argument('--nocc', type=int, default=320)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--synthetic-cderi', action='store_true')
     ap.add_argument('--cpu-sample-j', type=int, default=96)
     ap.add_argument('--cpu-sample-k', type=int, default=24)
     return ap.parse_args()
@@ -97,11 +101,28 @@ def main():
     nocc = args.nocc
     log(f'(H2O){args.nwater}: nao {nao} naux {naux} npairs {npairs} nocc {nocc}')
 
-    g = torch.Generator(device='cuda')
-    g.manual_seed(0)
-    cderi = torch.empty((naux, npairs), dtype=torch.float64, device='cuda')
-    for r0 in range(0, naux, 512):
-        cderi[r0:r0 + 512].normal_(generator=g)
+    build = None
+    if args.synthetic_cderi:
+        g = torch.Generator(device='cuda')
+        g.manual_seed(0)
+        cderi = torch.empty((naux, npairs), dtype=torch.float64, device='cuda')
+        for r0 in range(0, naux, 512):
+            cderi[r0:r0 + 512].normal_(generator=g)
+    else:
+        from electrolites_amd import fastdfcderi
+        log('building the cderi on the GPU')
+        timer = fastdfcderi._Timer()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cderi, _ = fastdfcderi.cholesky_eri(mol, timer=timer)
+        torch.cuda.synchronize()
+        wall_b = time.perf_counter() - t0
+        st = timer.totals()
+        fm = float(npairs) * naux * naux
+        build = {'wall_s': wall_b, 'stages_ms': st,
+                 'trmm_tflops': fm / (st['trmm'] * 1e-3) / 1e12 if st.get('trmm') else None,
+                 'metric_flop': fm, 'stats': fastdfcderi.stats()}
+        log(f'cderi built in {wall_b:.2f} s: {st}')
     rng = np.random.default_rng(0)
     c = np.linalg.qr(rng.standard_normal((nao, nocc)))[0]
     dm = torch.from_numpy(2 * c @ c.T).cuda()
@@ -156,12 +177,16 @@ def main():
         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': ms_step,
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f64', 'data': 'synthetic (seeded N(0,1) cderi of the '
-                                '(H2O)64 shape; random rank-320 PSD density)',
+        'dtype': 'f64',
+        'data': ('synthetic (seeded N(0,1) cderi of the (H2O)64 shape; random '
+                 'rank-320 PSD density)') if args.synthetic_cderi else
+                ('synthetic geometry: seeded (H2O)64 cluster, cderi built on the '
+                 'GPU from its integrals; random rank-320 PSD density'),
         'config': {'workload': f'(H2O){args.nwater} get_jk FP64', 'nao': nao,
                    'naux': naux, 'npairs': npairs, 'nocc': nocc, 'n_dm': 1,
                    'cderi_GB': S / 1e9},
         'get_jk_ms': {'j': tj * 1e3, 'k': tk * 1e3},
+        'cderi_build': build,
         'roofline': {'bound': 'mfma', 'achieved': fk / tk / 1e12,
                      'peak': FP64_MFMA_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': fk / tk / FP64_MFMA_PEAK, 'traffic': None,

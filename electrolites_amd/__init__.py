@@ -1,1 +1,92 @@
-"""MI355X-native density-fitted J/K path."""
+"""MI355X-native density-fitted two-electron path (the Electrolites DF drop-in).
+
+When GPU4PySCF is installed, ``patch()`` installs this package's functions
+over the two entry points the reference replaces (electrolites/__init__.py
+:59-150, ``_TARGETS`` entries 'fastdf' and 'fastdfcderi'):
+
+    gpu4pyscf.df.df_jk.get_jk        -> electrolites_amd.fastdf.get_jk
+    gpu4pyscf.df.df._cholesky_eri    -> electrolites_amd.fastdfcderi._cholesky_eri
+
+Targets are checked before anything is assigned (``check_targets``, as
+electrolites/__init__.py:302-322), the originals are kept for fall-through
+(``fastdf._orig_get_jk``), and there is no unpatch (:19-22).
+
+Without GPU4PySCF the same functions run on this package's own objects::
+
+    from electrolites_amd.gto import water_cluster
+    from electrolites_amd.df import DF
+    mol = water_cluster(64, seed=0)
+    df = DF(mol).build()                 # cderi on the GPU
+    vj, vk = df.get_jk(dm)               # FP64 J/K on the GPU
+"""
+import importlib
+
+__version__ = '0.1.0'
+
+#: module -> ((target module, attribute), ...)
+_TARGETS = {
+    'fastdf': (('gpu4pyscf.df.df_jk', 'get_jk'),),
+    'fastdfcderi': (('gpu4pyscf.df.df', '_cholesky_eri'),),
+}
+MODULES = ('fastdf', 'fastdfcderi')
+_patched = []
+
+
+def check_targets(names=None):
+    """``{module: None}`` when every target exists, else ``{module: reason}``."""
+    out = {}
+    for name in (names or MODULES):
+        out[name] = None
+        for mod_path, attr in _TARGETS[name]:
+            try:
+                obj = importlib.import_module(mod_path)
+                for part in attr.split('.'):
+                    obj = getattr(obj, part)
+            except (ImportError, AttributeError) as exc:
+                out[name] = f'{mod_path}.{attr} is missing ({exc})'
+                break
+    return out
+
+
+def _install(name):
+    if name == 'fastdf':
+        from . import fastdf
+        mod = importlib.import_module('gpu4pyscf.df.df_jk')
+        if fastdf._orig_get_jk is None:
+            fastdf._orig_get_jk = mod.get_jk
+        mod.get_jk = fastdf.get_jk
+        # DF.get_jk looks the function up through the module at call time
+        # (gpu4pyscf/df/df.py:117); rebinding the attribute covers it
+    elif name == 'fastdfcderi':
+        from . import fastdfcderi
+        mod = importlib.import_module('gpu4pyscf.df.df')
+        if getattr(fastdfcderi, '_orig_cholesky_eri', None) is None:
+            fastdfcderi._orig_cholesky_eri = mod._cholesky_eri
+        mod._cholesky_eri = fastdfcderi._cholesky_eri
+
+
+def patch(*names, strict=True):
+    """Patch the named modules (default: all).  ``strict`` raises when a
+    target is missing; otherwise missing ones are skipped."""
+    names = [n for g in (names or (MODULES,)) for n in ((g,) if isinstance(g, str) else g)]
+    unknown = [n for n in names if n not in _TARGETS]
+    if unknown:
+        raise ValueError(f'unknown module(s) {unknown}; known: {list(MODULES)}')
+    missing = {k: v for k, v in check_targets(names).items() if v}
+    if missing:
+        if strict:
+            raise RuntimeError(f'GPU4PySCF does not expose the entry point(s) '
+                               f'these modules replace: {missing}')
+        names = [n for n in names if n not in missing]
+    for name in names:
+        _install(name)
+        if name not in _patched:
+            _patched.append(name)
+    return tuple(_patched)
+
+
+def patched():
+    return tuple(_patched)
+
+
+__all__ = ['patch', 'patched', 'check_targets', 'MODULES', '__version__']

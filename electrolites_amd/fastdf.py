@@ -79,7 +79,7 @@ def _cderi_of(dfobj, missing_ok=False):
             return None
         raise ValueError('dfobj has no cderi; call build() first')
     t = _dev.torch()
-    dev = t.cuda.current_device()
+    dev = t.cuda.current_device() if t.cuda.is_available() else 0
     try:
         return store[dev]
     except (KeyError, IndexError, TypeError):
@@ -250,12 +250,6 @@ def _get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
         return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
                              'FASTDF_OFF')
     t = _dev.torch()
-    if not t.cuda.is_available():
-        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
-                             'no GPU')
-    if t.cuda.device_count() != 1 and os.environ.get('MDF_ALLOW_MULTI', '0') == '0':
-        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
-                             'more than one device')
     dt = getattr(dms, 'dtype', None)
     if dt not in (np.float64, t.float64):
         return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
@@ -264,6 +258,12 @@ def _get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
     if omega not in (None, 0, 0.):
         return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
                              'range-separated omega')
+    if not t.cuda.is_available():
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'no GPU')
+    if t.cuda.device_count() != 1 and os.environ.get('MDF_ALLOW_MULTI', '0') == '0':
+        return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
+                             'more than one device')
     if getattr(dfobj, '_cderi', None) is None:
         dfobj.build()
     cderi = _cderi_of(dfobj)

@@ -46,22 +46,57 @@ def _basis(mol, which):
     return b
 
 
-def decompose_j2c(aux):
+class _Timer:
+    """Per-stage device time from HIP events on the launch stream."""
+
+    def __init__(self):
+        self.events = {}
+
+    def mark(self, name):
+        t = _dev.torch()
+        ev = t.cuda.Event(enable_timing=True)
+        ev.record()
+        self.events.setdefault(name, []).append(ev)
+
+    def totals(self):
+        _dev.torch().cuda.synchronize()
+        out = {}
+        for name, evs in self.events.items():
+            if name.endswith(':0'):
+                base = name[:-2]
+                ends = self.events.get(base + ':1', [])
+                out[base] = sum(a.elapsed_time(b) for a, b in zip(evs, ends))
+        return out
+
+
+class _NoTimer:
+    def mark(self, name):
+        pass
+
+
+def decompose_j2c(aux, timer=None):
     """``(Linv, None)`` from the Cholesky route, or ``(None, coef)`` with
     ``coef = (v / sqrt(w))^T`` from the eigen filter (non-PD metric)."""
     t = _dev.torch()
+    tm = timer or _NoTimer()
     n = aux.nao
+    tm.mark('int2c2e:0')
     j2c = ints.int2c2e(aux)
+    tm.mark('int2c2e:1')
     info = t.zeros(1, dtype=t.int32, device=_dev.device())
     work = t.empty(_lib.query('mdf_potrf_workspace', n), dtype=t.uint8, device=_dev.device())
     low = j2c.clone()
+    tm.mark('potrf:0')
     _lib.call('mdf_potrf', _dev.ptr(low), n, n, _dev.ptr(info), _dev.ptr(work),
               work.numel(), _dev.stream_ptr())
+    tm.mark('potrf:1')
     if int(info.item()) == 0:
         linv = t.empty((n, n), dtype=t.float64, device=_dev.device())
         w2 = t.empty(_lib.query('mdf_trtri_workspace', n), dtype=t.uint8, device=_dev.device())
+        tm.mark('trtri:0')
         _lib.call('mdf_trtri', _dev.ptr(low), n, n, _dev.ptr(work), _dev.ptr(linv), n,
                   _dev.ptr(w2), w2.numel(), _dev.stream_ptr())
+        tm.mark('trtri:1')
         return linv, None
     STATS['eigen_fallback'] += 1
     w, v = t.linalg.eigh(j2c)
@@ -88,16 +123,18 @@ def _chunks(shells, naux, budget):
     return out
 
 
-def cholesky_eri(mol, chunk_bytes=None):
+def cholesky_eri(mol, chunk_bytes=None, timer=None):
     """Build the cderi of ``mol`` (orbital basis ``mol.ao``, fitting basis
     ``mol.aux``) on the current device.  Returns ``(cderi, (pair_addr,
-    diag_idx))``."""
+    diag_idx))``.  ``timer`` (a :class:`_Timer`) collects per-stage device
+    times."""
     _dev.require_gpu()
     t = _dev.torch()
+    tm = timer or _NoTimer()
     ao, aux = _basis(mol, 'ao'), _basis(mol, 'aux')
     nao, naux = ao.nao, aux.nao
     npairs = nao * (nao + 1) // 2
-    linv, coef = decompose_j2c(aux)
+    linv, coef = decompose_j2c(aux, timer)
     nrow = naux if coef is None else coef.shape[0]
     cderi = t.empty((nrow, npairs), dtype=t.float64, device=_dev.device())
     bound = getattr(mol, '_mdf_pair_bound', None)
@@ -116,15 +153,19 @@ def cholesky_eri(mol, chunk_bytes=None):
         STATS['pairs_screened'] += (I1 - I0) * (I0 + I1 + 1) // 2 - tasks.npairs_kept
         width = tasks.p1 - tasks.p0
         blk = buf[:naux * width]
+        tm.mark('int3c2e:0')
         _lib.call('mdf_fill_zero', _dev.ptr(blk), naux * width, _dev.stream_ptr())
         ints.int3c2e_chunk(ao, aux, tasks, blk, width)
+        tm.mark('int3c2e:1')
         dst = cderi[:, tasks.p0:]
+        tm.mark('trmm:0')
         if coef is None:
             _lib.call('mdf_trmm', _dev.ptr(linv), naux, naux, _dev.ptr(blk), width, width,
                       _dev.ptr(dst), npairs, _dev.stream_ptr())
         else:
             _lib.call('mdf_gemm', 0, 0, 0, nrow, width, naux, 1., _dev.ptr(coef), naux,
                       _dev.ptr(blk), width, 0., _dev.ptr(dst), npairs, _dev.stream_ptr())
+        tm.mark('trmm:1')
         STATS['chunks'] += 1
     STATS['builds'] += 1
     rows, cols = np.tril_indices(nao)

@@ -56,23 +56,28 @@ class DeviceBasis:
 
 
 def pair_bound(shells):
-    """Per shell pair (I >= J): max over primitive pairs of
-    |c_a c_b| exp(-ab/(a+b) |A-B|^2) (pi/(a+b))^(3/2)."""
+    """Per shell pair (I, J): max over primitive pairs of
+    |c_a c_b| exp(-ab/(a+b) |A-B|^2) (pi/(a+b))^(3/2)  (symmetric)."""
     n = shells.nbas
+    npm = int(shells.shl_nprim.max()) if n else 1
+    e = np.ones((n, npm))
+    c = np.zeros((n, npm))
+    for s_ in range(n):
+        o, k = shells.shl_prim_off[s_], shells.shl_nprim[s_]
+        e[s_, :k] = shells.exps[o:o + k]
+        c[s_, :k] = np.abs(shells.coefs[o:o + k])
     xyz = shells.shl_xyz
     out = np.zeros((n, n))
-    for I in range(n):
-        ea = shells.exps[shells.shl_prim_off[I]:shells.shl_prim_off[I] + shells.shl_nprim[I]]
-        ca = shells.coefs[shells.shl_prim_off[I]:shells.shl_prim_off[I] + shells.shl_nprim[I]]
-        r2 = np.sum((xyz[:I + 1] - xyz[I]) ** 2, axis=1)
-        for J in range(I + 1):
-            o = shells.shl_prim_off[J]
-            eb = shells.exps[o:o + shells.shl_nprim[J]]
-            cb = shells.coefs[o:o + shells.shl_nprim[J]]
-            p = ea[:, None] + eb[None, :]
-            v = np.abs(ca[:, None] * cb[None, :]) * np.exp(-ea[:, None] * eb[None, :] / p * r2[J]) \
-                * (np.pi / p) ** 1.5
-            out[I, J] = v.max()
+    step = max(1, 4000000 // max(n * npm * npm, 1))
+    for I0 in range(0, n, step):
+        I1 = min(n, I0 + step)
+        r2 = np.sum((xyz[I0:I1, None, :] - xyz[None, :, :]) ** 2, axis=-1)
+        ea = e[I0:I1, None, :, None]
+        eb = e[None, :, None, :]
+        p = ea + eb
+        v = (c[I0:I1, None, :, None] * c[None, :, None, :] *
+             np.exp(-ea * eb / p * r2[:, :, None, None]) * (np.pi / p) ** 1.5)
+        out[I0:I1] = v.reshape(I1 - I0, n, -1).max(axis=-1)
     return out
 
 
@@ -81,16 +86,16 @@ class PairTasks:
 
     def __init__(self, shells, I0, I1, bound=None, thr=PAIR_SCREEN):
         l = np.asarray(shells.shl_l)
-        pairs = []
-        for I in range(I0, I1):
-            for J in range(I + 1):
-                if bound is not None and bound[I, J] < thr:
-                    continue
-                pairs.append((I, J))
-        pairs = np.array(pairs, dtype=np.int32).reshape(-1, 2)
-        cls = l[pairs[:, 0]] * (LMAX_AO + 1) + l[pairs[:, 1]] if len(pairs) else np.zeros(0, int)
+        ii, jj = np.tril_indices(I1)
+        sel = ii >= I0
+        ii, jj = ii[sel], jj[sel]
+        if bound is not None:
+            keep = bound[ii, jj] >= thr
+            ii, jj = ii[keep], jj[keep]
+        pairs = np.stack([ii, jj], axis=1).astype(np.int32)
+        cls = l[ii] * (LMAX_AO + 1) + l[jj]
         order = np.argsort(cls, kind='stable')
-        self.pairs = pairs[order]
+        self.pairs = np.ascontiguousarray(pairs[order])
         cnt = np.bincount(cls, minlength=(LMAX_AO + 1) ** 2)
         self.class_off = np.zeros((LMAX_AO + 1) ** 2 + 1, dtype=np.int64)
         self.class_off[1:] = np.cumsum(cnt)
@@ -99,9 +104,16 @@ class PairTasks:
         self.p0 = i0 * (i0 + 1) // 2
         self.p1 = i1 * (i1 + 1) // 2
         self.npairs_kept = len(pairs)
-        t = _dev.torch()
-        self.dev = _dev.to_device(self.pairs.reshape(-1), dtype=t.int32) if len(pairs) else \
-            t.zeros(2, dtype=t.int32, device=_dev.device())
+        self._dev = None
+
+    @property
+    def dev(self):
+        """The pair list on the device (uploaded on first use)."""
+        if self._dev is None:
+            t = _dev.torch()
+            self._dev = _dev.to_device(self.pairs.reshape(-1), dtype=t.int32) if \
+                self.npairs_kept else t.zeros(2, dtype=t.int32, device=_dev.device())
+        return self._dev
 
 
 def _lp(a):

@@ -1,0 +1,108 @@
+"""The drop-in boundary without a GPU: patch registry over a stand-in
+GPU4PySCF module tree (the reference's own boundary test walks GPU4PySCF for
+stale aliases, tests/test_patch_aliases.py:1-33), decline bookkeeping, and
+the library's error reporting."""
+import sys
+import types
+
+import numpy as np
+import pytest
+
+
+@pytest.fixture
+def fake_gpu4pyscf(monkeypatch):
+    calls = []
+
+    def orig_get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
+        calls.append(('get_jk', omega))
+        return 'orig-vj', 'orig-vk'
+
+    def orig_cholesky_eri(intopt, omega=None, use_gpu_memory=None):
+        calls.append(('cholesky_eri', omega))
+        return 'orig'
+
+    root = types.ModuleType('gpu4pyscf')
+    df_pkg = types.ModuleType('gpu4pyscf.df')
+    df_jk = types.ModuleType('gpu4pyscf.df.df_jk')
+    df_mod = types.ModuleType('gpu4pyscf.df.df')
+    df_jk.get_jk = orig_get_jk
+    df_mod._cholesky_eri = orig_cholesky_eri
+    root.df = df_pkg
+    df_pkg.df_jk = df_jk
+    df_pkg.df = df_mod
+    for name, mod in [('gpu4pyscf', root), ('gpu4pyscf.df', df_pkg),
+                      ('gpu4pyscf.df.df_jk', df_jk), ('gpu4pyscf.df.df', df_mod)]:
+        monkeypatch.setitem(sys.modules, name, mod)
+    import electrolites_amd
+    from electrolites_amd import fastdf, fastdfcderi
+    monkeypatch.setattr(fastdf, '_orig_get_jk', None)
+    monkeypatch.setattr(fastdfcderi, '_orig_cholesky_eri', None, raising=False)
+    monkeypatch.setattr(electrolites_amd, '_patched', [])
+    return types.SimpleNamespace(df_jk=df_jk, df=df_mod, calls=calls,
+                                 orig_get_jk=orig_get_jk, orig_cholesky_eri=orig_cholesky_eri)
+
+
+def test_patch_installs_and_keeps_originals(fake_gpu4pyscf):
+    import electrolites_amd
+    from electrolites_amd import fastdf, fastdfcderi
+    assert electrolites_amd.check_targets() == {'fastdf': None, 'fastdfcderi': None}
+    assert electrolites_amd.patch() == ('fastdf', 'fastdfcderi')
+    assert fake_gpu4pyscf.df_jk.get_jk is fastdf.get_jk
+    assert fake_gpu4pyscf.df._cholesky_eri is fastdfcderi._cholesky_eri
+    assert fastdf._orig_get_jk is fake_gpu4pyscf.orig_get_jk
+    assert fastdfcderi._orig_cholesky_eri is fake_gpu4pyscf.orig_cholesky_eri
+    # patching twice keeps the true originals (no alias of ourselves)
+    electrolites_amd.patch()
+    assert fastdf._orig_get_jk is fake_gpu4pyscf.orig_get_jk
+
+
+def test_declined_call_falls_through(fake_gpu4pyscf):
+    import electrolites_amd
+    from electrolites_amd import fastdf
+    electrolites_amd.patch('fastdf')
+    before = fastdf.stats()['declined']
+    dm = np.zeros((4, 4), dtype=np.complex128)
+    out = fake_gpu4pyscf.df_jk.get_jk(object(), dm, hermi=1)
+    assert out == ('orig-vj', 'orig-vk')
+    st = fastdf.stats()
+    assert st['declined'] == before + 1 and st['declined_reasons'].get('complex density')
+    out = fake_gpu4pyscf.df_jk.get_jk(object(), np.zeros((4, 4)), hermi=1, omega=.3)
+    assert out == ('orig-vj', 'orig-vk')
+    assert fake_gpu4pyscf.calls[-1] == ('get_jk', .3)
+
+
+def test_missing_target_is_reported(fake_gpu4pyscf, monkeypatch):
+    import electrolites_amd
+    monkeypatch.delattr(fake_gpu4pyscf.df, '_cholesky_eri')
+    rep = electrolites_amd.check_targets()
+    assert rep['fastdf'] is None and 'missing' in rep['fastdfcderi']
+    with pytest.raises(RuntimeError):
+        electrolites_amd.patch()
+    assert electrolites_amd.patch(strict=False) == ('fastdf',)
+
+
+def test_standalone_decline_raises(monkeypatch):
+    from electrolites_amd import fastdf
+    monkeypatch.setattr(fastdf, '_orig_get_jk', None)
+    with pytest.raises(fastdf.Declined):
+        fastdf.get_jk(object(), np.zeros((3, 3), dtype=np.complex128))
+
+
+def test_narrow_tensor_never_falls_through(monkeypatch):
+    """fastdf.py:434-440: a float32 cderi must raise instead of reaching a
+    float64 reader."""
+    import torch
+    from electrolites_amd import fastdf
+
+    class DF:
+        _cderi = {0: torch.zeros((2, 3), dtype=torch.float32)}
+
+    monkeypatch.setattr(fastdf, '_orig_get_jk', lambda *a, **k: ('x', 'y'))
+    with pytest.raises(RuntimeError):
+        fastdf.get_jk(DF(), np.zeros((2, 2), dtype=np.complex128))
+
+
+def test_library_error_message():
+    from electrolites_amd import _lib
+    with pytest.raises(_lib.MDFError, match='bad sizes'):
+        _lib.call('mdf_get_k', None, -1, 0, 0, None, 0, None, None, 0, 0, None, 0, 0, None, 0, None)
