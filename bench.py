@@ -167,6 +167,20 @@ def main():
     tk = np.mean([e[2].elapsed_time(e[3]) for e in evs]) * 1e-3
     ms_step = wall / args.steps * 1e3
 
+    # the same K in FP32 (the mixed-precision cycles), outside the timed steps
+    fl, fr = fastdf.factorize_dm(dm, 1)
+    lay = fastdf._layout(dfobj, cderi, nao)
+    fastdf._vk(cderi, lay, fl, fr, 'fp32')
+    e32 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    e32[0].record()
+    vk32 = fastdf._vk(cderi, lay, fl, fr, 'fp32')
+    e32[1].record()
+    vk64 = fastdf._vk(cderi, lay, fl, fr, 'fp64')
+    torch.cuda.synchronize()
+    tk32 = e32[0].elapsed_time(e32[1]) * 1e-3
+    k32_err = float((vk32 - vk64).abs().max() / vk64.abs().max())
+    del vk32, vk64
+
     S = naux * npairs * 8
     bj = 2 * S + 16 * (npairs + naux)
     fk = 2. * naux * nao * nao * nocc + 1. * naux * nocc * nao * (nao + 1)
@@ -185,7 +199,8 @@ def main():
         'config': {'workload': f'(H2O){args.nwater} get_jk FP64', 'nao': nao,
                    'naux': naux, 'npairs': npairs, 'nocc': nocc, 'n_dm': 1,
                    'cderi_GB': S / 1e9},
-        'get_jk_ms': {'j': tj * 1e3, 'k': tk * 1e3},
+        'get_jk_ms': {'j': tj * 1e3, 'k': tk * 1e3, 'k_fp32': tk32 * 1e3,
+                      'k_fp32_rel_err_vs_fp64': k32_err},
         'cderi_build': build,
         'roofline': {'bound': 'mfma', 'achieved': fk / tk / 1e12,
                      'peak': FP64_MFMA_PEAK / 1e12, 'unit': 'TFLOP/s',

@@ -197,6 +197,127 @@ def gen_class(la, lb, lc, two_c):
     return name, '\n'.join(L)
 
 
+def gen_int1e(la, lb):
+    """S, T and V (all point charges) of one shell pair, one thread each."""
+    name = f'int1e_{la}{lb}'
+    ca, cb = cart_powers(la), cart_powers(lb)
+    nca, ncb = len(ca), len(cb)
+    Ca, Cb = cart2sph(la), cart2sph(lb)
+    nroots = (la + lb) // 2 + 1
+    L = []
+    w = L.append
+    w(f'__global__ __launch_bounds__({THREADS}) void {name}(const BasisDev ao, '
+      'const int *__restrict__ pairs, int np, int natm, const double *__restrict__ charges, '
+      'const double *__restrict__ atom_xyz, double *__restrict__ S, double *__restrict__ T, '
+      'double *__restrict__ V, int nao) {')
+    w(f'    const int ip = blockIdx.x * {THREADS} + threadIdx.x;')
+    w('    if (ip >= np) return;')
+    w('    const int ish = pairs[2 * ip], jsh = pairs[2 * ip + 1];')
+    w('    const double Ax = ao.xyz[3 * ish], Ay = ao.xyz[3 * ish + 1], Az = ao.xyz[3 * ish + 2];')
+    w('    const double Bx = ao.xyz[3 * jsh], By = ao.xyz[3 * jsh + 1], Bz = ao.xyz[3 * jsh + 2];')
+    w('    const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;')
+    w('    const double rab2 = ABx * ABx + ABy * ABy + ABz * ABz;')
+    w(f'    double sa[{nca * ncb}], ta[{nca * ncb}], va[{nca * ncb}];')
+    w('#pragma unroll')
+    w(f'    for (int t = 0; t < {nca * ncb}; t++) {{ sa[t] = 0.; ta[t] = 0.; va[t] = 0.; }}')
+    w('    const int npa = ao.nprim[ish], oa = ao.poff[ish], npb = ao.nprim[jsh], ob = ao.poff[jsh];')
+    w('    for (int ia = 0; ia < npa; ia++) {')
+    w('        const double ea = ao.exps[oa + ia], da = ao.coefs[oa + ia];')
+    w('        for (int ib = 0; ib < npb; ib++) {')
+    w('            const double eb = ao.exps[ob + ib];')
+    w('            const double p = ea + eb, ip_ = 1.0 / p, h = 0.5 * ip_;')
+    w('            const double fab = da * ao.coefs[ob + ib] * exp(-ea * eb * ip_ * rab2);')
+    w('            if (fabs(fab) < PRIM_SCREEN) continue;')
+    w('            const double Px = (ea * Ax + eb * Bx) * ip_, Py = (ea * Ay + eb * By) * ip_, '
+      'Pz = (ea * Az + eb * Bz) * ip_;')
+    w('            const double PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az;')
+    w('            const double PBx = Px - Bx, PBy = Py - By, PBz = Pz - Bz;')
+    # 1D overlaps s(i, j) up to (la, lb + 2) by Obara-Saika, unit prefactor
+    for d in 'xyz':
+        for i in range(la + 1):
+            for j in range(lb + 3):
+                if i == 0 and j == 0:
+                    w(f'            const double s{d}_0_0 = 1.0;')
+                    continue
+                if i > 0:
+                    terms = [f'PA{d} * s{d}_{i-1}_{j}']
+                    if i - 1 >= 1:
+                        terms.append(f'{i-1}.0 * h * s{d}_{i-2}_{j}')
+                    if j >= 1:
+                        terms.append(f'{j}.0 * h * s{d}_{i-1}_{j-1}')
+                else:
+                    terms = [f'PB{d} * s{d}_{i}_{j-1}']
+                    if j - 1 >= 1:
+                        terms.append(f'{j-1}.0 * h * s{d}_{i}_{j-2}')
+                w(f'            const double s{d}_{i}_{j} = ' + ' + '.join(terms) + ';')
+        for i in range(la + 1):
+            for j in range(lb + 1):
+                terms = [f'-2.0 * eb * {2 * j + 1}.0 * s{d}_{i}_{j}', f'4.0 * eb * eb * s{d}_{i}_{j+2}']
+                if j >= 2:
+                    terms.append(f'{j * (j - 1)}.0 * s{d}_{i}_{j-2}')
+                w(f'            const double k{d}_{i}_{j} = -0.5 * (' + ' + '.join(terms) + ');')
+    w('            const double sp = fab * PI_1P5 * ip_ * sqrt(ip_);')
+    for a_i, (ax, ay, az) in enumerate(ca):
+        for b_i, (bx, by, bz) in enumerate(cb):
+            k = a_i * ncb + b_i
+            w(f'            sa[{k}] += sp * sx_{ax}_{bx} * sy_{ay}_{by} * sz_{az}_{bz};')
+            w(f'            ta[{k}] += sp * (kx_{ax}_{bx} * sy_{ay}_{by} * sz_{az}_{bz} + '
+              f'sx_{ax}_{bx} * ky_{ay}_{by} * sz_{az}_{bz} + sx_{ax}_{bx} * sy_{ay}_{by} * kz_{az}_{bz});')
+    # nuclear attraction by Rys with point charges
+    lab = la + lb
+    w('            const double vp = -TWO_PI * ip_ * fab;')
+    w('            for (int c = 0; c < natm; c++) {')
+    w('                const double PCx = Px - atom_xyz[3 * c], PCy = Py - atom_xyz[3 * c + 1], '
+      'PCz = Pz - atom_xyz[3 * c + 2];')
+    w('                const double x = p * (PCx * PCx + PCy * PCy + PCz * PCz);')
+    w(f'                double ru[{nroots}], rw[{nroots}];')
+    w(f'                rys_roots<{nroots}>(x, ru, rw);')
+    w('                const double zc = vp * charges[c];')
+    w('#pragma unroll')
+    w(f'                for (int r = 0; r < {nroots}; r++) {{')
+    w('                    const double u = ru[r];')
+    if lab >= 2:
+        w('                    const double B10 = (1.0 - u) * h;')
+    for d in 'xyz':
+        w(f'                    const double C00{d} = PA{d} - u * PC{d};')
+        init = 'zc * rw[r]' if d == 'z' else '1.0'
+        w(f'                    const double g{d}_0 = {init};')
+        for e in range(1, lab + 1):
+            terms = [f'C00{d} * g{d}_{e-1}']
+            if e - 1 >= 1:
+                terms.append(f'{e-1}.0 * B10 * g{d}_{e-2}')
+            w(f'                    const double g{d}_{e} = ' + ' + '.join(terms) + ';')
+        for a in range(lab + 1):
+            w(f'                    const double I{d}_{a}_0 = g{d}_{a};')
+        for b in range(1, lb + 1):
+            for a in range(0, lab - b + 1):
+                w(f'                    const double I{d}_{a}_{b} = I{d}_{a+1}_{b-1} + AB{d} * I{d}_{a}_{b-1};')
+    for a_i, (ax, ay, az) in enumerate(ca):
+        for b_i, (bx, by, bz) in enumerate(cb):
+            w(f'                    va[{a_i * ncb + b_i}] += Ix_{ax}_{bx} * Iy_{ay}_{by} * Iz_{az}_{bz};')
+    w('                }')
+    w('            }')
+    w('        }')
+    w('    }')
+    w('    const int i0 = ao.aooff[ish], j0 = ao.aooff[jsh];')
+    for s_ in range(2 * la + 1):
+        for t in range(2 * lb + 1):
+            idx = []
+            for a in range(nca):
+                if abs(Ca[s_, a]) < 1e-15:
+                    continue
+                for b in range(ncb):
+                    if abs(Cb[t, b]) < 1e-15:
+                        continue
+                    idx.append((fmt(Ca[s_, a] * Cb[t, b]), a * ncb + b))
+            for arr, outn in (('sa', 'S'), ('ta', 'T'), ('va', 'V')):
+                expr = ' + '.join(f'{c} * {arr}[{k}]' for c, k in idx)
+                w(f'    {{ const double v = {expr}; {outn}[(long)(i0 + {s_}) * nao + j0 + {t}] = v; '
+                  f'{outn}[(long)(j0 + {t}) * nao + i0 + {s_}] = v; }}')
+    w('}')
+    return name, '\n'.join(L)
+
+
 def main():
     parts = ['// Generated by electrolites_amd/codegen/gen_int.py; do not edit.',
              '// Rys-quadrature int3c2e / int2c2e kernels, one per angular class.',
@@ -216,7 +337,29 @@ def main():
             names2[(la, lc)] = n
             parts.append(src)
             parts.append('')
+    names1 = {}
+    for la in range(LMAX_AO + 1):
+        for lb in range(LMAX_AO + 1):
+            n, src = gen_int1e(la, lb)
+            names1[(la, lb)] = n
+            parts.append(src)
+            parts.append('')
     # dispatch
+    parts.append('int launch_int1e(int la, int lb, const BasisDev &ao, const int *pairs, int np, int natm, '
+                 'const double *charges, const double *atom_xyz, double *S, double *T, double *V, int nao, '
+                 'hipStream_t st) {')
+    parts.append('    if (np == 0) return MDF_OK;')
+    parts.append(f'    const dim3 g((unsigned)((np + {THREADS - 1}) / {THREADS})), b({THREADS});')
+    parts.append(f'    switch (la * {LMAX_AO + 1} + lb) {{')
+    for (la, lb), n in names1.items():
+        parts.append(f'    case {la * (LMAX_AO + 1) + lb}: hipLaunchKernelGGL({n}, g, b, 0, st, ao, pairs, np, natm, '
+                     'charges, atom_xyz, S, T, V, nao); break;')
+    parts.append('    default: set_error("int1e: class (%d %d) not generated", la, lb); return MDF_EINVAL;')
+    parts.append('    }')
+    parts.append('    MDF_LAUNCHED("int1e");')
+    parts.append('    return MDF_OK;')
+    parts.append('}')
+    parts.append('')
     parts.append('int launch_int3c(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, '
                  'const int *pairs, int np, const int *kl, int nk, double *out, long ldo, long p0, '
                  'hipStream_t st) {')

@@ -1,62 +1,87 @@
-// FP64 MFMA tile engine shared by the K contraction and the metric solve.
+// MFMA tile engine shared by the K contraction and the metric solve.
 //
 // One workgroup (256 threads = 4 wavefronts as 2 x 2) computes a BM x BN
 // tile of C = sum_k A[m, k] B[k, n] over nk K-steps of BK = 16.  Operands are
 // staged through double-buffered LDS (A as As[k][m], B as Bs[k][n]) with the
 // next K-step's global loads held in registers while the current step's MFMAs
 // run, so there is one barrier per K-step.  Loaders decide where a tile comes
-// from (packed triangle, row-major matrix, transformed block ...):
+// from (packed triangle, row-major matrix, transformed block ...) and convert
+// to the engine's element type T on the way into LDS:
 //
 //   struct Loader {
-//     static constexpr int N;                 // elements per thread per K-step
-//     __device__ void load(int kt, double (&r)[N]);    // global -> registers
-//     __device__ void store(const double (&r)[N], double *lds, int pitch);
+//     static constexpr int N;                  // elements per thread per K-step
+//     __device__ void load(int kt, T (&r)[N]);  // global -> registers
+//     __device__ void store(const T (&r)[N], T *lds, int pitch);
 //   };
 //
-// v_mfma_f64_16x16x4_f64 operand layout on gfx950: A lane = m + 16 k,
-// B lane = n + 16 k, D element v of lane l is row l/16 + 4 v, column l % 16.
+// T = double: v_mfma_f64_16x16x4_f64; T = float: v_mfma_f32_16x16x4_f32.
+// Lane layouts measured on gfx950 with one-hot probes: A lane = m + 16 k,
+// B lane = n + 16 k for both; the result element v of lane l is row
+// l/16 + 4 v (f64) or 4 (l/16) + v (f32), column l % 16.
 #pragma once
 #include "mdf_common.h"
+
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
 namespace mdf {
 
 constexpr int G_THREADS = 256;
 constexpr int G_BK = 16;
 
-__device__ inline double4_t mfma16(double a, double b, double4_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+    using acc_t = double4_t;
+    __device__ static acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <>
+struct Mfma<float> {
+    using acc_t = float4_t;
+    __device__ static acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+__device__ inline double4_t mfma16(double a, double b, double4_t c) { return Mfma<double>::mma(a, b, c); }
 
 template <int BM, int BN>
 struct TileShape {
     static constexpr int WM = BM / 2, WN = BN / 2;       // per-wave tile
     static constexpr int MI = WM / 16, NI = WN / 16;     // MFMA tiles per wave
-    static constexpr int PA = BM + 16, PB = BN + 16;     // LDS pitches (doubles)
-    static constexpr int LDS_DOUBLES = 2 * G_BK * (PA + PB);
+    static constexpr int PA = BM + 16, PB = BN + 16;     // LDS pitches (elements)
+    static constexpr int LDS_ELEMS = 2 * G_BK * (PA + PB);
+    static constexpr int LDS_DOUBLES = LDS_ELEMS;
 };
 
 // Accumulator tile of one wavefront.
-template <int MI, int NI>
+template <int MI, int NI, typename T = double>
 struct Acc {
-    double4_t v[MI][NI];
+    typename Mfma<T>::acc_t v[MI][NI];
     __device__ void zero() {
 #pragma unroll
         for (int a = 0; a < MI; a++)
 #pragma unroll
-            for (int b = 0; b < NI; b++) v[a][b] = double4_t{0., 0., 0., 0.};
+            for (int b = 0; b < NI; b++) v[a][b] = typename Mfma<T>::acc_t{0, 0, 0, 0};
     }
 };
 
-// Runs the K loop.  `lds` must hold TileShape<BM,BN>::LDS_DOUBLES doubles.
-template <int BM, int BN, class LA, class LB>
-__device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, double *lds,
-                                     Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI> &acc) {
+// Runs the K loop.  `lds` must hold TileShape<BM,BN>::LDS_ELEMS elements of T.
+template <int BM, int BN, class LA, class LB, typename T = double>
+__device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, T *lds,
+                                     Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI, T> &acc) {
     using S = TileShape<BM, BN>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    double *As[2] = {lds, lds + G_BK * S::PA};
-    double *Bs[2] = {lds + 2 * G_BK * S::PA, lds + 2 * G_BK * S::PA + G_BK * S::PB};
-    double ra[LA::N], rb[LB::N];
+    T *As[2] = {lds, lds + G_BK * S::PA};
+    T *Bs[2] = {lds + 2 * G_BK * S::PA, lds + 2 * G_BK * S::PA + G_BK * S::PB};
+    T ra[LA::N], rb[LB::N];
     if (nk <= 0) return;
     la.load(0, ra);
     lb.load(0, rb);
@@ -70,12 +95,12 @@ __device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, double *lds,
             la.load(kt + 1, ra);
             lb.load(kt + 1, rb);
         }
-        const double *A = As[cur] + wm * S::WM + (lane & 15);
-        const double *B = Bs[cur] + wn * S::WN + (lane & 15);
+        const T *A = As[cur] + wm * S::WM + (lane & 15);
+        const T *B = Bs[cur] + wn * S::WN + (lane & 15);
 #pragma unroll
         for (int kq = 0; kq < G_BK / 4; kq++) {
             const int kk = kq * 4 + (lane >> 4);
-            double a[S::MI], b[S::NI];
+            T a[S::MI], b[S::NI];
 #pragma unroll
             for (int mi = 0; mi < S::MI; mi++) a[mi] = A[kk * S::PA + mi * 16];
 #pragma unroll
@@ -83,7 +108,7 @@ __device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, double *lds,
 #pragma unroll
             for (int mi = 0; mi < S::MI; mi++)
 #pragma unroll
-                for (int ni = 0; ni < S::NI; ni++) acc.v[mi][ni] = mfma16(a[mi], b[ni], acc.v[mi][ni]);
+                for (int ni = 0; ni < S::NI; ni++) acc.v[mi][ni] = Mfma<T>::mma(a[mi], b[ni], acc.v[mi][ni]);
         }
         if (more) {
             la.store(ra, As[cur ^ 1], S::PA);
@@ -94,8 +119,8 @@ __device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, double *lds,
 }
 
 // Visits every accumulator element with its tile-local (row, col).
-template <int BM, int BN, class F>
-__device__ inline void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI> &acc, F &&f) {
+template <int BM, int BN, typename T = double, class F>
+__device__ inline void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI, T> &acc, F &&f) {
     using S = TileShape<BM, BN>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -106,31 +131,32 @@ __device__ inline void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, TileShape<
             const int c = wn * S::WN + ni * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int r = wm * S::WM + mi * 16 + (lane >> 4) + 4 * v;
+                const int r = wm * S::WM + mi * 16 + Mfma<T>::row(lane, v);
                 f(r, c, acc.v[mi][ni][v]);
             }
         }
 }
 
 // ---------------------------------------------------------------- loaders
-// Row-major matrix M[row][k] (k contiguous) as the A or B operand: tile rows
-// r0 .. r0+R-1, K-step kt covers k = kt*BK .. +BK-1.  Out-of-range reads 0.
-template <int R>
+// Row-major matrix M[row][k] (k contiguous, element type S) as the A or B
+// operand: tile rows r0 .. r0+R-1, K-step kt covers k = kt*BK .. +BK-1.
+// Out-of-range reads 0.
+template <int R, typename T = double, typename Src = double>
 struct RowsLoader {
     static constexpr int N = R * G_BK / G_THREADS;
     static_assert(N >= 1 && (G_THREADS / (G_BK / N)) == R, "RowsLoader shape");
-    const double *base;   // &M[0][0]
+    const Src *base;      // &M[0][0]
     long ld;              // row stride
     int r0, nrows, k0, kmax;
-    __device__ void load(int kt, double (&r)[N]) const {
+    __device__ void load(int kt, T (&r)[N]) const {
         const int t = threadIdx.x;
         const int row = r0 + t / (G_BK / N);
         const int kb = k0 + kt * G_BK + (t % (G_BK / N)) * N;
-        const double *src = base + (long)row * ld + kb;
+        const Src *src = base + (long)row * ld + kb;
 #pragma unroll
-        for (int v = 0; v < N; v++) r[v] = (row < nrows && kb + v < kmax) ? src[v] : 0.;
+        for (int v = 0; v < N; v++) r[v] = (row < nrows && kb + v < kmax) ? (T)src[v] : (T)0;
     }
-    __device__ void store(const double (&r)[N], double *lds, int pitch) const {
+    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;
         const int row = t / (G_BK / N);
         const int kb = (t % (G_BK / N)) * N;
@@ -141,24 +167,24 @@ struct RowsLoader {
 
 // Row-major matrix M[k][col] (col contiguous) as the B operand: tile columns
 // c0 .. c0+C-1.
-template <int C>
+template <int C, typename T = double, typename Src = double>
 struct ColsLoader {
     static constexpr int N = C * G_BK / G_THREADS;
-    const double *base;
+    const Src *base;
     long ld;
     int c0, ncols, k0, kmax;
-    __device__ void load(int kt, double (&r)[N]) const {
+    __device__ void load(int kt, T (&r)[N]) const {
         const int t = threadIdx.x;
         // each thread takes N consecutive columns of one k row
         constexpr int TPR = C / N;              // threads per k row
         const int kk = t / TPR;
         const int cb = c0 + (t % TPR) * N;
         const int k = k0 + kt * G_BK + kk;
-        const double *src = base + (long)k * ld + cb;
+        const Src *src = base + (long)k * ld + cb;
 #pragma unroll
-        for (int v = 0; v < N; v++) r[v] = (kk < G_BK && k < kmax && cb + v < ncols) ? src[v] : 0.;
+        for (int v = 0; v < N; v++) r[v] = (kk < G_BK && k < kmax && cb + v < ncols) ? (T)src[v] : (T)0;
     }
-    __device__ void store(const double (&r)[N], double *lds, int pitch) const {
+    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;
         constexpr int TPR = C / N;
         const int kk = t / TPR;

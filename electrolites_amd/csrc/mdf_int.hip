@@ -69,3 +69,29 @@ extern "C" int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const lo
     }
     return MDF_OK;
 }
+
+// Overlap, kinetic and nuclear-attraction matrices over the listed shell
+// pairs (both triangles written; screened pairs untouched).  The one-electron
+// part of the Fock matrix for the SCF driver (PySCF int1e_ovlp / int1e_kin /
+// int1e_nuc).
+extern "C" int mdf_int1e(const mdf_basis *ao, int nao, const int *pairs, const long *pair_class_off, int natm,
+                         const double *charges, const double *atom_xyz, double *S, double *T, double *V,
+                         void *stream) {
+    MDF_CHECK_ARG(ao && pair_class_off && S && T && V && (natm == 0 || (charges && atom_xyz)),
+                  "mdf_int1e: null argument");
+    MDF_CHECK_ARG(ao->lmax <= LMAX_AO, "mdf_int1e: orbital l %d > %d", ao->lmax, LMAX_AO);
+    hipStream_t st = as_stream(stream);
+    const BasisDev a = to_dev(ao);
+    MDF_CHECK_ARG(nao > 0, "mdf_int1e: nao must be positive");
+    for (int la = 0; la <= LMAX_AO; la++)
+        for (int lb = 0; lb <= LMAX_AO; lb++) {
+            const int cls = la * (LMAX_AO + 1) + lb;
+            const long q0 = pair_class_off[cls], q1 = pair_class_off[cls + 1];
+            if (q1 <= q0) continue;
+            MDF_CHECK_ARG(q1 - q0 < (1L << 31), "mdf_int1e: too many pairs in one class");
+            int rc = launch_int1e(la, lb, a, pairs + 2 * q0, (int)(q1 - q0), natm, charges, atom_xyz, S, T, V,
+                                  nao, st);
+            if (rc) return rc;
+        }
+    return MDF_OK;
+}

@@ -15,6 +15,13 @@
 // GEMM.  Step 2 splits the (L, k) reduction across workgroups (ksplit); the
 // per-split partials are summed in a fixed order by k_finalize, so results do
 // not depend on scheduling.
+//
+// Precision: prec = 0 runs both steps on FP64 MFMA; prec = 1 converts the
+// packed cderi and the factors to FP32 on the way into LDS, runs both steps
+// on FP32 MFMA with FP32 accumulation and stores R in FP32 (the mixed-
+// precision SCF cycles); the cross-split sum and the output stay FP64.
+#include <type_traits>
+
 #include "mdf_common.h"
 #include "mdf_gemm.h"
 #include "../../include/mi355df.h"
@@ -28,7 +35,7 @@ __device__ inline long tri(long i) { return i * (i + 1) / 2; }
 // ---------------------------------------------------------------- step 1
 // A operand of step 1: rows i0..i0+127 of the unpacked A_L, K-step kt covers
 // j = 16 kt .. 16 kt + 15.
-template <bool MAPPED>
+template <bool MAPPED, typename T = double>
 struct PackedALoader {
     static constexpr int N = 128 * G_BK / G_THREADS;     // 8
     static constexpr int TPJ = G_THREADS / G_BK;         // threads per j row (upper)
@@ -36,7 +43,7 @@ struct PackedALoader {
     const int *pmap;     // MAPPED: pair index of (i, j), -1 if not stored
     int nao, i0;
     int upper;           // layout used by the last load()
-    __device__ void load(int kt, double (&r)[N]) {
+    __device__ void load(int kt, T (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * G_BK;
         if (!MAPPED && j0 >= i0 + 127) {
             // every j > every i: (i, j) at tri(j) + i, contiguous in i
@@ -44,7 +51,7 @@ struct PackedALoader {
             const int j = j0 + t / TPJ, ib = i0 + (t % TPJ) * N;
             const double *src = row + tri(j) + ib;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? src[v] : 0.;
+            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? (T)src[v] : (T)0;
             return;
         }
         upper = 0;
@@ -53,7 +60,7 @@ struct PackedALoader {
             // every i >= every j: contiguous in j
             const double *src = row + tri(i) + jb;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (i < nao) ? src[v] : 0.;
+            for (int v = 0; v < N; v++) r[v] = (i < nao) ? (T)src[v] : (T)0;
             return;
         }
 #pragma unroll
@@ -68,10 +75,10 @@ struct PackedALoader {
                     x = row[i >= j ? tri(i) + j : tri(j) + i];
                 }
             }
-            r[v] = x;
+            r[v] = (T)x;
         }
     }
-    __device__ void store(const double (&r)[N], double *lds, int pitch) const {
+    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;
         if (upper) {
             const int jj = t / TPJ, ib = (t % TPJ) * N;
@@ -85,21 +92,21 @@ struct PackedALoader {
     }
 };
 
-template <int BN, bool MAPPED>
+template <int BN, bool MAPPED, typename T>
 __global__ __launch_bounds__(G_THREADS) void k_step1(
     const double *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao,
-    const double *__restrict__ F, long ldf, int nocc, double *__restrict__ R, long ldr,
+    const double *__restrict__ F, long ldf, int nocc, T *__restrict__ R, long ldr,
     long strideL) {
     using S = TileShape<128, BN>;
-    __shared__ double lds[S::LDS_DOUBLES];
+    __shared__ T lds[S::LDS_ELEMS];
     const int k0 = blockIdx.x * BN, i0 = blockIdx.y * 128;
-    PackedALoader<MAPPED> la{cderi + (long)blockIdx.z * ld, pmap, nao, i0, 0};
-    ColsLoader<BN> lb{F, ldf, k0, nocc, 0, nao};
-    Acc<S::MI, S::NI> acc;
+    PackedALoader<MAPPED, T> la{cderi + (long)blockIdx.z * ld, pmap, nao, i0, 0};
+    ColsLoader<BN, T, double> lb{F, ldf, k0, nocc, 0, nao};
+    Acc<S::MI, S::NI, T> acc;
     acc.zero();
     gemm_mainloop<128, BN>(la, lb, (nao + G_BK - 1) / G_BK, lds, acc);
-    double *Rl = R + (long)blockIdx.z * strideL;
-    gemm_epilogue<128, BN>(acc, [&](int r, int c, double v) {
+    T *Rl = R + (long)blockIdx.z * strideL;
+    gemm_epilogue<128, BN, T>(acc, [&](int r, int c, T v) {
         const int i = i0 + r, k = k0 + c;
         if (i < nao && k < nocc) Rl[(long)i * ldr + k] = v;
     });
@@ -108,39 +115,42 @@ __global__ __launch_bounds__(G_THREADS) void k_step1(
 // ---------------------------------------------------------------- step 2
 // Rows r0..r0+127 of R[L] (k contiguous); K-step kt of this split is
 // (L, kc) = divmod(s0 + kt, kchunks), columns kc*16 .. kc*16 + 15.
-template <bool VEC2>
+template <bool VEC, typename T>
 struct TransformedLoader {
     static constexpr int N = 128 * G_BK / G_THREADS;     // 8
-    const double *R;
+    static constexpr int EPV = 16 / (int)sizeof(T);       // elements per 16-byte load
+    using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    const T *R;
     long strideL, ldr;
     int r0, nao, nocc, kchunks;
     long L;              // current aux row of the next load()
     int kc;              // current k chunk of the next load()
     // thread t owns row t % 128 and k half t / 128, so a wavefront's LDS
     // stores cover 64 consecutive rows of one k (no bank conflicts)
-    __device__ void load(int, double (&r)[N]) {
+    __device__ void load(int, T (&r)[N]) {
         // called for kt = 0, 1, 2, ... in order: advance (L, kc) in place
         const int t = threadIdx.x;
         const int kb = kc * G_BK + (t >> 7) * N;
         const int row = r0 + (t & 127);
-        const double *src = R + L * strideL + (long)row * ldr + kb;
-        if (VEC2 && row < nao && kb + N <= nocc) {
+        const T *src = R + L * strideL + (long)row * ldr + kb;
+        if (VEC && row < nao && kb + N <= nocc) {
 #pragma unroll
-            for (int v = 0; v < N; v += 2) {
-                const double2 x = *reinterpret_cast<const double2 *>(src + v);
-                r[v] = x.x;
-                r[v + 1] = x.y;
+            for (int v = 0; v < N; v += EPV) {
+                const V x = *reinterpret_cast<const V *>(src + v);
+                const T *xe = reinterpret_cast<const T *>(&x);
+#pragma unroll
+                for (int e = 0; e < EPV; e++) r[v + e] = xe[e];
             }
         } else {
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (row < nao && kb + v < nocc) ? src[v] : 0.;
+            for (int v = 0; v < N; v++) r[v] = (row < nao && kb + v < nocc) ? src[v] : (T)0;
         }
         if (++kc == kchunks) {
             kc = 0;
             ++L;
         }
     }
-    __device__ void store(const double (&r)[N], double *lds, int pitch) const {
+    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x, ii = t & 127, kb = (t >> 7) * N;
 #pragma unroll
         for (int v = 0; v < N; v++) lds[(kb + v) * pitch + ii] = r[v];
@@ -149,13 +159,13 @@ struct TransformedLoader {
 
 constexpr int S2_B = 128;
 
-template <bool SYM, bool VEC2>
+template <bool SYM, bool VEC, typename T>
 __global__ __launch_bounds__(G_THREADS, 2) void k_step2(
-    const double *__restrict__ Rl, const double *__restrict__ Rr, int nao, int nocc,
+    const T *__restrict__ Rl, const T *__restrict__ Rr, int nao, int nocc,
     long ldr, long strideL, int nL, int ntile, double *__restrict__ part, long part_stride,
     int beta) {
     using S = TileShape<S2_B, S2_B>;
-    __shared__ double lds[S::LDS_DOUBLES];
+    __shared__ T lds[S::LDS_ELEMS];
     int ti, tj;
     if (SYM) {
         const int x = blockIdx.x;
@@ -173,17 +183,17 @@ __global__ __launch_bounds__(G_THREADS, 2) void k_step2(
     const long s0 = nsteps * blockIdx.y / gridDim.y, s1 = nsteps * (blockIdx.y + 1) / gridDim.y;
     const long Ls = s0 / kchunks;
     const int kcs = (int)(s0 - Ls * kchunks);
-    TransformedLoader<VEC2> la{Rl, strideL, ldr, i0, nao, nocc, kchunks, Ls, kcs};
-    TransformedLoader<VEC2> lb{Rr, strideL, ldr, j0, nao, nocc, kchunks, Ls, kcs};
-    Acc<S::MI, S::NI> acc;
+    TransformedLoader<VEC, T> la{Rl, strideL, ldr, i0, nao, nocc, kchunks, Ls, kcs};
+    TransformedLoader<VEC, T> lb{Rr, strideL, ldr, j0, nao, nocc, kchunks, Ls, kcs};
+    Acc<S::MI, S::NI, T> acc;
     acc.zero();
     gemm_mainloop<S2_B, S2_B>(la, lb, (int)(s1 - s0), lds, acc);
     double *o = part + (long)blockIdx.y * part_stride;
-    gemm_epilogue<S2_B, S2_B>(acc, [&](int r, int c, double v) {
+    gemm_epilogue<S2_B, S2_B, T>(acc, [&](int r, int c, T v) {
         const int i = i0 + r, j = j0 + c;
         if (i < nao && j < nao) {
             double *dst = o + (long)i * nao + j;
-            *dst = beta ? *dst + v : v;
+            *dst = beta ? *dst + (double)v : (double)v;
         }
     });
 }
@@ -265,43 +275,101 @@ KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows) {
     return p;
 }
 
-template <int BN>
+template <int BN, typename T>
 void launch_step1(dim3 g, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
-                  const double *f, long ldf, int nocc, double *r, long ldr, long strideL) {
+                  const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
     if (pmap)
-        hipLaunchKernelGGL((k_step1<BN, true>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
+        hipLaunchKernelGGL((k_step1<BN, true, T>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
                            nocc, r, ldr, strideL);
     else
-        hipLaunchKernelGGL((k_step1<BN, false>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
+        hipLaunchKernelGGL((k_step1<BN, false, T>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
                            nocc, r, ldr, strideL);
 }
 
+template <typename T>
 int step1(const KPlan &p, int nL, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
-          const double *f, long ldf, int nocc, double *r, long ldr, long strideL) {
+          const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
     dim3 g((unsigned)cdiv(nocc, p.bn), (unsigned)cdiv(nao, 128), (unsigned)nL);
     switch (p.bn) {
-    case 160: launch_step1<160>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    case 128: launch_step1<128>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    default: launch_step1<64>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 160: launch_step1<160, T>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 128: launch_step1<128, T>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    default: launch_step1<64, T>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     }
     MDF_LAUNCHED("k_step1");
     return MDF_OK;
 }
 
+template <typename T>
+int step2(const KPlan &p, int nL, bool two, hipStream_t st, const T *rl, const T *rr, int nao, int nocc,
+          long ldr, long strideL, double *part, long part_stride, int beta) {
+    dim3 g2((unsigned)p.ntiles_grid, (unsigned)p.ksplit);
+    const bool vec = (ldr % (16 / (long)sizeof(T)) == 0) && ((uintptr_t)rl % 16 == 0) && ((uintptr_t)rr % 16 == 0);
+    if (two && vec)
+        hipLaunchKernelGGL((k_step2<false, true, T>), g2, dim3(G_THREADS), 0, st, rl, rr, nao, nocc, ldr,
+                           strideL, nL, p.ntile, part, part_stride, beta);
+    else if (two)
+        hipLaunchKernelGGL((k_step2<false, false, T>), g2, dim3(G_THREADS), 0, st, rl, rr, nao, nocc, ldr,
+                           strideL, nL, p.ntile, part, part_stride, beta);
+    else if (vec)
+        hipLaunchKernelGGL((k_step2<true, true, T>), g2, dim3(G_THREADS), 0, st, rl, rl, nao, nocc, ldr,
+                           strideL, nL, p.ntile, part, part_stride, beta);
+    else
+        hipLaunchKernelGGL((k_step2<true, false, T>), g2, dim3(G_THREADS), 0, st, rl, rl, nao, nocc, ldr,
+                           strideL, nL, p.ntile, part, part_stride, beta);
+    MDF_LAUNCHED("k_step2");
+    return MDF_OK;
+}
+
+// The block loop of mdf_get_k for element type T.
+template <typename T>
+int run_k(const KPlan &p, const double *cderi, long naux, long ld, const int *pair_map, int nao,
+          const double *fl, const double *fr, int nocc, long ldf, double *vk, long ldvk, int accumulate,
+          void *work, hipStream_t st) {
+    const bool two = fr != nullptr;
+    double *part = (double *)work;
+    T *rl = (T *)(part + align_up(p.part_words, 64));
+    T *rr = two ? rl + align_up(p.r_words, 64) : rl;
+    const long ldr = nocc, strideL = (long)nao * nocc;
+    const long part_stride = (long)nao * nao;
+    for (long L0 = 0; L0 < naux; L0 += p.nL) {
+        const int nL = (int)(naux - L0 < p.nL ? naux - L0 : p.nL);
+        const double *blk = cderi + L0 * ld;
+        int rc = step1<T>(p, nL, st, blk, ld, pair_map, nao, fl, ldf, nocc, rl, ldr, strideL);
+        if (rc) return rc;
+        if (two) {
+            rc = step1<T>(p, nL, st, blk, ld, pair_map, nao, fr, ldf, nocc, rr, ldr, strideL);
+            if (rc) return rc;
+        }
+        rc = step2<T>(p, nL, two, st, rl, rr, nao, nocc, ldr, strideL, part, part_stride, L0 > 0);
+        if (rc) return rc;
+    }
+    long nt = (long)nao * nao;
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)cdiv(nt, 256L)), dim3(256), 0, st,
+                       part, p.ksplit, part_stride, nao, two ? 0 : 1, vk, ldvk, accumulate);
+    MDF_LAUNCHED("k_finalize");
+    return MDF_OK;
+}
+
 }  // namespace
 
-extern "C" size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_factors, long max_block_rows) {
+static size_t k_workspace(const KPlan &p, bool two, size_t elem) {
+    return align_up(p.part_words, 64) * sizeof(double) + (two ? 2 : 1) * align_up(p.r_words, 64) * elem;
+}
+
+extern "C" size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_factors, int prec,
+                                      long max_block_rows) {
     KPlan p = plan_k(naux, nao, nocc, two_factors != 0, max_block_rows);
-    return (align_up(p.part_words, 64) + (two_factors ? 2 : 1) * align_up(p.r_words, 64)) * sizeof(double);
+    return k_workspace(p, two_factors != 0, prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double));
 }
 
 extern "C" int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *pair_map,
-                         int nao, const double *fl, const double *fr, int nocc, long ldf,
+                         int nao, const double *fl, const double *fr, int nocc, long ldf, int prec,
                          double *vk, long ldvk, int accumulate, void *work, size_t work_bytes,
                          void *stream) {
     MDF_CHECK_ARG(naux >= 0 && nao >= 0 && nocc >= 0, "mdf_get_k: bad sizes");
     MDF_CHECK_ARG(cderi && fl && vk, "mdf_get_k: null pointer");
     MDF_CHECK_ARG(ld >= npairs && ldf >= nocc && ldvk >= nao, "mdf_get_k: leading dimension too small");
+    MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32, "mdf_get_k: unknown precision %d", prec);
     MDF_CHECK_ARG(pair_map || npairs == (long)nao * (nao + 1) / 2,
                   "mdf_get_k: without a pair map the cderi must hold the full packed triangle "
                   "(npairs=%ld, nao=%d)", npairs, nao);
@@ -314,54 +382,19 @@ extern "C" int mdf_get_k(const double *cderi, long naux, long npairs, long ld, c
         return MDF_OK;
     }
     MDF_CHECK_ARG(work != nullptr, "mdf_get_k: workspace is null");
+    const size_t elem = prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double);
     KPlan p0 = plan_k(naux, nao, nocc, two, 1);
-    size_t need1 = mdf_get_k_workspace(naux, nao, nocc, two, 1);
+    size_t need1 = k_workspace(p0, two, elem);
     if (work_bytes < need1) {
         set_error("mdf_get_k: workspace %zu bytes < %zu required", work_bytes, need1);
         return MDF_EWORKSPACE;
     }
-    size_t per_row = (size_t)nao * nocc * sizeof(double) * (two ? 2 : 1);
+    size_t per_row = (size_t)nao * nocc * elem * (two ? 2 : 1);
     size_t part_bytes = align_up(p0.part_words, 64) * sizeof(double);
     long rows = (long)((work_bytes - part_bytes) / per_row);
     KPlan p = plan_k(naux, nao, nocc, two, rows);
-    while (p.nL > 1 && mdf_get_k_workspace(naux, nao, nocc, two, p.nL) > work_bytes)
-        p = plan_k(naux, nao, nocc, two, p.nL - 1);
-
-    double *part = (double *)work;
-    double *rl = part + align_up(p.part_words, 64);
-    double *rr = two ? rl + align_up(p.r_words, 64) : rl;
-    const long ldr = nocc, strideL = (long)nao * nocc;
-    const long part_stride = (long)nao * nao;
-
-    for (long L0 = 0; L0 < naux; L0 += p.nL) {
-        const int nL = (int)(naux - L0 < p.nL ? naux - L0 : p.nL);
-        const double *blk = cderi + L0 * ld;
-        int rc = step1(p, nL, st, blk, ld, pair_map, nao, fl, ldf, nocc, rl, ldr, strideL);
-        if (rc) return rc;
-        if (two) {
-            rc = step1(p, nL, st, blk, ld, pair_map, nao, fr, ldf, nocc, rr, ldr, strideL);
-            if (rc) return rc;
-        }
-        dim3 g2((unsigned)p.ntiles_grid, (unsigned)p.ksplit);
-        const int beta = L0 > 0;
-        const bool vec2 = (ldr % 2 == 0) && ((uintptr_t)rl % 16 == 0) && ((uintptr_t)rr % 16 == 0);
-        if (two && vec2)
-            hipLaunchKernelGGL((k_step2<false, true>), g2, dim3(G_THREADS), 0, st, rl, rr, nao, nocc, ldr,
-                               strideL, nL, p.ntile, part, part_stride, beta);
-        else if (two)
-            hipLaunchKernelGGL((k_step2<false, false>), g2, dim3(G_THREADS), 0, st, rl, rr, nao, nocc, ldr,
-                               strideL, nL, p.ntile, part, part_stride, beta);
-        else if (vec2)
-            hipLaunchKernelGGL((k_step2<true, true>), g2, dim3(G_THREADS), 0, st, rl, rl, nao, nocc, ldr,
-                               strideL, nL, p.ntile, part, part_stride, beta);
-        else
-            hipLaunchKernelGGL((k_step2<true, false>), g2, dim3(G_THREADS), 0, st, rl, rl, nao, nocc, ldr,
-                               strideL, nL, p.ntile, part, part_stride, beta);
-        MDF_LAUNCHED("k_step2");
-    }
-    long nt = (long)nao * nao;
-    hipLaunchKernelGGL(k_finalize, dim3((unsigned)cdiv(nt, 256L)), dim3(256), 0, st,
-                       part, p.ksplit, part_stride, nao, two ? 0 : 1, vk, ldvk, accumulate);
-    MDF_LAUNCHED("k_finalize");
-    return MDF_OK;
+    while (p.nL > 1 && k_workspace(p, two, elem) > work_bytes) p = plan_k(naux, nao, nocc, two, p.nL - 1);
+    if (prec == MDF_PREC_FP32)
+        return run_k<float>(p, cderi, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
+    return run_k<double>(p, cderi, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
 }

@@ -10,6 +10,10 @@ HIP kernels behind libmi355df's C ABI:
 * K: ``mdf_get_k`` (packed-triangle unpack fused into an FP64-MFMA step-1
   transform, then a symmetric rank-k step 2).
 
+The exchange precision is a per-object knob, ``dfobj.k_precision`` ('fp64',
+the default, or 'fp32'), so the signature stays the reference's; the SCF
+driver lowers it for early cycles.
+
 ``dfobj`` is anything with the reference DF object's fields: ``_cderi``
 (per-device mapping to a ``(naux, npairs)`` float64 device array),
 ``_cderi_idx = (pair_addresses, diag_idx)``, ``build()``.  This package's own
@@ -210,22 +214,28 @@ def _vj(cderi, lay, dms3, hermi):
     return vj
 
 
-def _vk(cderi, lay, fl, fr):
-    """``vk[n] = sum_L (A_L fl[n])(A_L fr[n])^T`` on the HIP path."""
+PRECISIONS = {'fp64': 0, 'fp32': 1}
+
+
+def _vk(cderi, lay, fl, fr, precision='fp64'):
+    """``vk[n] = sum_L (A_L fl[n])(A_L fr[n])^T`` on the HIP path.
+    ``precision`` 'fp64' or 'fp32' (FP32 MFMA for both steps)."""
     t = _dev.torch()
     n, nao, nocc = fl.shape
     naux, npairs = cderi.shape
+    prec = PRECISIONS[precision]
     vk = t.empty((n, nao, nao), dtype=t.float64, device=fl.device)
     two = fr is not None
-    per_row = nao * nocc * 8 * (2 if two else 1)
+    elem = 4 if prec == 1 else 8
+    per_row = nao * nocc * elem * (2 if two else 1)
     rows = max(1, min(naux, K_WORKSPACE_BYTES // max(per_row, 1)))
-    nbytes = _lib.query('mdf_get_k_workspace', naux, nao, nocc, int(two), rows)
+    nbytes = _lib.query('mdf_get_k_workspace', naux, nao, nocc, int(two), prec, rows)
     work = _dev.WORKSPACE.get(nbytes)
     for i in range(n):
         STATS['k_calls'] += 1
         _lib.call('mdf_get_k', _dev.ptr(cderi), naux, npairs, cderi.stride(0),
                   _dev.ptr(lay.map), nao, _dev.ptr(fl[i]),
-                  _dev.ptr(fr[i]) if two else None, nocc, nocc,
+                  _dev.ptr(fr[i]) if two else None, nocc, nocc, prec,
                   _dev.ptr(vk[i]), nao, 0, _dev.ptr(work), work.numel(),
                   _dev.stream_ptr())
     return vk
@@ -297,7 +307,11 @@ def _get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
         raise ValueError(f'{fl.shape[0]} density factors for '
                          f'{dms3.shape[0]} densities')
     vj = _vj(cderi, lay, dms3, hermi) if with_j else None
-    vk = _vk(cderi, lay, fl, fr)
+    prec = getattr(dfobj, 'k_precision', 'fp64') or 'fp64'
+    if prec not in PRECISIONS:
+        raise ValueError(f'k_precision {prec!r}: expected one of {sorted(PRECISIONS)}')
+    STATS['k_' + prec] = STATS.get('k_' + prec, 0) + 1
+    vk = _vk(cderi, lay, fl, fr, prec)
     return ((None if vj is None else _finish(vj, dms, out_device)),
             _finish(vk, dms, out_device))
 

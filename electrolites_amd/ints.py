@@ -140,3 +140,17 @@ def int2c2e(aux, out=None):
     _lib.call('mdf_int2c2e', ctypes.byref(aux.desc), _dev.ptr(aux.sorted),
               _lp(aux.l_off), _dev.ptr(out), out.stride(0), _dev.stream_ptr())
     return out
+
+
+def int1e(ao, mol, bound=None):
+    """``(S, T, V)`` device matrices of the orbital basis of ``mol``."""
+    t = _dev.torch()
+    n = ao.nao
+    tasks = PairTasks(mol.ao, 0, mol.ao.nbas, bound)
+    mats = [t.zeros((n, n), dtype=t.float64, device=_dev.device()) for _ in range(3)]
+    z = _dev.to_device(mol.atom_charges())
+    xyz = _dev.to_device(mol.atom_coords().reshape(-1))
+    _lib.call('mdf_int1e', ctypes.byref(ao.desc), n, _dev.ptr(tasks.dev), _lp(tasks.class_off),
+              len(mol.symbols), _dev.ptr(z), _dev.ptr(xyz), _dev.ptr(mats[0]), _dev.ptr(mats[1]),
+              _dev.ptr(mats[2]), _dev.stream_ptr())
+    return tuple(mats)

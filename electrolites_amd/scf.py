@@ -1,0 +1,147 @@
+"""Device-resident DF-RHF driver on the MI355X DF path.
+
+The SCF loop PySCF/GPU4PySCF runs around ``df_jk.get_jk`` (the reference's
+only caller of the hot path, gpu4pyscf ``scf.hf.kernel`` -> ``_DFHF.get_veff``
+-> ``DF.get_jk``), restated with every matrix kept in HBM:
+
+* one-electron Hamiltonian ``H = T + V`` and overlap from ``mdf_int1e``;
+* the cderi from the GPU build (:mod:`electrolites_amd.fastdfcderi`);
+* per cycle: ``J, K`` from :func:`electrolites_amd.fastdf.get_jk` on a
+  ``mo_coeff``-tagged density (one-factor exchange), ``F = H + J - K/2``,
+  ``E = tr(D (H + F)) / 2 + E_nuc``, DIIS on the orthogonalised ``FDS - SDF``,
+  and a FP64 diagonalisation of ``X^T F X`` (``X = S^-1/2``, canonical).
+
+The initial guess is the core Hamiltonian (PySCF's MINAO guess needs a basis
+set that is not available offline; the converged energy does not depend on
+the guess).
+
+Precision schedule (``k_precision``): 'fp64' runs every cycle in FP64;
+'mixed' builds K (step 1, the occupied transform, and step 2) on FP32 MFMA
+while the DIIS error is above ``switch_err`` (1e-5), then restarts DIIS and
+finishes in FP64.  J, the metric, the diagonalisation and the energy sums are
+FP64 throughout.  The per-cycle precision is recorded in ``history``.
+"""
+import math
+
+import numpy as np
+
+from . import _dev
+from . import fastdf
+from . import ints
+from .df import DF
+
+
+class RHF:
+    def __init__(self, mol, conv_tol=1e-10, conv_tol_grad=None, max_cycle=100, diis_space=8,
+                 df=None, verbose=0, k_precision='fp64', switch_err=1e-5):
+        if k_precision not in ('fp64', 'fp32', 'mixed'):
+            raise ValueError(f'k_precision {k_precision!r}')
+        self.k_precision = k_precision
+        self.switch_err = switch_err
+        self.mol = mol
+        self.conv_tol = conv_tol
+        self.conv_tol_grad = conv_tol_grad if conv_tol_grad is not None else math.sqrt(conv_tol)
+        self.max_cycle = max_cycle
+        self.diis_space = diis_space
+        self.with_df = df or DF(mol)
+        self.verbose = verbose
+        self.e_tot = None
+        self.converged = False
+        self.cycles = 0
+        self.history = []
+
+    # ------------------------------------------------------------ setup
+    def build(self):
+        t = _dev.torch()
+        _dev.require_gpu()
+        mol = self.mol
+        if self.with_df._cderi is None:
+            self.with_df.build()
+        ao = getattr(mol, '_mdf_dev_ao', None) or ints.DeviceBasis(mol.ao)
+        mol._mdf_dev_ao = ao
+        bound = getattr(mol, '_mdf_pair_bound', None)
+        if bound is None:
+            bound = ints.pair_bound(mol.ao)
+            mol._mdf_pair_bound = bound
+        self.S, T, V = ints.int1e(ao, mol, bound)
+        self.H = T + V
+        w, v = t.linalg.eigh(self.S)
+        keep = w > 1e-9
+        self.X = (v[:, keep] / t.sqrt(w[keep])).contiguous()
+        self.enuc = mol.energy_nuc()
+        self.nocc = mol.nocc
+        return self
+
+    def _density(self, C):
+        t = _dev.torch()
+        nocc = self.nocc
+        co = C[:, :nocc].contiguous()
+        dm = 2. * co @ co.T
+        occ = t.full((nocc,), 2., dtype=t.float64, device=C.device)
+        return fastdf.tag_array(dm, mo_coeff=co, mo_occ=occ)
+
+    def _diag(self, F):
+        t = _dev.torch()
+        e, c = t.linalg.eigh(self.X.T @ F @ self.X)
+        return e, self.X @ c
+
+    # ------------------------------------------------------------- loop
+    def kernel(self, dm0=None):
+        t = _dev.torch()
+        if not hasattr(self, 'H'):
+            self.build()
+        if dm0 is None:
+            _, C = self._diag(self.H)
+            dm = self._density(C)
+        else:
+            dm = dm0
+        focks, errs = [], []
+        e_old = 0.
+        self.converged = False
+        prec = 'fp64' if self.k_precision == 'fp64' else 'fp32'
+        for cycle in range(1, self.max_cycle + 1):
+            self.with_df.k_precision = prec
+            vj, vk = self.with_df.get_jk(dm, hermi=1)
+            F = self.H + vj - .5 * vk
+            e_tot = float(.5 * t.sum(dm * (self.H + F))) + self.enuc
+            err = F @ dm @ self.S
+            err = self.X.T @ (err - err.T) @ self.X
+            focks.append(F)
+            errs.append(err)
+            focks, errs = focks[-self.diis_space:], errs[-self.diis_space:]
+            n = len(focks)
+            if n > 1:
+                B = -t.ones((n + 1, n + 1), dtype=t.float64, device=F.device)
+                B[n, n] = 0.
+                E = t.stack([e.reshape(-1) for e in errs])
+                B[:n, :n] = E @ E.T
+                rhs = t.zeros(n + 1, dtype=t.float64, device=F.device)
+                rhs[n] = -1.
+                coef = t.linalg.solve(B, rhs)[:n]
+                F = t.einsum('i,ijk->jk', coef, t.stack(focks))
+            err_max = float(err.abs().max())
+            de = e_tot - e_old
+            e_old = e_tot
+            self.history.append({'cycle': cycle, 'e_tot': e_tot, 'de': de, 'err': err_max,
+                                 'k_precision': prec})
+            if self.verbose:
+                print(f'cycle {cycle}: E = {e_tot:.12f}  dE = {de:.3e}  |err| = {err_max:.3e}')
+            self.mo_energy, self.mo_coeff = self._diag(F)
+            dm = self._density(self.mo_coeff)
+            if prec != 'fp64' and self.k_precision == 'mixed' and err_max < self.switch_err:
+                prec = 'fp64'               # finishing cycles in FP64, fresh DIIS
+                focks, errs = [], []
+                continue
+            if prec == 'fp64' and abs(de) < self.conv_tol and err_max < self.conv_tol_grad:
+                self.converged = True
+                break
+            if prec != 'fp64' and abs(de) < self.conv_tol and err_max < self.conv_tol_grad:
+                self.converged = True       # an all-FP32 run converges in FP32
+                break
+        self.cycles = cycle
+        self.with_df.k_precision = 'fp64'
+        vj, vk = self.with_df.get_jk(dm, hermi=1)
+        F = self.H + vj - .5 * vk
+        self.e_tot = float(.5 * t.sum(dm * (self.H + F))) + self.enuc
+        self.dm = dm
+        return self.e_tot

@@ -25,7 +25,11 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 1
+#define MDF_ABI_VERSION 2
+
+/* arithmetic precision of a contraction */
+#define MDF_PREC_FP64 0
+#define MDF_PREC_FP32 1
 
 /* ---- library / device ------------------------------------------------- */
 int mdf_abi_version(void);
@@ -57,11 +61,16 @@ int mdf_get_j(const double *cderi, long naux, long npairs, long ld,
  *             when the cderi holds the full packed triangle
  *   fl, fr    (nao, nocc) density factors, row stride ldf
  *   vk        (nao, nao) out, row stride ldvk; accumulate != 0 adds.
+ * prec: MDF_PREC_FP64, or MDF_PREC_FP32 (both steps on FP32 MFMA with FP32
+ *   accumulation, R stored in FP32; inputs and vk stay FP64) for the
+ *   mixed-precision SCF cycles -- the reference's only narrow mode is FP32
+ *   *storage* widened before arithmetic (electrolites/_dffp32.py:166-196).
  * max_block_rows bounds the aux rows transformed per pass (workspace size).
  */
-size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_factors, long max_block_rows);
+size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_factors, int prec,
+                           long max_block_rows);
 int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *pair_map,
-              int nao, const double *fl, const double *fr, int nocc, long ldf,
+              int nao, const double *fl, const double *fr, int nocc, long ldf, int prec,
               double *vk, long ldvk, int accumulate, void *work, size_t work_bytes,
               void *stream);
 
@@ -119,6 +128,16 @@ int mdf_int3c2e(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
  * Replaces electrolites/_dfj2c.py:234-262 (int2c2e). */
 int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_off,
                 double *out, long ldo, void *stream);
+
+/* Overlap, kinetic and nuclear attraction (natm point charges, charges[] and
+ * atom_xyz[3*natm] device arrays) over the listed shell pairs, written to
+ * both triangles of the (nao, nao) outputs; pairs as for mdf_int3c2e (screened
+ * pairs untouched: zero-fill first).  The one-electron Hamiltonian of the SCF
+ * driver (PySCF int1e_ovlp, int1e_kin, int1e_nuc); no reference counterpart
+ * on the DF path itself. */
+int mdf_int1e(const mdf_basis *ao, int nao, const int *pairs, const long *pair_class_off, int natm,
+              const double *charges, const double *atom_xyz, double *S, double *T, double *V,
+              void *stream);
 
 /* ---- FP64 dense linear algebra of the metric (MFMA tile engine) --------
  * Replace the cuSOLVER/cuBLAS calls of electrolites/fastdfcderi.py:

@@ -35,4 +35,5 @@ def test_binding_covers_header():
 def test_workspace_queries_are_host_only():
     from electrolites_amd import _lib
     assert _lib.query('mdf_get_j_workspace', 7000, 1180416, 1) > 0
-    assert _lib.query('mdf_get_k_workspace', 7000, 1536, 320, 0, 256) > 256 * 1536 * 320 * 8
+    assert _lib.query('mdf_get_k_workspace', 7000, 1536, 320, 0, 0, 256) > 256 * 1536 * 320 * 8
+    assert _lib.query('mdf_get_k_workspace', 7000, 1536, 320, 0, 1, 256) > 256 * 1536 * 320 * 4

@@ -105,4 +105,4 @@ def test_narrow_tensor_never_falls_through(monkeypatch):
 def test_library_error_message():
     from electrolites_amd import _lib
     with pytest.raises(_lib.MDFError, match='bad sizes'):
-        _lib.call('mdf_get_k', None, -1, 0, 0, None, 0, None, None, 0, 0, None, 0, 0, None, 0, None)
+        _lib.call('mdf_get_k', None, -1, 0, 0, None, 0, None, None, 0, 0, 0, None, 0, 0, None, 0, None)

@@ -182,3 +182,22 @@ def test_device_in_device_out(water2):
     assert vj.is_cuda and vk.is_cuda
     rj, rk = O.get_jk(cderi, pa, 2 * c @ c.T)
     assert rel(vj.cpu().numpy(), rj) < TOL and rel(vk.cpu().numpy(), rk) < TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('nao,naux,nocc', [(64, 130, 17), (200, 611, 90)])
+def test_fp32_k_error(nao, naux, nocc):
+    """FP32 MFMA exchange: within FP32 rounding of the FP64 result
+    (relative 1e-5 bar; measured error reported by the mixed SCF)."""
+    from electrolites_amd import fastdf
+    rng = np.random.default_rng(nao + 7)
+    pa, _ = O.pair_addresses(nao)
+    cderi = rng.standard_normal((naux, len(pa))) * .1
+    c = rand_psd(nao, nocc, rng)
+    dm = fastdf.tag_array(2 * c @ c.T, mo_coeff=c, mo_occ=np.full(nocc, 2.))
+    df = FakeDF(cderi, pa, nao)
+    df.k_precision = 'fp32'
+    _, vk32 = fastdf.get_jk(df, dm, hermi=1)
+    rk = O.get_k_factor(cderi, pa, c * np.sqrt(2.))
+    err = rel(vk32, rk)
+    assert 1e-12 < err < 1e-5, err
