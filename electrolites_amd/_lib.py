@@ -42,6 +42,10 @@ _SIGS = {
     'mdf_trtri_workspace': (SZ, [L]),
     'mdf_trtri': (I, [P, L, L, P, P, L, P, SZ, P]),
     'mdf_trmm': (I, [P, L, L, P, L, L, P, L, P]),
+    'mdf_grouped_plan_create': (I, [P, I, I, I, I, I, ctypes.POINTER(P), ctypes.POINTER(SZ)]),
+    'mdf_grouped_jk': (I, [P, P, SZ, P]),
+    'mdf_grouped_plan_destroy': (I, [P]),
+    'mdf_grouped_plan_info': (I, [P, ctypes.POINTER(L), I]),
 }
 
 

@@ -1,0 +1,393 @@
+// Grouped J/K over many (molecule, density) items in persistent launches.
+//
+// The reference has no batched path: each get_jk call contracts one
+// molecule's cderi (electrolites/fastdf.py:805-1026).  Many small molecules
+// leave most of an MI355X idle, so this entry contracts a whole batch at once:
+// a plan flattens every item's work into tile tables (J pass-1 tiles, J pass-2
+// tiles, K step-1 tiles, K step-2 tiles and the fixed-order reductions), each
+// phase is ONE persistent grid-stride launch over its table, and a run is
+// kernel launches only (tables and item descriptors live in device memory the
+// plan owns), so a run can be captured in a HIP graph and replayed.
+//
+// Each item is cut exactly as the single-item entries cut it (mdf_jcore.h /
+// mdf_kcore.h plans depend on the item's own shape only) and runs the same
+// tile bodies, so an item's J and K are bit-identical alone and in a batch.
+#include <cstring>
+#include <vector>
+
+#include "mdf_common.h"
+#include "mdf_gemm.h"
+#include "mdf_jcore.h"
+#include "mdf_kcore.h"
+#include "../../include/mi355df.h"
+
+using namespace mdf;
+
+namespace {
+
+constexpr int GJ_NDM = 8;   // densities per molecule in one grouped J pass
+
+// Per-molecule J work on the device.
+struct GJItem {
+    const double *cderi;
+    long ld, naux, npairs;
+    const double *d;        // (ndm, npairs)
+    double *rho, *vj;       // (ndm, naux), (ndm, npairs)
+    double *part1, *part2;  // workspace
+    long nchunk, ngroup, rows_per_group, ncblock;
+    int ndm, vec;
+};
+
+// Per-(molecule, density) K work on the device.
+struct GKItem {
+    const double *cderi;
+    long ld, naux;
+    const int *pmap;
+    const double *fl, *fr;  // (nao, nocc); fr NULL -> symmetric
+    double *vk;             // (nao, nao)
+    void *rl, *rr;          // transformed blocks (T)
+    double *part;           // ksplit x nao x nao
+    int nao, nocc, ksplit, ntile, vec;
+};
+
+struct Tile {
+    int item, a, b, c;
+};
+
+}  // namespace
+
+struct mdf_grouped_plan {
+    int with_j, with_k, prec, bn;
+    int nj, nk;                         // items
+    long t_j1, t_jr1, t_j2, t_jr2;      // tile counts
+    long t_k1, t_k2, t_kf;
+    void *dev;                          // owned device memory: items + tables
+    GJItem *jitems;
+    GKItem *kitems;
+    Tile *j1, *jr1, *j2, *jr2, *k1, *k2, *kf;
+    size_t work_bytes;
+    // host copies of the per-item workspace offsets (relative to work)
+    std::vector<GJItem> hj;
+    std::vector<GKItem> hk;
+};
+
+namespace {
+
+constexpr long RED_CHUNK = 4096;    // elements per reduction tile
+
+__global__ __launch_bounds__(jcore::J_THREADS) void gj_pass1(const GJItem *items, const Tile *tiles, long n,
+                                                             char *work) {
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];
+        const GJItem &it = items[e.item];
+        double *part = (double *)(work + (size_t)it.part1);
+        if (it.vec == 2)
+            jcore::rho_partial_tile<GJ_NDM, 2>(it.cderi, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part,
+                                               e.a, e.b);
+        else
+            jcore::rho_partial_tile<GJ_NDM, 1>(it.cderi, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part,
+                                               e.a, e.b);
+    }
+}
+
+__global__ __launch_bounds__(jcore::J_THREADS) void gj_reduce1(const GJItem *items, const Tile *tiles, long n,
+                                                               char *work) {
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];
+        const GJItem &it = items[e.item];
+        const long e0 = (long)e.a * RED_CHUNK, e1 = min(e0 + RED_CHUNK, (long)it.ndm * it.naux);
+        jcore::rho_reduce_range((const double *)(work + (size_t)it.part1), it.nchunk, it.ndm, it.naux, it.rho,
+                                it.naux, e0, e1);
+    }
+}
+
+__global__ __launch_bounds__(jcore::J_THREADS) void gj_pass2(const GJItem *items, const Tile *tiles, long n,
+                                                             char *work) {
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];
+        const GJItem &it = items[e.item];
+        const long r0 = (long)e.b * it.rows_per_group, r1 = min(r0 + it.rows_per_group, it.naux);
+        double *out = (double *)(work + (size_t)it.part2) + (long)e.b * it.ndm * it.npairs;
+        if (it.vec == 2)
+            jcore::vj_pass_tile<GJ_NDM, 2>(it.cderi, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out,
+                                           it.npairs, e.a);
+        else
+            jcore::vj_pass_tile<GJ_NDM, 1>(it.cderi, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out,
+                                           it.npairs, e.a);
+    }
+}
+
+__global__ __launch_bounds__(jcore::J_THREADS) void gj_reduce2(const GJItem *items, const Tile *tiles, long n,
+                                                               char *work) {
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];
+        const GJItem &it = items[e.item];
+        const long e0 = (long)e.a * RED_CHUNK, e1 = min(e0 + RED_CHUNK, (long)it.ndm * it.npairs);
+        jcore::vj_reduce_range((const double *)(work + (size_t)it.part2), it.ngroup, it.ndm, it.npairs, it.vj,
+                               it.npairs, 0, e0, e1);
+    }
+}
+
+template <int BN, typename T>
+__global__ __launch_bounds__(G_THREADS) void gk_step1(const GKItem *items, const Tile *tiles, long n, char *work) {
+    __shared__ T lds[TileShape<128, BN>::LDS_ELEMS];
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];            // (item, L, i-tile, k-tile | right-factor flag << 16)
+        const GKItem &it = items[e.item];
+        const bool right = (e.c >> 16) != 0;
+        const int ktile = e.c & 0xffff;
+        const double *F = right ? it.fr : it.fl;
+        T *R = (T *)(work + (size_t)(right ? it.rr : it.rl)) + (long)e.a * it.nao * it.nocc;
+        const double *row = it.cderi + (long)e.a * it.ld;
+        if (it.pmap)
+            kcore::step1_tile<BN, true, T>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
+                                           ktile * BN, lds);
+        else
+            kcore::step1_tile<BN, false, T>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
+                                            ktile * BN, lds);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(G_THREADS, 2) void gk_step2(const GKItem *items, const Tile *tiles, long n,
+                                                         char *work) {
+    __shared__ T lds[TileShape<kcore::S2_B, kcore::S2_B>::LDS_ELEMS];
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];            // (item, tile index, split)
+        const GKItem &it = items[e.item];
+        int ti, tj;
+        if (it.fr == nullptr) {
+            kcore::tri_tile(e.a, ti, tj);
+        } else {
+            ti = e.a / it.ntile;
+            tj = e.a % it.ntile;
+        }
+        const int kchunks = (it.nocc + G_BK - 1) / G_BK;
+        const long nsteps = it.naux * kchunks;
+        const long s0 = nsteps * e.b / it.ksplit, s1 = nsteps * (e.b + 1) / it.ksplit;
+        const T *rl = (const T *)(work + (size_t)it.rl);
+        const T *rr = it.fr ? (const T *)(work + (size_t)it.rr) : rl;
+        double *o = (double *)(work + (size_t)it.part) + (long)e.b * it.nao * it.nao;
+        if (it.vec)
+            kcore::step2_tile<true, T>(rl, rr, it.nao, it.nocc, it.nocc, (long)it.nao * it.nocc, ti * kcore::S2_B,
+                                       tj * kcore::S2_B, s0, s1, o, 0, lds);
+        else
+            kcore::step2_tile<false, T>(rl, rr, it.nao, it.nocc, it.nocc, (long)it.nao * it.nocc, ti * kcore::S2_B,
+                                        tj * kcore::S2_B, s0, s1, o, 0, lds);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void gk_finalize(const GKItem *items, const Tile *tiles, long n, char *work) {
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];
+        const GKItem &it = items[e.item];
+        const long nn = (long)it.nao * it.nao;
+        const long e0 = (long)e.a * RED_CHUNK, e1 = min(e0 + RED_CHUNK, nn);
+        kcore::finalize_range((const double *)(work + (size_t)it.part), it.ksplit, nn, it.nao, it.fr == nullptr,
+                              it.vk, it.nao, 0, e0, e1);
+    }
+}
+
+int persistent_grid(long ntiles, int per_cu) {
+    int dev = 0, cu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+    long g = (long)cu * per_cu;
+    if (g > ntiles) g = ntiles;
+    return (int)(g > 0 ? g : 1);
+}
+
+}  // namespace
+
+extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int with_j, int with_k, int prec,
+                                       int bn, mdf_grouped_plan **out, size_t *work_bytes) {
+    MDF_CHECK_ARG(items && nitems >= 0 && out && work_bytes, "mdf_grouped_plan_create: null argument");
+    MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32, "mdf_grouped_plan_create: precision %d", prec);
+    MDF_CHECK_ARG(bn == 0 || bn == 64 || bn == 128 || bn == 160, "mdf_grouped_plan_create: bn %d not in {0,64,128,160}", bn);
+    const size_t elem = prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double);
+    mdf_grouped_plan *p = new mdf_grouped_plan();
+    p->with_j = with_j;
+    p->with_k = with_k;
+    p->prec = prec;
+    std::vector<Tile> j1, jr1, j2, jr2, k1, k2, kf;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off = align_up(off + bytes, 256); return o; };
+    int maxocc = 0;
+    for (int m = 0; m < nitems; m++) maxocc = items[m].nocc > maxocc ? items[m].nocc : maxocc;
+    p->bn = bn ? bn : kcore::pick_bn(maxocc > 0 ? maxocc : 1);
+    for (int m = 0; m < nitems; m++) {
+        const mdf_jk_item &x = items[m];
+        if (x.n_dm < 1 || x.n_dm > GJ_NDM || x.naux < 0 || x.nao < 0 || x.ld < x.npairs || !x.cderi ||
+            (!x.pair_map && x.npairs != (long)x.nao * (x.nao + 1) / 2)) {
+            set_error("mdf_grouped_plan_create: item %d invalid (n_dm %d, shapes, or pair map)", m, x.n_dm);
+            delete p;
+            return MDF_EINVAL;
+        }
+        if (with_j) {
+            if (!x.dm_pairs || !x.rho || !x.vj_pairs) {
+                set_error("mdf_grouped_plan_create: item %d lacks J buffers", m);
+                delete p;
+                return MDF_EINVAL;
+            }
+            GJItem g{};
+            g.cderi = x.cderi; g.ld = x.ld; g.naux = x.naux; g.npairs = x.npairs;
+            g.d = x.dm_pairs; g.rho = x.rho; g.vj = x.vj_pairs; g.ndm = x.n_dm;
+            g.vec = ((x.ld % 2 == 0) && (x.npairs % 2 == 0) && ((uintptr_t)x.cderi % 16 == 0) &&
+                     ((uintptr_t)x.dm_pairs % 16 == 0)) ? 2 : 1;
+            jcore::JPlan pl = jcore::plan_j(x.naux, x.npairs, x.n_dm, g.vec);
+            g.nchunk = pl.nchunk; g.ngroup = pl.ngroup; g.rows_per_group = pl.rows_per_group; g.ncblock = pl.ncblock;
+            g.part1 = (double *)take(pl.part1_words * sizeof(double));
+            g.part2 = (double *)take(pl.part2_words * sizeof(double));
+            const int ji = (int)p->hj.size();
+            for (long c = 0; c < pl.nchunk; c++)
+                for (long r = 0; r < pl.nrblock; r++) j1.push_back({ji, (int)c, (int)r, 0});
+            for (long e = 0; e < cdiv((long)x.n_dm * x.naux, RED_CHUNK); e++) jr1.push_back({ji, (int)e, 0, 0});
+            for (long g2 = 0; g2 < pl.ngroup; g2++)
+                for (long c = 0; c < pl.ncblock; c++) j2.push_back({ji, (int)c, (int)g2, 0});
+            for (long e = 0; e < cdiv((long)x.n_dm * x.npairs, RED_CHUNK); e++) jr2.push_back({ji, (int)e, 0, 0});
+            p->hj.push_back(g);
+        }
+        if (with_k && x.nocc > 0 && x.naux > 0) {
+            if (!x.fl || !x.vk) {
+                set_error("mdf_grouped_plan_create: item %d lacks K buffers", m);
+                delete p;
+                return MDF_EINVAL;
+            }
+            const bool two = x.fr != nullptr;
+            kcore::KPlan kp = kcore::plan_k(x.naux, x.nao, x.nocc, two, x.naux);
+            for (int d = 0; d < x.n_dm; d++) {
+                GKItem g{};
+                g.cderi = x.cderi; g.ld = x.ld; g.naux = x.naux; g.pmap = x.pair_map;
+                g.fl = x.fl + (long)d * x.nao * x.nocc;
+                g.fr = two ? x.fr + (long)d * x.nao * x.nocc : nullptr;
+                g.vk = x.vk + (long)d * x.nao * x.nao;
+                g.nao = x.nao; g.nocc = x.nocc; g.ksplit = kp.ksplit; g.ntile = kp.ntile;
+                g.vec = (x.nocc % (16 / (int)elem)) == 0;
+                g.rl = (void *)take(kp.r_words * elem);
+                g.rr = two ? (void *)take(kp.r_words * elem) : nullptr;
+                g.part = (double *)take(kp.part_words * sizeof(double));
+                const int ki = (int)p->hk.size();
+                const int nkt = (int)cdiv(x.nocc, p->bn), nit = (int)cdiv(x.nao, 128);
+                for (int side = 0; side < (two ? 2 : 1); side++)
+                    for (long L = 0; L < x.naux; L++)
+                        for (int it = 0; it < nit; it++)
+                            for (int kt = 0; kt < nkt; kt++) k1.push_back({ki, (int)L, it, kt | (side << 16)});
+                for (int s = 0; s < kp.ksplit; s++)
+                    for (int tl = 0; tl < kp.ntiles_grid; tl++) k2.push_back({ki, tl, s, 0});
+                for (long e = 0; e < cdiv((long)x.nao * x.nao, RED_CHUNK); e++) kf.push_back({ki, (int)e, 0, 0});
+                p->hk.push_back(g);
+            }
+        }
+    }
+    p->nj = (int)p->hj.size();
+    p->nk = (int)p->hk.size();
+    p->t_j1 = j1.size(); p->t_jr1 = jr1.size(); p->t_j2 = j2.size(); p->t_jr2 = jr2.size();
+    p->t_k1 = k1.size(); p->t_k2 = k2.size(); p->t_kf = kf.size();
+    p->work_bytes = off + 256;
+    // device block: items then tables
+    size_t dsz = 0;
+    auto dtake = [&](size_t bytes) { size_t o = dsz; dsz = align_up(dsz + bytes, 256); return o; };
+    const size_t o_j = dtake(p->hj.size() * sizeof(GJItem)), o_k = dtake(p->hk.size() * sizeof(GKItem));
+    const size_t o_j1 = dtake(j1.size() * sizeof(Tile)), o_jr1 = dtake(jr1.size() * sizeof(Tile));
+    const size_t o_j2 = dtake(j2.size() * sizeof(Tile)), o_jr2 = dtake(jr2.size() * sizeof(Tile));
+    const size_t o_k1 = dtake(k1.size() * sizeof(Tile)), o_k2 = dtake(k2.size() * sizeof(Tile));
+    const size_t o_kf = dtake(kf.size() * sizeof(Tile));
+    std::vector<char> host(dsz + 256, 0);
+    auto put = [&](size_t o, const void *src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
+    put(o_j, p->hj.data(), p->hj.size() * sizeof(GJItem));
+    put(o_k, p->hk.data(), p->hk.size() * sizeof(GKItem));
+    put(o_j1, j1.data(), j1.size() * sizeof(Tile));
+    put(o_jr1, jr1.data(), jr1.size() * sizeof(Tile));
+    put(o_j2, j2.data(), j2.size() * sizeof(Tile));
+    put(o_jr2, jr2.data(), jr2.size() * sizeof(Tile));
+    put(o_k1, k1.data(), k1.size() * sizeof(Tile));
+    put(o_k2, k2.data(), k2.size() * sizeof(Tile));
+    put(o_kf, kf.data(), kf.size() * sizeof(Tile));
+    hipError_t e = hipMalloc(&p->dev, host.size());
+    if (e != hipSuccess) {
+        set_error("mdf_grouped_plan_create: hipMalloc(%zu) failed: %s", host.size(), hipGetErrorString(e));
+        delete p;
+        return MDF_EHIP;
+    }
+    e = hipMemcpy(p->dev, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error("mdf_grouped_plan_create: copy failed: %s", hipGetErrorString(e));
+        (void)hipFree(p->dev);
+        delete p;
+        return MDF_EHIP;
+    }
+    char *d = (char *)p->dev;
+    p->jitems = (GJItem *)(d + o_j); p->kitems = (GKItem *)(d + o_k);
+    p->j1 = (Tile *)(d + o_j1); p->jr1 = (Tile *)(d + o_jr1); p->j2 = (Tile *)(d + o_j2);
+    p->jr2 = (Tile *)(d + o_jr2); p->k1 = (Tile *)(d + o_k1); p->k2 = (Tile *)(d + o_k2);
+    p->kf = (Tile *)(d + o_kf);
+    *out = p;
+    *work_bytes = p->work_bytes;
+    return MDF_OK;
+}
+
+template <int BN>
+static void launch_gk1(const mdf_grouped_plan *p, char *w, hipStream_t st) {
+    const int g = persistent_grid(p->t_k1, 2);
+    if (p->prec == MDF_PREC_FP32)
+        hipLaunchKernelGGL((gk_step1<BN, float>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, p->k1, p->t_k1, w);
+    else
+        hipLaunchKernelGGL((gk_step1<BN, double>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, p->k1, p->t_k1, w);
+}
+
+extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work_bytes, void *stream) {
+    MDF_CHECK_ARG(p && (work || p->work_bytes <= 256), "mdf_grouped_jk: null argument");
+    if (work_bytes < p->work_bytes) {
+        set_error("mdf_grouped_jk: workspace %zu bytes < %zu required", work_bytes, p->work_bytes);
+        return MDF_EWORKSPACE;
+    }
+    hipStream_t st = as_stream(stream);
+    char *w = (char *)work;
+    if (p->with_j && p->nj) {
+        hipLaunchKernelGGL(gj_pass1, dim3(persistent_grid(p->t_j1, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
+                           p->j1, p->t_j1, w);
+        MDF_LAUNCHED("gj_pass1");
+        hipLaunchKernelGGL(gj_reduce1, dim3(persistent_grid(p->t_jr1, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
+                           p->jr1, p->t_jr1, w);
+        MDF_LAUNCHED("gj_reduce1");
+        hipLaunchKernelGGL(gj_pass2, dim3(persistent_grid(p->t_j2, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
+                           p->j2, p->t_j2, w);
+        MDF_LAUNCHED("gj_pass2");
+        hipLaunchKernelGGL(gj_reduce2, dim3(persistent_grid(p->t_jr2, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
+                           p->jr2, p->t_jr2, w);
+        MDF_LAUNCHED("gj_reduce2");
+    }
+    if (p->with_k && p->nk) {
+        switch (p->bn) {
+        case 160: launch_gk1<160>(p, w, st); break;
+        case 128: launch_gk1<128>(p, w, st); break;
+        default: launch_gk1<64>(p, w, st); break;
+        }
+        MDF_LAUNCHED("gk_step1");
+        const int g2 = persistent_grid(p->t_k2, 2);
+        if (p->prec == MDF_PREC_FP32)
+            hipLaunchKernelGGL(gk_step2<float>, dim3(g2), dim3(G_THREADS), 0, st, p->kitems, p->k2, p->t_k2, w);
+        else
+            hipLaunchKernelGGL(gk_step2<double>, dim3(g2), dim3(G_THREADS), 0, st, p->kitems, p->k2, p->t_k2, w);
+        MDF_LAUNCHED("gk_step2");
+        hipLaunchKernelGGL(gk_finalize, dim3(persistent_grid(p->t_kf, 4)), dim3(256), 0, st, p->kitems, p->kf,
+                           p->t_kf, w);
+        MDF_LAUNCHED("gk_finalize");
+    }
+    return MDF_OK;
+}
+
+extern "C" int mdf_grouped_plan_destroy(mdf_grouped_plan *p) {
+    if (!p) return MDF_OK;
+    if (p->dev) MDF_HIP(hipFree(p->dev));
+    delete p;
+    return MDF_OK;
+}
+
+extern "C" int mdf_grouped_plan_info(const mdf_grouped_plan *p, long *tiles, int ntiles) {
+    MDF_CHECK_ARG(p && tiles && ntiles >= 7, "mdf_grouped_plan_info: bad arguments");
+    tiles[0] = p->t_j1; tiles[1] = p->t_jr1; tiles[2] = p->t_j2; tiles[3] = p->t_jr2;
+    tiles[4] = p->t_k1; tiles[5] = p->t_k2; tiles[6] = p->t_kf;
+    return MDF_OK;
+}

@@ -1,0 +1,252 @@
+// Tile bodies of the exchange (K) contraction, shared by the single-item
+// launches (mdf_k.hip) and the grouped persistent launch (mdf_grouped.hip),
+// so an item gives bit-identical results alone and in a batch.
+//
+//     R[L, i, k] = sum_j A_L[i, j] F[j, k]                 (step 1)
+//     K[i, j]   += sum_{L,k} R_l[L, i, k] R_r[L, j, k]     (step 2)
+#pragma once
+#include <type_traits>
+
+#include "mdf_common.h"
+#include "mdf_gemm.h"
+
+namespace mdf {
+namespace kcore {
+
+__device__ inline long tri(long i) { return i * (i + 1) / 2; }
+
+// ---------------------------------------------------------------- step 1
+// A operand of step 1: rows i0..i0+127 of the unpacked A_L, K-step kt covers
+// j = 16 kt .. 16 kt + 15.
+template <bool MAPPED, typename T = double>
+struct PackedALoader {
+    static constexpr int N = 128 * G_BK / G_THREADS;     // 8
+    static constexpr int TPJ = G_THREADS / G_BK;         // threads per j row (upper)
+    const double *row;   // cderi row L
+    const int *pmap;     // MAPPED: pair index of (i, j), -1 if not stored
+    int nao, i0;
+    int upper;           // layout used by the last load()
+    __device__ void load(int kt, T (&r)[N]) {
+        const int t = threadIdx.x, j0 = kt * G_BK;
+        if (!MAPPED && j0 >= i0 + 127) {
+            // every j > every i: (i, j) at tri(j) + i, contiguous in i
+            upper = 1;
+            const int j = j0 + t / TPJ, ib = i0 + (t % TPJ) * N;
+            const double *src = row + tri(j) + ib;
+#pragma unroll
+            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? (T)src[v] : (T)0;
+            return;
+        }
+        upper = 0;
+        const int i = i0 + (t >> 1), jb = j0 + (t & 1) * N;
+        if (!MAPPED && i0 >= j0 + G_BK - 1) {
+            // every i >= every j: contiguous in j
+            const double *src = row + tri(i) + jb;
+#pragma unroll
+            for (int v = 0; v < N; v++) r[v] = (i < nao) ? (T)src[v] : (T)0;
+            return;
+        }
+#pragma unroll
+        for (int v = 0; v < N; v++) {
+            const int j = jb + v;
+            double x = 0.;
+            if (i < nao && j < nao) {
+                if (MAPPED) {
+                    const int p = pmap[(long)i * nao + j];
+                    x = p >= 0 ? row[p] : 0.;
+                } else {
+                    x = row[i >= j ? tri(i) + j : tri(j) + i];
+                }
+            }
+            r[v] = (T)x;
+        }
+    }
+    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
+        const int t = threadIdx.x;
+        if (upper) {
+            const int jj = t / TPJ, ib = (t % TPJ) * N;
+#pragma unroll
+            for (int v = 0; v < N; v++) lds[jj * pitch + ib + v] = r[v];
+        } else {
+            const int ii = t >> 1, jb = (t & 1) * N;
+#pragma unroll
+            for (int v = 0; v < N; v++) lds[(jb + v) * pitch + ii] = r[v];
+        }
+    }
+};
+
+// One 128 x BN tile of step 1 for aux row `row` (the cderi row L): rows
+// i0.., columns (occupied index) k0.. of R_L.
+template <int BN, bool MAPPED, typename T>
+__device__ inline void step1_tile(const double *__restrict__ row, const int *__restrict__ pmap, int nao,
+                                  const double *__restrict__ F, long ldf, int nocc, T *__restrict__ Rl, long ldr,
+                                  int i0, int k0, T *lds) {
+    using S = TileShape<128, BN>;
+    PackedALoader<MAPPED, T> la{row, pmap, nao, i0, 0};
+    ColsLoader<BN, T, double> lb{F, ldf, k0, nocc, 0, nao};
+    Acc<S::MI, S::NI, T> acc;
+    acc.zero();
+    gemm_mainloop<128, BN>(la, lb, (nao + G_BK - 1) / G_BK, lds, acc);
+    gemm_epilogue<128, BN, T>(acc, [&](int r, int c, T v) {
+        const int i = i0 + r, k = k0 + c;
+        if (i < nao && k < nocc) Rl[(long)i * ldr + k] = v;
+    });
+}
+
+// ---------------------------------------------------------------- step 2
+// Rows r0..r0+127 of R[L] (k contiguous); K-step kt of this split is
+// (L, kc) = divmod(s0 + kt, kchunks), columns kc*16 .. kc*16 + 15.
+template <bool VEC, typename T>
+struct TransformedLoader {
+    static constexpr int N = 128 * G_BK / G_THREADS;     // 8
+    static constexpr int EPV = 16 / (int)sizeof(T);       // elements per 16-byte load
+    using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    const T *R;
+    long strideL, ldr;
+    int r0, nao, nocc, kchunks;
+    long L;              // current aux row of the next load()
+    int kc;              // current k chunk of the next load()
+    // thread t owns row t % 128 and k half t / 128, so a wavefront's LDS
+    // stores cover 64 consecutive rows of one k (no bank conflicts)
+    __device__ void load(int, T (&r)[N]) {
+        // called for kt = 0, 1, 2, ... in order: advance (L, kc) in place
+        const int t = threadIdx.x;
+        const int kb = kc * G_BK + (t >> 7) * N;
+        const int row = r0 + (t & 127);
+        const T *src = R + L * strideL + (long)row * ldr + kb;
+        if (VEC && row < nao && kb + N <= nocc) {
+#pragma unroll
+            for (int v = 0; v < N; v += EPV) {
+                const V x = *reinterpret_cast<const V *>(src + v);
+                const T *xe = reinterpret_cast<const T *>(&x);
+#pragma unroll
+                for (int e = 0; e < EPV; e++) r[v + e] = xe[e];
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < N; v++) r[v] = (row < nao && kb + v < nocc) ? src[v] : (T)0;
+        }
+        if (++kc == kchunks) {
+            kc = 0;
+            ++L;
+        }
+    }
+    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
+        const int t = threadIdx.x, ii = t & 127, kb = (t >> 7) * N;
+#pragma unroll
+        for (int v = 0; v < N; v++) lds[(kb + v) * pitch + ii] = r[v];
+    }
+};
+
+constexpr int S2_B = 128;
+
+// One 128 x 128 output tile (i0, j0) of step 2 over the K-steps [s0, s1) of
+// the (L, k-chunk) reduction; writes (beta = 0) or adds (beta = 1) into o.
+template <bool VEC, typename T>
+__device__ inline void step2_tile(const T *__restrict__ Rl, const T *__restrict__ Rr, int nao, int nocc, long ldr,
+                                  long strideL, int i0, int j0, long s0, long s1, double *__restrict__ o, int beta,
+                                  T *lds) {
+    using S = TileShape<S2_B, S2_B>;
+    const int kchunks = (nocc + G_BK - 1) / G_BK;
+    const long Ls = s0 / kchunks;
+    const int kcs = (int)(s0 - Ls * kchunks);
+    TransformedLoader<VEC, T> la{Rl, strideL, ldr, i0, nao, nocc, kchunks, Ls, kcs};
+    TransformedLoader<VEC, T> lb{Rr, strideL, ldr, j0, nao, nocc, kchunks, Ls, kcs};
+    Acc<S::MI, S::NI, T> acc;
+    acc.zero();
+    gemm_mainloop<S2_B, S2_B>(la, lb, (int)(s1 - s0), lds, acc);
+    gemm_epilogue<S2_B, S2_B, T>(acc, [&](int r, int c, T v) {
+        const int i = i0 + r, j = j0 + c;
+        if (i < nao && j < nao) {
+            double *dst = o + (long)i * nao + j;
+            *dst = beta ? *dst + (double)v : (double)v;
+        }
+    });
+}
+
+// Lower-triangle tile index x -> (ti >= tj), row-major.
+__device__ inline void tri_tile(int x, int &ti, int &tj) {
+    ti = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+    while (ti * (ti + 1) / 2 > x) ti--;
+    while ((ti + 1) * (ti + 2) / 2 <= x) ti++;
+    tj = x - ti * (ti + 1) / 2;
+}
+
+// vk[i, j] (+)= sum_s part[s, I, J] for the flat range [e0, e1); SYM reads the
+// lower-tile copy for upper-tile entries.
+__device__ inline void finalize_range(const double *__restrict__ part, int ksplit, long part_stride, int nao,
+                                      int sym, double *__restrict__ vk, long ldvk, int accumulate, long e0, long e1) {
+    for (long t = e0 + threadIdx.x; t < e1; t += 256) {
+        const int i = (int)(t / nao), j = (int)(t % nao);
+        int I = i, J = j;
+        if (sym && (i / S2_B) < (j / S2_B)) { I = j; J = i; }
+        double s = 0.;
+        for (int k = 0; k < ksplit; k++) s += part[k * part_stride + (long)I * nao + J];
+        double *dst = vk + (long)i * ldvk + j;
+        *dst = accumulate ? *dst + s : s;
+    }
+}
+
+// step-1 N tile: the width in {64, 128, 160} that pads nocc least (ties: wider)
+inline int pick_bn(int nocc) {
+    const int cand[3] = {160, 128, 64};
+    int best = 64;
+    long best_pad = 1L << 40;
+    for (int b : cand) {
+        const long pad = (long)((nocc + b - 1) / b) * b - nocc;
+        if (pad < best_pad) { best_pad = pad; best = b; }
+    }
+    return best;
+}
+
+struct KPlan {
+    int ntile, ntiles_grid, ksplit, bn;
+    long nL;            // aux rows per block
+    size_t r_words;     // one R buffer
+    size_t part_words;
+};
+
+// Resident step-2 workgroups on this device: 2 per CU (LDS and registers).
+inline int resident_step2_wgs() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 512;
+    if (!cached[dev]) {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+            cu = 256;
+        cached[dev] = 2 * cu;
+    }
+    return cached[dev];
+}
+
+// Split of the (L, k) reduction: the ksplit in 1..32 whose grid fills whole
+// rounds of resident workgroups best (ties: the larger split).
+inline int pick_ksplit(int ntiles_grid) {
+    const int res = resident_step2_wgs();
+    int best = 1;
+    double best_eff = -1.;
+    for (int ks = 1; ks <= 32; ks++) {
+        const long wgs = (long)ntiles_grid * ks;
+        const long rounds = (wgs + res - 1) / res;
+        const double eff = (double)wgs / (double)(rounds * res);
+        if (eff >= best_eff - 1e-9) { best_eff = eff; best = ks; }
+    }
+    return best;
+}
+
+inline KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows) {
+    KPlan p;
+    p.ntile = (nao + S2_B - 1) / S2_B;
+    p.ntiles_grid = two ? p.ntile * p.ntile : p.ntile * (p.ntile + 1) / 2;
+    p.ksplit = pick_ksplit(p.ntiles_grid);
+    p.bn = pick_bn(nocc);
+    p.nL = max_rows < naux ? max_rows : naux;
+    if (p.nL < 1) p.nL = 1;
+    p.r_words = (size_t)p.nL * nao * nocc;
+    p.part_words = (size_t)p.ksplit * nao * nao;
+    return p;
+}
+
+}  // namespace kcore
+}  // namespace mdf

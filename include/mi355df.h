@@ -74,6 +74,46 @@ int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *p
               double *vk, long ldvk, int accumulate, void *work, size_t work_bytes,
               void *stream);
 
+/* ---- grouped J/K over many (molecule, density) items -------------------
+ * No counterpart in the reference (each get_jk call there contracts one
+ * molecule, electrolites/fastdf.py:805-1026); this batches many.  One item
+ * per molecule (all pointers DEVICE memory):
+ *   cderi (naux, npairs), row stride ld, pair_map as for mdf_get_k;
+ *   n_dm (1..8) densities: dm_pairs (n_dm, npairs) packed as mdf_pack_dm
+ *   writes, rho (n_dm, naux) and vj_pairs (n_dm, npairs) out (J);
+ *   fl (n_dm, nao, nocc) [and fr, or NULL for symmetric] factors, vk
+ *   (n_dm, nao, nao) out (K).
+ * mdf_grouped_plan_create flattens the batch into tile tables kept in device
+ * memory the plan owns (it allocates; the runs do not) and reports the
+ * workspace a run needs; bn = 0 picks the step-1 tile width.
+ * mdf_grouped_jk runs the batch as persistent launches only (capturable in a
+ * HIP graph).  Each item is cut and computed exactly as the single-item
+ * entries cut it, so its J/K are bit-identical alone and in a batch. */
+typedef struct {
+    const double *cderi;
+    long ld;
+    long naux;
+    long npairs;
+    int nao;
+    const int *pair_map;
+    int n_dm;
+    const double *dm_pairs;
+    double *rho;
+    double *vj_pairs;
+    const double *fl;
+    const double *fr;
+    int nocc;
+    double *vk;
+} mdf_jk_item;
+
+typedef struct mdf_grouped_plan mdf_grouped_plan;
+int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int with_j, int with_k, int prec,
+                            int bn, mdf_grouped_plan **plan, size_t *work_bytes);
+int mdf_grouped_jk(const mdf_grouped_plan *plan, void *work, size_t work_bytes, void *stream);
+int mdf_grouped_plan_destroy(mdf_grouped_plan *plan);
+/* tiles[0..6]: J pass-1, J reduce-1, J pass-2, J reduce-2, K step-1, K step-2, K finalize */
+int mdf_grouped_plan_info(const mdf_grouped_plan *plan, long *tiles, int ntiles);
+
 /* ---- pair-vector helpers ----------------------------------------------
  * mdf_pack_dm replaces electrolites/fastdf.py:674-686 (_sparse_dm): the
  * density gathered onto the stored pairs, 2 D_ij (hermi 1) or D_ij + D_ji
