@@ -9,10 +9,14 @@
 // to the engine's element type T on the way into LDS:
 //
 //   struct Loader {
-//     static constexpr int N;                  // elements per thread per K-step
-//     __device__ void load(int kt, T (&r)[N]);  // global -> registers
-//     __device__ void store(const T (&r)[N], T *lds, int pitch);
+//     using reg_t = ...;                      // register type (the source type)
+//     static constexpr int N;                 // elements per thread per K-step
+//     void load(int kt, reg_t (&r)[N]);       // global -> registers
+//     void store(const reg_t (&r)[N], T *lds, int pitch);   // convert -> LDS
 //   };
+//
+// Prefetched values stay in the source type until the store, so converting
+// them never makes the current K-step's MFMAs wait on the prefetch.
 //
 // T = double: v_mfma_f64_16x16x4_f64; T = float: v_mfma_f32_16x16x4_f32.
 // Lane layouts measured on gfx950 with one-hot probes: A lane = m + 16 k,
@@ -22,6 +26,11 @@
 #include "mdf_common.h"
 
 typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// LDS (address space 3) pointers: operand reads through them compile to
+// ds_read; through a generic pointer they become flat loads that wait on the
+// in-flight global prefetch (s_waitcnt vmcnt(0)) and break the pipeline.
+#define MDF_LDS __attribute__((address_space(3)))
 
 namespace mdf {
 
@@ -34,22 +43,22 @@ struct Mfma;
 template <>
 struct Mfma<double> {
     using acc_t = double4_t;
-    __device__ static acc_t mma(double a, double b, acc_t c) {
+    __device__ __forceinline__ static acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
-    __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+    __device__ __forceinline__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 
 template <>
 struct Mfma<float> {
     using acc_t = float4_t;
-    __device__ static acc_t mma(float a, float b, acc_t c) {
+    __device__ __forceinline__ static acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
-    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+    __device__ __forceinline__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-__device__ inline double4_t mfma16(double a, double b, double4_t c) { return Mfma<double>::mma(a, b, c); }
+__device__ __forceinline__ double4_t mfma16(double a, double b, double4_t c) { return Mfma<double>::mma(a, b, c); }
 
 template <int BM, int BN>
 struct TileShape {
@@ -64,7 +73,7 @@ struct TileShape {
 template <int MI, int NI, typename T = double>
 struct Acc {
     typename Mfma<T>::acc_t v[MI][NI];
-    __device__ void zero() {
+    __device__ __forceinline__ void zero() {
 #pragma unroll
         for (int a = 0; a < MI; a++)
 #pragma unroll
@@ -74,19 +83,22 @@ struct Acc {
 
 // Runs the K loop.  `lds` must hold TileShape<BM,BN>::LDS_ELEMS elements of T.
 template <int BM, int BN, class LA, class LB, typename T = double>
-__device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, T *lds,
+__device__ __forceinline__ void gemm_mainloop(LA &la, LB &lb, int nk, T *lds,
                                      Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI, T> &acc) {
     using S = TileShape<BM, BN>;
+    constexpr int SA = G_BK * S::PA, SB = G_BK * S::PB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    T *As[2] = {lds, lds + G_BK * S::PA};
-    T *Bs[2] = {lds + 2 * G_BK * S::PA, lds + 2 * G_BK * S::PA + G_BK * S::PB};
-    T ra[LA::N], rb[LB::N];
+    T *const Bbase = lds + 2 * SA;
+    const MDF_LDS T *Ar = (const MDF_LDS T *)(lds + wm * S::WM + (lane & 15));
+    const MDF_LDS T *Br = (const MDF_LDS T *)(Bbase + wn * S::WN + (lane & 15));
+    typename LA::reg_t ra[LA::N];
+    typename LB::reg_t rb[LB::N];
     if (nk <= 0) return;
     la.load(0, ra);
     lb.load(0, rb);
-    la.store(ra, As[0], S::PA);
-    lb.store(rb, Bs[0], S::PB);
+    la.store(ra, lds, S::PA);
+    lb.store(rb, Bbase, S::PB);
     __syncthreads();
     for (int kt = 0; kt < nk; kt++) {
         const int cur = kt & 1;
@@ -95,8 +107,8 @@ __device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, T *lds,
             la.load(kt + 1, ra);
             lb.load(kt + 1, rb);
         }
-        const T *A = As[cur] + wm * S::WM + (lane & 15);
-        const T *B = Bs[cur] + wn * S::WN + (lane & 15);
+        const MDF_LDS T *A = Ar + cur * SA;
+        const MDF_LDS T *B = Br + cur * SB;
 #pragma unroll
         for (int kq = 0; kq < G_BK / 4; kq++) {
             const int kk = kq * 4 + (lane >> 4);
@@ -111,16 +123,18 @@ __device__ inline void gemm_mainloop(LA &la, LB &lb, int nk, T *lds,
                 for (int ni = 0; ni < S::NI; ni++) acc.v[mi][ni] = Mfma<T>::mma(a[mi], b[ni], acc.v[mi][ni]);
         }
         if (more) {
-            la.store(ra, As[cur ^ 1], S::PA);
-            lb.store(rb, Bs[cur ^ 1], S::PB);
+            la.store(ra, lds + (cur ^ 1) * SA, S::PA);
+            lb.store(rb, Bbase + (cur ^ 1) * SB, S::PB);
         }
         __syncthreads();
     }
 }
 
-// Visits every accumulator element with its tile-local (row, col).
-template <int BM, int BN, typename T = double, class F>
-__device__ inline void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI, T> &acc, F &&f) {
+// Visits every accumulator element with its tile-local (row, col); the
+// element-to-row layout is that of the MFMA type LT (an FP64 shadow of an
+// FP32 accumulator keeps the FP32 layout).
+template <int BM, int BN, typename LT, class AccT, class F>
+__device__ __forceinline__ void gemm_epilogue_layout(const AccT &acc, F &&f) {
     using S = TileShape<BM, BN>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -131,10 +145,15 @@ __device__ inline void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, TileShape<
             const int c = wn * S::WN + ni * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int r = wm * S::WM + mi * 16 + Mfma<T>::row(lane, v);
+                const int r = wm * S::WM + mi * 16 + Mfma<LT>::row(lane, v);
                 f(r, c, acc.v[mi][ni][v]);
             }
         }
+}
+
+template <int BM, int BN, typename T = double, class F>
+__device__ __forceinline__ void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI, T> &acc, F &&f) {
+    gemm_epilogue_layout<BM, BN, T>(acc, f);
 }
 
 // ---------------------------------------------------------------- loaders
@@ -143,25 +162,26 @@ __device__ inline void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, TileShape<
 // Out-of-range reads 0.
 template <int R, typename T = double, typename Src = double>
 struct RowsLoader {
+    using reg_t = Src;
     static constexpr int N = R * G_BK / G_THREADS;
     static_assert(N >= 1 && (G_THREADS / (G_BK / N)) == R, "RowsLoader shape");
     const Src *base;      // &M[0][0]
     long ld;              // row stride
     int r0, nrows, k0, kmax;
-    __device__ void load(int kt, T (&r)[N]) const {
+    __device__ __forceinline__ void load(int kt, Src (&r)[N]) const {
         const int t = threadIdx.x;
         const int row = r0 + t / (G_BK / N);
         const int kb = k0 + kt * G_BK + (t % (G_BK / N)) * N;
         const Src *src = base + (long)row * ld + kb;
 #pragma unroll
-        for (int v = 0; v < N; v++) r[v] = (row < nrows && kb + v < kmax) ? (T)src[v] : (T)0;
+        for (int v = 0; v < N; v++) r[v] = (row < nrows && kb + v < kmax) ? src[v] : (Src)0;
     }
-    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
+    __device__ __forceinline__ void store(const Src (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;
         const int row = t / (G_BK / N);
         const int kb = (t % (G_BK / N)) * N;
 #pragma unroll
-        for (int v = 0; v < N; v++) lds[(kb + v) * pitch + row] = r[v];
+        for (int v = 0; v < N; v++) lds[(kb + v) * pitch + row] = (T)r[v];
     }
 };
 
@@ -169,11 +189,12 @@ struct RowsLoader {
 // c0 .. c0+C-1.
 template <int C, typename T = double, typename Src = double>
 struct ColsLoader {
+    using reg_t = Src;
     static constexpr int N = C * G_BK / G_THREADS;
     const Src *base;
     long ld;
     int c0, ncols, k0, kmax;
-    __device__ void load(int kt, T (&r)[N]) const {
+    __device__ __forceinline__ void load(int kt, Src (&r)[N]) const {
         const int t = threadIdx.x;
         // each thread takes N consecutive columns of one k row
         constexpr int TPR = C / N;              // threads per k row
@@ -182,16 +203,16 @@ struct ColsLoader {
         const int k = k0 + kt * G_BK + kk;
         const Src *src = base + (long)k * ld + cb;
 #pragma unroll
-        for (int v = 0; v < N; v++) r[v] = (kk < G_BK && k < kmax && cb + v < ncols) ? (T)src[v] : (T)0;
+        for (int v = 0; v < N; v++) r[v] = (kk < G_BK && k < kmax && cb + v < ncols) ? src[v] : (Src)0;
     }
-    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
+    __device__ __forceinline__ void store(const Src (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;
         constexpr int TPR = C / N;
         const int kk = t / TPR;
         const int cb = (t % TPR) * N;
         if (kk < G_BK) {
 #pragma unroll
-            for (int v = 0; v < N; v++) lds[kk * pitch + cb + v] = r[v];
+            for (int v = 0; v < N; v++) lds[kk * pitch + cb + v] = (T)r[v];
         }
     }
 };

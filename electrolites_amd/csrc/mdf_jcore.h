@@ -24,7 +24,7 @@ struct vec_t<2> { using T = double2; };
 template <int VEC>
 constexpr long chunk_cols() { return 4L * 64 * VEC * J_U; }
 
-__device__ inline double wave_sum(double v) {
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -35,7 +35,7 @@ __device__ inline double wave_sum(double v) {
 // product is reduced across the wavefront and the four wavefronts.
 // part[(chunk * ndm + n) * naux + r]; ndm <= NDM densities are live.
 template <int NDM, int VEC>
-__device__ inline void rho_partial_tile(const double *__restrict__ L, long ld, long naux, long npairs,
+__device__ __forceinline__ void rho_partial_tile(const double *__restrict__ L, long ld, long naux, long npairs,
                                         const double *__restrict__ d, long ldd, int ndm,
                                         double *__restrict__ part, long chunk, long rblock) {
     constexpr int WCOLS = 64 * VEC * J_U;
@@ -107,7 +107,7 @@ __device__ inline void rho_partial_tile(const double *__restrict__ L, long ld, l
 }
 
 // rho[n, r] = sum_c part[c, n, r] for the flat element range [e0, e1).
-__device__ inline void rho_reduce_range(const double *__restrict__ part, long nchunk, int ndm, long naux,
+__device__ __forceinline__ void rho_reduce_range(const double *__restrict__ part, long nchunk, int ndm, long naux,
                                         double *__restrict__ rho, long ldrho, long e0, long e1) {
     for (long t = e0 + threadIdx.x; t < e1; t += J_THREADS) {
         const int n = (int)(t / naux);
@@ -121,7 +121,7 @@ __device__ inline void rho_reduce_range(const double *__restrict__ part, long nc
 // Pass-2 tile (cblock, rows [r0, r1)): a thread owns VEC consecutive columns.
 // out[n * ldo + c] for n < ndm.
 template <int NDM, int VEC>
-__device__ inline void vj_pass_tile(const double *__restrict__ L, long ld, long npairs,
+__device__ __forceinline__ void vj_pass_tile(const double *__restrict__ L, long ld, long npairs,
                                     const double *__restrict__ rho, long ldrho, int ndm, long r0, long r1,
                                     double *__restrict__ out, long ldo, long cblock) {
     const long c = (cblock * J_THREADS + threadIdx.x) * VEC;
@@ -185,7 +185,7 @@ __device__ inline void vj_pass_tile(const double *__restrict__ L, long ld, long 
 }
 
 // vj[n, c] (+)= sum_g part[g, n, c] for the flat element range [e0, e1).
-__device__ inline void vj_reduce_range(const double *__restrict__ part, long ngroup, int ndm, long npairs,
+__device__ __forceinline__ void vj_reduce_range(const double *__restrict__ part, long ngroup, int ndm, long npairs,
                                        double *__restrict__ vj, long ldvj, int accumulate, long e0, long e1) {
     for (long t = e0 + threadIdx.x; t < e1; t += J_THREADS) {
         const int n = (int)(t / npairs);

@@ -13,20 +13,21 @@
 namespace mdf {
 namespace kcore {
 
-__device__ inline long tri(long i) { return i * (i + 1) / 2; }
+__device__ __forceinline__ long tri(long i) { return i * (i + 1) / 2; }
 
 // ---------------------------------------------------------------- step 1
 // A operand of step 1: rows i0..i0+127 of the unpacked A_L, K-step kt covers
 // j = 16 kt .. 16 kt + 15.
 template <bool MAPPED, typename T = double>
 struct PackedALoader {
+    using reg_t = double;
     static constexpr int N = 128 * G_BK / G_THREADS;     // 8
     static constexpr int TPJ = G_THREADS / G_BK;         // threads per j row (upper)
     const double *row;   // cderi row L
     const int *pmap;     // MAPPED: pair index of (i, j), -1 if not stored
     int nao, i0;
     int upper;           // layout used by the last load()
-    __device__ void load(int kt, T (&r)[N]) {
+    __device__ __forceinline__ void load(int kt, double (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * G_BK;
         if (!MAPPED && j0 >= i0 + 127) {
             // every j > every i: (i, j) at tri(j) + i, contiguous in i
@@ -34,7 +35,7 @@ struct PackedALoader {
             const int j = j0 + t / TPJ, ib = i0 + (t % TPJ) * N;
             const double *src = row + tri(j) + ib;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? (T)src[v] : (T)0;
+            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? src[v] : 0.;
             return;
         }
         upper = 0;
@@ -43,7 +44,7 @@ struct PackedALoader {
             // every i >= every j: contiguous in j
             const double *src = row + tri(i) + jb;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (i < nao) ? (T)src[v] : (T)0;
+            for (int v = 0; v < N; v++) r[v] = (i < nao) ? src[v] : 0.;
             return;
         }
 #pragma unroll
@@ -58,19 +59,19 @@ struct PackedALoader {
                     x = row[i >= j ? tri(i) + j : tri(j) + i];
                 }
             }
-            r[v] = (T)x;
+            r[v] = x;
         }
     }
-    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
+    __device__ __forceinline__ void store(const double (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;
         if (upper) {
             const int jj = t / TPJ, ib = (t % TPJ) * N;
 #pragma unroll
-            for (int v = 0; v < N; v++) lds[jj * pitch + ib + v] = r[v];
+            for (int v = 0; v < N; v++) lds[jj * pitch + ib + v] = (T)r[v];
         } else {
             const int ii = t >> 1, jb = (t & 1) * N;
 #pragma unroll
-            for (int v = 0; v < N; v++) lds[(jb + v) * pitch + ii] = r[v];
+            for (int v = 0; v < N; v++) lds[(jb + v) * pitch + ii] = (T)r[v];
         }
     }
 };
@@ -78,7 +79,7 @@ struct PackedALoader {
 // One 128 x BN tile of step 1 for aux row `row` (the cderi row L): rows
 // i0.., columns (occupied index) k0.. of R_L.
 template <int BN, bool MAPPED, typename T>
-__device__ inline void step1_tile(const double *__restrict__ row, const int *__restrict__ pmap, int nao,
+__device__ __forceinline__ void step1_tile(const double *__restrict__ row, const int *__restrict__ pmap, int nao,
                                   const double *__restrict__ F, long ldf, int nocc, T *__restrict__ Rl, long ldr,
                                   int i0, int k0, T *lds) {
     using S = TileShape<128, BN>;
@@ -98,6 +99,7 @@ __device__ inline void step1_tile(const double *__restrict__ row, const int *__r
 // (L, kc) = divmod(s0 + kt, kchunks), columns kc*16 .. kc*16 + 15.
 template <bool VEC, typename T>
 struct TransformedLoader {
+    using reg_t = T;
     static constexpr int N = 128 * G_BK / G_THREADS;     // 8
     static constexpr int EPV = 16 / (int)sizeof(T);       // elements per 16-byte load
     using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
@@ -108,7 +110,7 @@ struct TransformedLoader {
     int kc;              // current k chunk of the next load()
     // thread t owns row t % 128 and k half t / 128, so a wavefront's LDS
     // stores cover 64 consecutive rows of one k (no bank conflicts)
-    __device__ void load(int, T (&r)[N]) {
+    __device__ __forceinline__ void load(int, T (&r)[N]) {
         // called for kt = 0, 1, 2, ... in order: advance (L, kc) in place
         const int t = threadIdx.x;
         const int kb = kc * G_BK + (t >> 7) * N;
@@ -131,7 +133,7 @@ struct TransformedLoader {
             ++L;
         }
     }
-    __device__ void store(const T (&r)[N], T *lds, int pitch) const {
+    __device__ __forceinline__ void store(const T (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x, ii = t & 127, kb = (t >> 7) * N;
 #pragma unroll
         for (int v = 0; v < N; v++) lds[(kb + v) * pitch + ii] = r[v];
@@ -139,11 +141,15 @@ struct TransformedLoader {
 };
 
 constexpr int S2_B = 128;
+// FP32 step 2 adds its MFMA accumulators into an FP64 shadow every S2_FLUSH
+// K-steps (S2_FLUSH * 16 products per element), so the rounding error of the
+// long (L, k) reduction grows with the flush length, not with naux * nocc.
+constexpr int S2_FLUSH = 32;
 
 // One 128 x 128 output tile (i0, j0) of step 2 over the K-steps [s0, s1) of
 // the (L, k-chunk) reduction; writes (beta = 0) or adds (beta = 1) into o.
 template <bool VEC, typename T>
-__device__ inline void step2_tile(const T *__restrict__ Rl, const T *__restrict__ Rr, int nao, int nocc, long ldr,
+__device__ __forceinline__ void step2_tile(const T *__restrict__ Rl, const T *__restrict__ Rr, int nao, int nocc, long ldr,
                                   long strideL, int i0, int j0, long s0, long s1, double *__restrict__ o, int beta,
                                   T *lds) {
     using S = TileShape<S2_B, S2_B>;
@@ -153,19 +159,38 @@ __device__ inline void step2_tile(const T *__restrict__ Rl, const T *__restrict_
     TransformedLoader<VEC, T> la{Rl, strideL, ldr, i0, nao, nocc, kchunks, Ls, kcs};
     TransformedLoader<VEC, T> lb{Rr, strideL, ldr, j0, nao, nocc, kchunks, Ls, kcs};
     Acc<S::MI, S::NI, T> acc;
-    acc.zero();
-    gemm_mainloop<S2_B, S2_B>(la, lb, (int)(s1 - s0), lds, acc);
-    gemm_epilogue<S2_B, S2_B, T>(acc, [&](int r, int c, T v) {
+    auto emit = [&](int r, int c, double v) {
         const int i = i0 + r, j = j0 + c;
         if (i < nao && j < nao) {
             double *dst = o + (long)i * nao + j;
-            *dst = beta ? *dst + (double)v : (double)v;
+            *dst = beta ? *dst + v : v;
         }
-    });
+    };
+    if constexpr (sizeof(T) == 8) {
+        acc.zero();
+        gemm_mainloop<S2_B, S2_B>(la, lb, (int)(s1 - s0), lds, acc);
+        gemm_epilogue<S2_B, S2_B, T>(acc, emit);
+    } else {
+        Acc<S::MI, S::NI, double> sum;
+        sum.zero();
+        for (long s = s0; s < s1; s += S2_FLUSH) {
+            acc.zero();
+            // the loaders advance (L, k-chunk) in place, so consecutive calls
+            // continue the reduction where the previous one stopped
+            gemm_mainloop<S2_B, S2_B>(la, lb, (int)min((long)S2_FLUSH, s1 - s), lds, acc);
+#pragma unroll
+            for (int a = 0; a < S::MI; a++)
+#pragma unroll
+                for (int b = 0; b < S::NI; b++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) sum.v[a][b][v] += (double)acc.v[a][b][v];
+        }
+        gemm_epilogue_layout<S2_B, S2_B, T>(sum, emit);
+    }
 }
 
 // Lower-triangle tile index x -> (ti >= tj), row-major.
-__device__ inline void tri_tile(int x, int &ti, int &tj) {
+__device__ __forceinline__ void tri_tile(int x, int &ti, int &tj) {
     ti = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
     while (ti * (ti + 1) / 2 > x) ti--;
     while ((ti + 1) * (ti + 2) / 2 <= x) ti++;
@@ -174,7 +199,7 @@ __device__ inline void tri_tile(int x, int &ti, int &tj) {
 
 // vk[i, j] (+)= sum_s part[s, I, J] for the flat range [e0, e1); SYM reads the
 // lower-tile copy for upper-tile entries.
-__device__ inline void finalize_range(const double *__restrict__ part, int ksplit, long part_stride, int nao,
+__device__ __forceinline__ void finalize_range(const double *__restrict__ part, int ksplit, long part_stride, int nao,
                                       int sym, double *__restrict__ vk, long ldvk, int accumulate, long e0, long e1) {
     for (long t = e0 + threadIdx.x; t < e1; t += 256) {
         const int i = (int)(t / nao), j = (int)(t % nao);

@@ -87,61 +87,82 @@ class RHF:
 
     # ------------------------------------------------------------- loop
     def kernel(self, dm0=None):
-        t = _dev.torch()
+        """Run to convergence with J/K from ``with_df.get_jk``; returns E."""
+        self.start(dm0)
+        while not self.done:
+            self.with_df.k_precision = self.prec
+            vj, vk = self.with_df.get_jk(self.dm, hermi=1)
+            self.absorb(vj, vk)
+        self.with_df.k_precision = 'fp64'
+        return self.e_tot
+
+    def start(self, dm0=None):
+        """Initial density; afterwards the caller alternates J/K of ``self.dm``
+        at precision ``self.prec`` with :meth:`absorb` until ``self.done``.
+        This split lets a serving loop compute many solves' J/K in one
+        grouped launch while each solve follows exactly this sequence."""
         if not hasattr(self, 'H'):
             self.build()
         if dm0 is None:
             _, C = self._diag(self.H)
-            dm = self._density(C)
-        else:
-            dm = dm0
-        focks, errs = [], []
-        e_old = 0.
+            dm0 = self._density(C)
+        self.dm = dm0
+        self._focks, self._errs = [], []
+        self._e_old = 0.
+        self._final = False
         self.converged = False
-        prec = 'fp64' if self.k_precision == 'fp64' else 'fp32'
-        for cycle in range(1, self.max_cycle + 1):
-            self.with_df.k_precision = prec
-            vj, vk = self.with_df.get_jk(dm, hermi=1)
-            F = self.H + vj - .5 * vk
-            e_tot = float(.5 * t.sum(dm * (self.H + F))) + self.enuc
-            err = F @ dm @ self.S
-            err = self.X.T @ (err - err.T) @ self.X
-            focks.append(F)
-            errs.append(err)
-            focks, errs = focks[-self.diis_space:], errs[-self.diis_space:]
-            n = len(focks)
-            if n > 1:
-                B = -t.ones((n + 1, n + 1), dtype=t.float64, device=F.device)
-                B[n, n] = 0.
-                E = t.stack([e.reshape(-1) for e in errs])
-                B[:n, :n] = E @ E.T
-                rhs = t.zeros(n + 1, dtype=t.float64, device=F.device)
-                rhs[n] = -1.
-                coef = t.linalg.solve(B, rhs)[:n]
-                F = t.einsum('i,ijk->jk', coef, t.stack(focks))
-            err_max = float(err.abs().max())
-            de = e_tot - e_old
-            e_old = e_tot
-            self.history.append({'cycle': cycle, 'e_tot': e_tot, 'de': de, 'err': err_max,
-                                 'k_precision': prec})
-            if self.verbose:
-                print(f'cycle {cycle}: E = {e_tot:.12f}  dE = {de:.3e}  |err| = {err_max:.3e}')
-            self.mo_energy, self.mo_coeff = self._diag(F)
-            dm = self._density(self.mo_coeff)
-            if prec != 'fp64' and self.k_precision == 'mixed' and err_max < self.switch_err:
-                prec = 'fp64'               # finishing cycles in FP64, fresh DIIS
-                focks, errs = [], []
-                continue
-            if prec == 'fp64' and abs(de) < self.conv_tol and err_max < self.conv_tol_grad:
-                self.converged = True
-                break
-            if prec != 'fp64' and abs(de) < self.conv_tol and err_max < self.conv_tol_grad:
-                self.converged = True       # an all-FP32 run converges in FP32
-                break
-        self.cycles = cycle
-        self.with_df.k_precision = 'fp64'
-        vj, vk = self.with_df.get_jk(dm, hermi=1)
+        self.done = False
+        self.cycles = 0
+        self.prec = 'fp64' if self.k_precision == 'fp64' else 'fp32'
+        return self
+
+    def absorb(self, vj, vk):
+        """One SCF cycle from the J/K of ``self.dm``: Fock, energy, DIIS,
+        diagonalisation and the next density.  After convergence (or
+        ``max_cycle``) one more FP64 J/K gives the energy of the final
+        density."""
+        t = _dev.torch()
         F = self.H + vj - .5 * vk
-        self.e_tot = float(.5 * t.sum(dm * (self.H + F))) + self.enuc
-        self.dm = dm
-        return self.e_tot
+        e_tot = float(.5 * t.sum(self.dm * (self.H + F))) + self.enuc
+        if self._final:
+            self.e_tot = e_tot
+            self.done = True
+            return
+        self.cycles = cycle = self.cycles + 1
+        prec = self.prec
+        err = F @ self.dm @ self.S
+        err = self.X.T @ (err - err.T) @ self.X
+        focks, errs = self._focks, self._errs
+        focks.append(F)
+        errs.append(err)
+        focks, errs = focks[-self.diis_space:], errs[-self.diis_space:]
+        self._focks, self._errs = focks, errs
+        n = len(focks)
+        if n > 1:
+            B = -t.ones((n + 1, n + 1), dtype=t.float64, device=F.device)
+            B[n, n] = 0.
+            E = t.stack([e.reshape(-1) for e in errs])
+            B[:n, :n] = E @ E.T
+            rhs = t.zeros(n + 1, dtype=t.float64, device=F.device)
+            rhs[n] = -1.
+            coef = t.linalg.solve(B, rhs)[:n]
+            F = t.einsum('i,ijk->jk', coef, t.stack(focks))
+        err_max = float(err.abs().max())
+        de = e_tot - self._e_old
+        self._e_old = e_tot
+        self.history.append({'cycle': cycle, 'e_tot': e_tot, 'de': de, 'err': err_max,
+                             'k_precision': prec})
+        if self.verbose:
+            print(f'cycle {cycle}: E = {e_tot:.12f}  dE = {de:.3e}  |err| = {err_max:.3e}')
+        self.mo_energy, self.mo_coeff = self._diag(F)
+        self.dm = self._density(self.mo_coeff)
+        if prec != 'fp64' and self.k_precision == 'mixed' and err_max < self.switch_err:
+            self.prec = 'fp64'              # finishing cycles in FP64, fresh DIIS
+            self._focks, self._errs = [], []
+        elif abs(de) < self.conv_tol and err_max < self.conv_tol_grad:
+            self.converged = True           # (an all-FP32 run converges in FP32)
+            self._final = True
+        if cycle >= self.max_cycle:
+            self._final = True
+        if self._final:
+            self.prec = 'fp64'

@@ -1,0 +1,268 @@
+"""Continuous-batching DF-SCF serving on one GPU.
+
+A stream of DF-RHF energy requests (molecules of uneven size) is served by a
+loop that keeps every admitted request's state in HBM -- its cderi, one-
+electron matrices, density factors, DIIS history and the device buffers of
+its J/K item -- and advances all of them one SCF cycle per tick:
+
+1. admission: queued requests are admitted in arrival order while the
+   resident bytes (cderi + per-request matrices, :func:`footprint`) stay
+   within ``hbm_budget`` and the batch has a free slot; admitting builds the
+   request's cderi on the GPU (:mod:`electrolites_amd.fastdfcderi`);
+2. one grouped J/K run per precision group (:class:`batch.GroupedJK`): every
+   active request's current density goes into one persistent launch per
+   phase, so a request that arrived while others were mid-solve shares their
+   launches from its first cycle on;
+3. each request absorbs its J/K (:meth:`scf.RHF.absorb`: Fock, energy, DIIS,
+   diagonalisation, next density, precision switch); finished requests leave
+   and free their bytes for the queue.
+
+Each request runs exactly the cycle sequence of a stand-alone
+:class:`scf.RHF` with the same precision schedule, and its grouped J/K item
+is cut exactly as the single-item entries cut it, so its energy is the same
+alone and in any mix (bit-identical; the tests compare to 1e-12 Eh).
+
+The reference has no serving loop (one ``get_jk`` per call,
+electrolites/fastdf.py:805-1026); this is the caller-side batching built on
+the grouped launch.
+"""
+import collections
+import math
+import time
+
+import numpy as np
+
+from . import _dev
+from . import _lib
+from . import batch
+from . import fastdf
+from .scf import RHF
+
+#: default resident-byte budget (of the 288 GB of HBM, leaving headroom for
+#: workspaces and the framework's allocator)
+HBM_BUDGET = 250e9
+
+
+def footprint(mol, diis_space=8):
+    """Resident device bytes of one request: the packed cderi (upper bound:
+    every pair stored) plus its nao x nao matrices (S, H, X, DIIS Fock and
+    error history, J/K buffers)."""
+    nao, naux = mol.nao, mol.naux
+    npairs = nao * (nao + 1) // 2
+    return 8 * (naux * npairs + (2 * diis_space + 10) * nao * nao + 2 * npairs + naux)
+
+
+def molecule_mix(n, seed=0, lo=3, hi=40):
+    """``n`` molecules with a log-uniform number of waters in [lo, hi]; every
+    second one carries a Li+ ion at the centroid (closed shell, charge +1)."""
+    from .gto.cluster import water_cluster
+    rng = np.random.default_rng(seed)
+    mols = []
+    for i in range(n):
+        nw = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        mols.append(water_cluster(nw, seed=int(rng.integers(1 << 30)), li=bool(i % 2)))
+    return mols
+
+
+def poisson_arrivals(n, rate, seed=0):
+    """Arrival times (s) of a Poisson stream of ``n`` requests at ``rate``/s."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1. / rate, size=n)) if rate > 0 else np.zeros(n)
+
+
+class Request:
+    """One energy request.  Filled in by the server: ``t_admit``, ``t_done``
+    (seconds on the server clock), ``e_tot``, ``cycles``, ``converged``,
+    ``history``."""
+
+    def __init__(self, mol, arrival=0., rid=None, k_precision='mixed'):
+        self.mol = mol
+        self.arrival = float(arrival)
+        self.rid = rid
+        self.k_precision = k_precision
+        self.bytes = footprint(mol)
+        self.t_admit = self.t_done = None
+        self.e_tot = None
+        self.cycles = 0
+        self.converged = False
+        self.history = None
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.arrival
+
+
+class _Slot:
+    """An admitted request: its SCF state and its grouped J/K item."""
+
+    def __init__(self, req, conv_tol, max_cycle):
+        t = _dev.torch()
+        self.req = req
+        self.scf = RHF(req.mol, conv_tol=conv_tol, max_cycle=max_cycle,
+                       k_precision=req.k_precision)
+        self.scf.build()
+        self.scf.start()
+        dfobj = self.scf.with_df
+        self.cderi = fastdf._cderi_of(dfobj)
+        nao = req.mol.nao
+        self.lay = fastdf._layout(dfobj, self.cderi, nao)
+        self.entry = batch.Entry(self.cderi, nao, 1, self.scf.nocc, pair_map=self.lay.map)
+        self.entry.layout = self.lay
+        self.nao = nao
+        self._vj = t.empty((1, nao, nao), dtype=t.float64, device=self.cderi.device)
+
+    def load_density(self):
+        """Density factors and packed density of the current ``scf.dm``,
+        computed as :func:`fastdf.get_jk` computes them."""
+        dm = self.scf.dm
+        fl, fr = fastdf.factorize_dm(dm, 1)
+        assert fr is None and fl.shape == self.entry.fl.shape
+        self.entry.fl.copy_(fl)
+        e = self.entry
+        _lib.call('mdf_pack_dm', _dev.ptr(dm), 1, self.nao, self.nao * self.nao,
+                  _dev.ptr(self.lay.addr), e.npairs, 1, _dev.ptr(e.dm_pairs), e.npairs,
+                  _dev.stream_ptr())
+
+    def absorb(self):
+        e = self.entry
+        _lib.call('mdf_unpack_pairs', _dev.ptr(e.vj_pairs), 1, e.npairs, _dev.ptr(self.lay.addr),
+                  e.npairs, self.nao, _dev.ptr(self._vj), self.nao * self.nao, _dev.stream_ptr())
+        self.scf.absorb(self._vj[0], e.vk[0])
+        return self.scf.done
+
+
+class Server:
+    """The serving loop.  ``submit`` queues requests; ``step`` runs one tick;
+    ``serve`` drives a timed arrival stream to completion."""
+
+    def __init__(self, hbm_budget=HBM_BUDGET, max_batch=64, conv_tol=1e-10, max_cycle=100,
+                 graphs=False):
+        self.hbm_budget = float(hbm_budget)
+        self.max_batch = max_batch
+        self.conv_tol = conv_tol
+        self.max_cycle = max_cycle
+        self.graphs = graphs
+        self.queue = collections.deque()
+        self.active = []
+        self.done = []
+        self.resident = 0.
+        self._plans = {}
+        self.ticks = 0
+        self.t0 = time.perf_counter()
+        self.trace = []               # (time, n_active, n_queued, resident bytes)
+
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    def submit(self, req):
+        if req.bytes > self.hbm_budget:
+            raise ValueError(f'request {req.rid}: {req.bytes / 1e9:.1f} GB does not fit the '
+                             f'{self.hbm_budget / 1e9:.1f} GB budget')
+        self.queue.append(req)
+
+    def _admit(self):
+        while self.queue and len(self.active) < self.max_batch and \
+                self.resident + self.queue[0].bytes <= self.hbm_budget:
+            req = self.queue.popleft()
+            req.t_admit = self.now()
+            self.active.append(_Slot(req, self.conv_tol, self.max_cycle))
+            self.resident += req.bytes
+
+    def _plan(self, slots, prec):
+        key = (prec,) + tuple(id(s) for s in slots)
+        g = self._plans.get(key)
+        if g is None:
+            g = batch.GroupedJK([s.entry for s in slots], precision=prec, tune=False)
+            g.uses = 0
+            self._plans[key] = g
+        g.uses += 1
+        if self.graphs and g.graph is None and g.uses == 2:
+            g.capture_graph()
+        return key, g
+
+    def step(self):
+        """Admit, then advance every active request by one SCF cycle."""
+        self._admit()
+        if not self.active:
+            return 0
+        self.ticks += 1
+        groups = collections.defaultdict(list)
+        for s in self.active:
+            s.load_density()
+            groups[s.scf.prec].append(s)
+        live = set()
+        for prec, slots in sorted(groups.items()):
+            key, g = self._plan(slots, prec)
+            live.add(key)
+            g.run()
+        for key in list(self._plans):
+            if key not in live:
+                self._plans.pop(key).close()
+        still = []
+        for s in self.active:
+            if s.absorb():
+                r, scf = s.req, s.scf
+                r.t_done = self.now()
+                r.e_tot, r.cycles, r.converged = scf.e_tot, scf.cycles, scf.converged
+                r.history = scf.history
+                self.resident -= r.bytes
+                self.done.append(r)
+            else:
+                still.append(s)
+        self.active = still
+        self.trace.append((self.now(), len(self.active), len(self.queue), self.resident))
+        return len(self.active)
+
+    def serve(self, requests):
+        """Run a stream to completion: each request is submitted once the
+        server clock passes its ``arrival``; returns :func:`summary`."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        self.t0 = time.perf_counter()
+        while pending or self.queue or self.active:
+            now = self.now()
+            while pending and pending[0].arrival <= now:
+                self.submit(pending.popleft())
+            if not self.queue and not self.active:
+                time.sleep(max(0., pending[0].arrival - self.now()))
+                continue
+            self.step()
+        self.close()
+        return summary(self.done)
+
+    def close(self):
+        for g in self._plans.values():
+            g.close()
+        self._plans.clear()
+
+
+def serve_one_at_a_time(requests, conv_tol=1e-10, max_cycle=100, k_precision='fp64'):
+    """The plain path: each request solved alone with :class:`scf.RHF` (one
+    ``get_jk`` per cycle) in arrival order, waiting for its arrival."""
+    t0 = time.perf_counter()
+    out = []
+    for r in sorted(requests, key=lambda r: r.arrival):
+        wait = r.arrival - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        r.t_admit = time.perf_counter() - t0
+        mf = RHF(r.mol, conv_tol=conv_tol, max_cycle=max_cycle, k_precision=k_precision)
+        r.e_tot = mf.kernel()
+        r.cycles, r.converged, r.history = mf.cycles, mf.converged, mf.history
+        _dev.torch().cuda.synchronize()
+        r.t_done = time.perf_counter() - t0
+        out.append(r)
+    return summary(out)
+
+
+def summary(reqs):
+    """Throughput and latency of a finished stream."""
+    if not reqs:
+        return {'n': 0}
+    lat = np.array([r.latency for r in reqs])
+    t_first = min(r.arrival for r in reqs)
+    t_last = max(r.t_done for r in reqs)
+    span = max(t_last - t_first, 1e-9)
+    return {'n': len(reqs), 'span_s': span, 'solves_per_hr': 3600. * len(reqs) / span,
+            'p50_s': float(np.percentile(lat, 50)), 'p99_s': float(np.percentile(lat, 99)),
+            'mean_s': float(lat.mean()), 'converged': int(sum(bool(r.converged) for r in reqs)),
+            'cycles': int(sum(r.cycles for r in reqs))}

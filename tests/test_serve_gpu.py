@@ -1,0 +1,79 @@
+"""Continuous-batching serving: each request's energy is the same alone and
+in the mix (requests arriving while others are mid-solve), the resident bytes
+never exceed the admission budget, and the mixed-precision schedule lands
+within 1e-8 Eh of the all-FP64 solve."""
+import numpy as np
+import pytest
+
+
+def _mols():
+    from electrolites_amd.gto import water_cluster
+    return [water_cluster(2, seed=11, side=5.0), water_cluster(3, seed=12, side=5.5, li=True),
+            water_cluster(1, seed=13, side=4.0, li=True), water_cluster(4, seed=14, side=6.0),
+            water_cluster(2, seed=15, side=5.0, li=True)]
+
+
+def test_footprint_and_streams():
+    from electrolites_amd import serve
+    mols = serve.molecule_mix(6, seed=3, lo=3, hi=40)
+    assert [m.natm for m in mols] == [m.natm for m in serve.molecule_mix(6, seed=3, lo=3, hi=40)]
+    for i, m in enumerate(mols):
+        nw = (m.natm - (i % 2)) // 3
+        assert 3 <= nw <= 40 and m.charge == i % 2
+        nao, naux = m.nao, m.naux
+        assert serve.footprint(m) >= 8 * naux * nao * (nao + 1) // 2
+    t = serve.poisson_arrivals(2000, rate=4., seed=1)
+    assert np.all(np.diff(t) >= 0) and abs(t[-1] / 2000 - .25) < .03
+    s = serve.Server(hbm_budget=1e6)
+    with pytest.raises(ValueError, match='does not fit'):
+        s.submit(serve.Request(mols[0]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_alone_equals_mix(precision):
+    from electrolites_amd import serve
+    from electrolites_amd.scf import RHF
+    mols = _mols()
+    # in the mix: a new request joins every other tick while earlier ones run
+    srv = serve.Server(conv_tol=1e-10)
+    reqs = [serve.Request(m, rid=i, k_precision=precision) for i, m in enumerate(mols)]
+    pending = list(reqs)
+    peak = 0
+    while pending or srv.queue or srv.active:
+        if pending and srv.ticks % 2 == 0:
+            srv.submit(pending.pop(0))
+        srv.step()
+        peak = max(peak, len(srv.active))
+    srv.close()
+    assert peak >= 3, 'requests did not overlap'
+    for r in reqs:
+        assert r.converged
+        alone = serve.Server(conv_tol=1e-10)
+        a = serve.Request(r.mol, k_precision=precision)
+        alone.serve([a])
+        mf = RHF(r.mol, conv_tol=1e-10, k_precision=precision)
+        e = mf.kernel()
+        assert abs(r.e_tot - a.e_tot) <= 1e-12, (r.rid, r.e_tot, a.e_tot)
+        assert abs(r.e_tot - e) <= 1e-12, (r.rid, r.e_tot, e)
+        assert r.cycles == a.cycles == mf.cycles
+        if precision == 'mixed':
+            assert 'fp32' in [h['k_precision'] for h in r.history]
+            e64 = RHF(r.mol, conv_tol=1e-10).kernel()
+            assert abs(r.e_tot - e64) <= 1e-8
+
+
+@pytest.mark.gpu
+def test_admission_budget():
+    from electrolites_amd import serve
+    mols = _mols()
+    reqs = [serve.Request(m, rid=i, k_precision='mixed') for i, m in enumerate(mols)]
+    big = sorted(r.bytes for r in reqs)
+    budget = big[-1] + big[-2]                  # at most two of the largest resident
+    srv = serve.Server(hbm_budget=budget, conv_tol=1e-9)
+    stats = srv.serve(reqs)
+    assert stats['n'] == len(reqs) and stats['converged'] == len(reqs)
+    assert max(tr[3] for tr in srv.trace) <= budget
+    assert srv.resident == 0
+    for r in reqs:
+        assert r.t_admit >= r.arrival and r.t_done > r.t_admit
