@@ -41,6 +41,14 @@ def parse():
     ap.add_argument('--synthetic-cderi', action='store_true')
     ap.add_argument('--cpu-sample-j', type=int, default=96)
     ap.add_argument('--cpu-sample-k', type=int, default=24)
+    ap.add_argument('--workload', choices=('getjk', 'batch', 'serve'), default='getjk',
+                    help='getjk: one (H2O)64 get_jk (default); batch: one grouped J/K launch '
+                         'over a molecule mix; serve: continuous-batching DF-SCF serving')
+    ap.add_argument('--nmol', type=int, default=48, help='batch: molecules')
+    ap.add_argument('--requests', type=int, default=24, help='serve: requests per rate')
+    ap.add_argument('--rates', default='0.5,2,8', help='serve: Poisson rates (req/s)')
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--graphs', action='store_true', help='serve: replay grouped runs as HIP graphs')
     return ap.parse_args()
 
 
@@ -84,6 +92,162 @@ def cpu_baseline(mol_nao, naux, npairs, nocc, args):
             'j_ms': tj * 1e3, 'k_ms': tk * 1e3}
 
 
+class _ResidentDF:
+    """The two fields the DF path reads, around a cderi already in HBM."""
+
+    def __init__(self, cderi, idx=None):
+        import torch
+        self._cderi = {torch.cuda.current_device(): cderi}
+        self._cderi_idx = idx
+
+    def build(self):
+        pass
+
+
+def bench_batch(args):
+    """One grouped J/K launch over a molecule mix: ``--nmol`` molecules with
+    a log-uniform number of waters in [3, 40], every second one with a Li+ at
+    the centroid; each carries 1-8 densities: UHF alpha/beta (mo-tagged,
+    random orthonormal occupied orbitals) and random symmetric response
+    densities (N(0,1) entries scaled by 1e-3, full rank: two-factor K).
+    Timed: the grouped run (autotuned step-1 width) against the same items
+    through the single-molecule entry one after another."""
+    import torch
+    from electrolites_amd import _dev, batch, fastdf, fastdfcderi, serve
+    _dev.require_gpu()
+    mols = serve.molecule_mix(args.nmol, seed=args.seed)
+    rng = np.random.default_rng(args.seed + 1)
+    sets = []
+    t0 = time.perf_counter()
+    for mol in mols:
+        cderi, idx = fastdfcderi.cholesky_eri(mol)
+        df = _ResidentDF(cderi, idx)
+        nao, nocc = mol.nao, mol.nocc
+        n_dm = int(rng.integers(1, 9))
+        nuhf = min(2, n_dm)
+        cs = np.array([np.linalg.qr(rng.standard_normal((nao, nocc)))[0] for _ in range(nuhf)])
+        uhf = torch.from_numpy(np.einsum('nik,njk->nij', cs, cs)).cuda()
+        fastdf.tag_array(uhf, mo_coeff=torch.from_numpy(cs).cuda(),
+                         mo_occ=torch.ones((nuhf, nocc), dtype=torch.float64, device='cuda'))
+        sets.append((df, uhf))
+        if n_dm > 2:
+            a = rng.standard_normal((n_dm - 2, nao, nao)) * 1e-3
+            sets.append((df, torch.from_numpy(.5 * (a + a.transpose(0, 2, 1))).cuda()))
+    torch.cuda.synchronize()
+    log(f'{len(mols)} cderi built in {time.perf_counter() - t0:.1f} s; '
+        f'{len(sets)} items, {sum(d.shape[0] for _, d in sets)} densities')
+
+    entries = []
+    for df, dm in sets:
+        cd = fastdf._cderi_of(df)
+        nao = dm.shape[-1]
+        lay = fastdf._layout(df, cd, nao)
+        fl, fr = fastdf.factorize_dm(dm, 1)
+        d = fastdf._sparse_dm(dm, lay, 1)
+        e = batch.Entry(cd, nao, dm.shape[0], fl.shape[-1], dm_pairs=d, fl=fl, fr=fr, pair_map=lay.map)
+        e.layout = lay
+        entries.append(e)
+    t0 = time.perf_counter()
+    g = batch.GroupedJK(entries, tune=True)
+    torch.cuda.synchronize()
+    log(f'plan + autotune {time.perf_counter() - t0:.1f} s: bn {g.bn}, tiles {g.tiles()}')
+    for _ in range(args.warmup):
+        g.run()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(args.steps):
+        g.run()
+    ev[1].record()
+    torch.cuda.synchronize()
+    t_grouped = ev[0].elapsed_time(ev[1]) * 1e-3 / args.steps
+
+    # the same items one molecule at a time through the single entry
+    ev2 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev2[0].record()
+    singles = [fastdf.get_jk(df, dm, hermi=1) for df, dm in sets]
+    ev2[1].record()
+    torch.cuda.synchronize()
+    t_single = ev2[0].elapsed_time(ev2[1]) * 1e-3
+    same = all(torch.equal(e.vk.reshape(vk.shape), vk) for e, (vj, vk) in zip(entries, singles))
+    vj_same = True
+    for e, (vj, vk) in zip(entries, singles):
+        full = torch.empty((e.n_dm, e.nao, e.nao), dtype=torch.float64, device='cuda')
+        from electrolites_amd import _lib
+        _lib.call('mdf_unpack_pairs', _dev.ptr(e.vj_pairs), e.n_dm, e.npairs, _dev.ptr(e.layout.addr),
+                  e.npairs, e.nao, _dev.ptr(full), e.nao * e.nao, _dev.stream_ptr())
+        vj_same &= bool(torch.equal(full.reshape(vj.shape), vj))
+    torch.cuda.synchronize()
+    fk = 0.
+    for e in entries:
+        two = 2 if e.fr is not None else 1
+        fk += e.n_dm * (2. * two * e.naux * e.nao * e.nao * e.nocc +
+                        (1. if two == 2 else .5) * 2 * e.naux * e.nocc * e.nao * e.nao)
+    ndens = sum(e.n_dm for e in entries)
+    out = {
+        'metric': 'grouped J/K densities per second (48-molecule mix, FP64)',
+        'value': ndens / t_grouped, 'unit': 'densities/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': t_grouped * 1e3,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic geometries and densities; cderi built on the GPU from the integrals',
+        'config': {'workload': f'{len(mols)} molecules, (H2O)3-40 (+Li+ on half), 1-8 densities each',
+                   'items': len(entries), 'densities': ndens, 'bn': g.bn, 'tiles': g.tiles(),
+                   'nao': [m.nao for m in mols]},
+        'single_entry_s': t_single, 'grouped_s': t_grouped, 'speedup_vs_single': t_single / t_grouped,
+        'bit_identical_to_single': {'k': bool(same), 'j': bool(vj_same)},
+        'k_tflops': fk / t_grouped / 1e12, 'k_frac_fp64_peak': fk / t_grouped / FP64_MFMA_PEAK,
+    }
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Continuous-batching DF-SCF serving over the molecule mix (as
+    ``bench_batch``): for each Poisson rate, ``--requests`` energy requests
+    served with mixed precision (FP32 K until the DIIS error is below 1e-5,
+    FP64 finish), then the same stream solved one request at a time with the
+    plain FP64 driver.  Reports solves/hr and p50/p99 latency per rate."""
+    import torch
+    from electrolites_amd import _dev, serve
+    _dev.require_gpu()
+    mols = serve.molecule_mix(args.requests, seed=args.seed)
+    warm = serve.molecule_mix(1, seed=99, lo=3, hi=3)
+    serve.Server().serve([serve.Request(warm[0])])
+    serve.serve_one_at_a_time([serve.Request(warm[0])])
+    torch.cuda.synchronize()
+    rates = [float(x) for x in args.rates.split(',')]
+    per_rate = {}
+    for rate in rates:
+        arr = serve.poisson_arrivals(len(mols), rate, seed=args.seed + 7)
+        reqs = [serve.Request(m, arr[i], rid=i, k_precision='mixed') for i, m in enumerate(mols)]
+        srv = serve.Server(graphs=args.graphs)
+        s = srv.serve(reqs)
+        s['max_active'] = max(tr[1] for tr in srv.trace) if srv.trace else 0
+        s['ticks'] = srv.ticks
+        base = [serve.Request(m, arr[i], rid=i) for i, m in enumerate(mols)]
+        b = serve.serve_one_at_a_time(base)
+        de = max(abs(r.e_tot - q.e_tot) for r, q in zip(sorted(reqs, key=lambda r: r.rid), base))
+        per_rate[str(rate)] = {'serving': s, 'one_at_a_time_fp64': b, 'max_abs_dE_vs_fp64': de,
+                               'speedup': s['solves_per_hr'] / b['solves_per_hr']}
+        log(f'rate {rate}: serving {s["solves_per_hr"]:.0f}/hr p50 {s["p50_s"]:.2f} p99 {s["p99_s"]:.2f}; '
+            f'fp64 one-at-a-time {b["solves_per_hr"]:.0f}/hr p50 {b["p50_s"]:.2f} p99 {b["p99_s"]:.2f}; '
+            f'max |dE| {de:.2e}')
+    top = per_rate[str(max(rates))]
+    out = {
+        'metric': 'DF-SCF solves/hr, continuous batching, mixed precision (molecule mix)',
+        'value': top['serving']['solves_per_hr'], 'unit': 'solves/hr',
+        'n_gpus': 1, 'steps': len(mols), 'warmup': 1,
+        'ms_per_step': 3.6e6 / top['serving']['solves_per_hr'],
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64+f32',
+        'data': 'synthetic geometries (seeded water clusters, Li+ on half); Poisson arrivals',
+        'config': {'workload': f'{len(mols)} requests, (H2O)3-40 (+Li+ on half), rates {rates} req/s',
+                   'nao': [m.nao for m in mols]},
+        'speedup_vs_fp64_one_at_a_time': top['speedup'],
+        'per_rate': per_rate,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -91,6 +255,10 @@ def main():
               'the path shards along the aux index across GPUs but is built '
               'and measured on one', file=sys.stderr)
         sys.exit(2)
+    if args.workload == 'batch':
+        return bench_batch(args)
+    if args.workload == 'serve':
+        return bench_serve(args)
     import torch
     from electrolites_amd import _dev, _lib, fastdf
     from electrolites_amd.gto import water_cluster
@@ -167,19 +335,24 @@ def main():
     tk = np.mean([e[2].elapsed_time(e[3]) for e in evs]) * 1e-3
     ms_step = wall / args.steps * 1e3
 
-    # the same K in FP32 (the mixed-precision cycles), outside the timed steps
+    # the same K in the mixed-precision modes, outside the timed steps
     fl, fr = fastdf.factorize_dm(dm, 1)
     lay = fastdf._layout(dfobj, cderi, nao)
-    fastdf._vk(cderi, lay, fl, fr, 'fp32')
-    e32 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    e32[0].record()
-    vk32 = fastdf._vk(cderi, lay, fl, fr, 'fp32')
-    e32[1].record()
     vk64 = fastdf._vk(cderi, lay, fl, fr, 'fp64')
-    torch.cuda.synchronize()
-    tk32 = e32[0].elapsed_time(e32[1]) * 1e-3
-    k32_err = float((vk32 - vk64).abs().max() / vk64.abs().max())
-    del vk32, vk64
+    low = {}
+    for prec in ('fp32', 'bf16x3'):
+        fastdf._vk(cderi, lay, fl, fr, prec)
+        e32 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e32[0].record()
+        vkl = fastdf._vk(cderi, lay, fl, fr, prec)
+        e32[1].record()
+        torch.cuda.synchronize()
+        low[prec] = (e32[0].elapsed_time(e32[1]) * 1e-3,
+                     float((vkl - vk64).abs().max() / vk64.abs().max()))
+        del vkl
+        log(f'K {prec}: {low[prec][0] * 1e3:.1f} ms, rel err {low[prec][1]:.2e}')
+    del vk64
+    tk32, k32_err = low['fp32']
 
     S = naux * npairs * 8
     bj = 2 * S + 16 * (npairs + naux)
@@ -200,7 +373,8 @@ def main():
                    'naux': naux, 'npairs': npairs, 'nocc': nocc, 'n_dm': 1,
                    'cderi_GB': S / 1e9},
         'get_jk_ms': {'j': tj * 1e3, 'k': tk * 1e3, 'k_fp32': tk32 * 1e3,
-                      'k_fp32_rel_err_vs_fp64': k32_err},
+                      'k_fp32_rel_err_vs_fp64': k32_err, 'k_bf16x3': low['bf16x3'][0] * 1e3,
+                      'k_bf16x3_rel_err_vs_fp64': low['bf16x3'][1]},
         'cderi_build': build,
         'roofline': {'bound': 'mfma', 'achieved': fk / tk / 1e12,
                      'peak': FP64_MFMA_PEAK / 1e12, 'unit': 'TFLOP/s',

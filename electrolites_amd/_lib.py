@@ -9,7 +9,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, 'libmi355df.so')
-ABI_VERSION = 2
+ABI_VERSION = 3
 PREC_FP64 = 0
 PREC_FP32 = 1
 
@@ -42,7 +42,7 @@ _SIGS = {
     'mdf_trtri_workspace': (SZ, [L]),
     'mdf_trtri': (I, [P, L, L, P, P, L, P, SZ, P]),
     'mdf_trmm': (I, [P, L, L, P, L, L, P, L, P]),
-    'mdf_grouped_plan_create': (I, [P, I, I, I, I, I, ctypes.POINTER(P), ctypes.POINTER(SZ)]),
+    'mdf_grouped_plan_create': (I, [P, I, I, I, I, I, SZ, ctypes.POINTER(P), ctypes.POINTER(SZ)]),
     'mdf_grouped_jk': (I, [P, P, SZ, P]),
     'mdf_grouped_plan_destroy': (I, [P]),
     'mdf_grouped_plan_info': (I, [P, ctypes.POINTER(L), I]),

@@ -73,8 +73,10 @@ BN_CANDIDATES = (64, 128, 160)
 class GroupedJK:
     """A device plan for one batch of :class:`Entry` objects."""
 
-    def __init__(self, entries, with_j=True, with_k=True, precision='fp64', bn=None, tune=True):
+    def __init__(self, entries, with_j=True, with_k=True, precision='fp64', bn=None, tune=True,
+                 work_limit=0):
         self.entries = list(entries)
+        self.work_limit = int(work_limit)
         self.with_j, self.with_k = with_j, with_k
         self.precision = precision
         self.prec = fastdf.PRECISIONS[precision]
@@ -95,7 +97,7 @@ class GroupedJK:
         plan = P()
         wb = ctypes.c_size_t()
         _lib.call('mdf_grouped_plan_create', ctypes.cast(self._arr, P), len(self.entries), int(self.with_j),
-                  int(self.with_k), self.prec, bn, ctypes.byref(plan), ctypes.byref(wb))
+                  int(self.with_k), self.prec, bn, self.work_limit, ctypes.byref(plan), ctypes.byref(wb))
         return plan, wb.value
 
     def _autotune(self, sig):
@@ -124,9 +126,10 @@ class GroupedJK:
         return best
 
     def tiles(self):
-        out = (ctypes.c_long * 7)()
-        _lib.call('mdf_grouped_plan_info', self._plan, out, 7)
-        names = ('j_pass1', 'j_reduce1', 'j_pass2', 'j_reduce2', 'k_step1', 'k_step2', 'k_finalize')
+        out = (ctypes.c_long * 9)()
+        _lib.call('mdf_grouped_plan_info', self._plan, out, 9)
+        names = ('j_pass1', 'j_reduce1', 'j_pass2', 'j_reduce2', 'k_step1', 'k_step2', 'k_finalize',
+                 'k_waves', 'k_launches')
         return dict(zip(names, list(out)))
 
     def run(self):
@@ -164,10 +167,11 @@ class GroupedJK:
             pass
 
 
-def batched_get_jk(dfobjs, dms_list, hermi=1, precision='fp64'):
+def batched_get_jk(dfobjs, dms_list, hermi=1, precision='fp64', work_limit=0):
     """``[(vj, vk), ...]`` for each (DF object, densities) pair in one grouped
     run.  Densities follow :func:`fastdf.get_jk` (tagged or not); outputs are
-    device tensors shaped like the inputs."""
+    device tensors shaped like the inputs.  ``work_limit`` bounds the K
+    workspace of one wave of items (bytes, 0 = the library default)."""
     t = _dev.torch()
     entries, shapes = [], []
     for dfobj, dms in zip(dfobjs, dms_list):
@@ -181,7 +185,7 @@ def batched_get_jk(dfobjs, dms_list, hermi=1, precision='fp64'):
         e.layout = lay
         entries.append(e)
         shapes.append(tuple(np.shape(dms)))
-    g = GroupedJK(entries, precision=precision)
+    g = GroupedJK(entries, precision=precision, work_limit=work_limit)
     g.run()
     out = []
     for e, shp in zip(entries, shapes):

@@ -10,8 +10,16 @@
 // plan owns), so a run can be captured in a HIP graph and replayed.
 //
 // Each item is cut exactly as the single-item entries cut it (mdf_jcore.h /
-// mdf_kcore.h plans depend on the item's own shape only) and runs the same
+// mdf_kcore.h plans depend on the item's own shape only: J chunks and row
+// groups, K passes of block_rows() aux rows, step-2 splits) and runs the same
 // tile bodies, so an item's J and K are bit-identical alone and in a batch.
+//
+// K runs in waves: consecutive items whose transformed blocks and step-2
+// partials fit the workspace limit together share one launch per pass
+// (step 1 of pass b for every item of the wave, then step 2 of pass b), and
+// the wave ends with its finalize launch; the next wave reuses the same
+// workspace.  J runs before K, so K's workspace overlaps J's.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -45,13 +53,25 @@ struct GKItem {
     const int *pmap;
     const double *fl, *fr;  // (nao, nocc); fr NULL -> symmetric
     double *vk;             // (nao, nao)
-    void *rl, *rr;          // transformed blocks (T)
+    void *rl, *rr;          // transformed blocks (T), nL aux rows each
     double *part;           // ksplit x nao x nao
+    long nL;                // aux rows per pass
     int nao, nocc, ksplit, ntile, vec;
+    // split-BF16 (prec 2): workspace offsets of [side][FT hi, FT lo, R hi, R lo]
+    void *b3[2][4];
+    long ldr, ldft;
 };
 
 struct Tile {
     int item, a, b, c;
+};
+
+// A launch of the K sequence: kind 0 = factor split (prec 2), 1 = step 1
+// (4 = step 1 of items with a pair map, prec 2), 2 = step 2, 3 = finalize,
+// over tiles [start, start + count) of that kind's table.
+struct Phase {
+    int kind;
+    long start, count;
 };
 
 }  // namespace
@@ -64,7 +84,10 @@ struct mdf_grouped_plan {
     void *dev;                          // owned device memory: items + tables
     GJItem *jitems;
     GKItem *kitems;
-    Tile *j1, *jr1, *j2, *jr2, *k1, *k2, *kf;
+    Tile *j1, *jr1, *j2, *jr2, *k0, *k1, *k2, *kf;
+    long t_k0;
+    std::vector<Phase> kphases;
+    int kwaves;
     size_t work_bytes;
     // host copies of the per-item workspace offsets (relative to work)
     std::vector<GJItem> hj;
@@ -74,6 +97,7 @@ struct mdf_grouped_plan {
 namespace {
 
 constexpr long RED_CHUNK = 4096;    // elements per reduction tile
+constexpr size_t MDF_GROUPED_DEFAULT_LIMIT = 32ull << 30;   // K wave workspace
 
 __global__ __launch_bounds__(jcore::J_THREADS) void gj_pass1(const GJItem *items, const Tile *tiles, long n,
                                                              char *work) {
@@ -137,7 +161,8 @@ __global__ __launch_bounds__(G_THREADS) void gk_step1(const GKItem *items, const
         const bool right = (e.c >> 16) != 0;
         const int ktile = e.c & 0xffff;
         const double *F = right ? it.fr : it.fl;
-        T *R = (T *)(work + (size_t)(right ? it.rr : it.rl)) + (long)e.a * it.nao * it.nocc;
+        const long lb = e.a % it.nL;        // row within the pass
+        T *R = (T *)(work + (size_t)(right ? it.rr : it.rl)) + lb * it.nao * it.nocc;
         const double *row = it.cderi + (long)e.a * it.ld;
         if (it.pmap)
             kcore::step1_tile<BN, true, T>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
@@ -154,8 +179,10 @@ __global__ __launch_bounds__(G_THREADS, 2) void gk_step2(const GKItem *items, co
                                                          char *work) {
     __shared__ T lds[TileShape<kcore::S2_B, kcore::S2_B>::LDS_ELEMS];
     for (long t = blockIdx.x; t < n; t += gridDim.x) {
-        const Tile e = tiles[t];            // (item, tile index, split)
+        const Tile e = tiles[t];            // (item, tile index, split, pass)
         const GKItem &it = items[e.item];
+        const long L0 = (long)e.c * it.nL;
+        const long nLb = min(it.nL, it.naux - L0);
         int ti, tj;
         if (it.fr == nullptr) {
             kcore::tri_tile(e.a, ti, tj);
@@ -164,17 +191,82 @@ __global__ __launch_bounds__(G_THREADS, 2) void gk_step2(const GKItem *items, co
             tj = e.a % it.ntile;
         }
         const int kchunks = (it.nocc + G_BK - 1) / G_BK;
-        const long nsteps = it.naux * kchunks;
+        const long nsteps = nLb * kchunks;
         const long s0 = nsteps * e.b / it.ksplit, s1 = nsteps * (e.b + 1) / it.ksplit;
         const T *rl = (const T *)(work + (size_t)it.rl);
         const T *rr = it.fr ? (const T *)(work + (size_t)it.rr) : rl;
         double *o = (double *)(work + (size_t)it.part) + (long)e.b * it.nao * it.nao;
         if (it.vec)
             kcore::step2_tile<true, T>(rl, rr, it.nao, it.nocc, it.nocc, (long)it.nao * it.nocc, ti * kcore::S2_B,
-                                       tj * kcore::S2_B, s0, s1, o, 0, lds);
+                                       tj * kcore::S2_B, s0, s1, o, e.c > 0, lds);
         else
             kcore::step2_tile<false, T>(rl, rr, it.nao, it.nocc, it.nocc, (long)it.nao * it.nocc, ti * kcore::S2_B,
-                                        tj * kcore::S2_B, s0, s1, o, 0, lds);
+                                        tj * kcore::S2_B, s0, s1, o, e.c > 0, lds);
+        __syncthreads();
+    }
+}
+
+// ---- split-BF16 (prec 2) counterparts
+__global__ __launch_bounds__(256) void gk_split_ft(const GKItem *items, const Tile *tiles, long n, char *work) {
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];            // (item, chunk, side)
+        const GKItem &it = items[e.item];
+        using kcore::kb3::u16;
+        const long e0 = (long)e.a * 1024, e1 = min(e0 + 1024, (long)it.nocc * it.ldft);
+        kcore::kb3::split_ft_range(e.b ? it.fr : it.fl, it.nocc, it.nao, it.nocc,
+                                   (u16 *)(work + (size_t)it.b3[e.b][0]), (u16 *)(work + (size_t)it.b3[e.b][1]),
+                                   it.ldft, e0, e1);
+    }
+}
+
+template <bool MAPPED>
+__global__ __launch_bounds__(G_THREADS, 2) void gk_step1_b3(const GKItem *items, const Tile *tiles, long n,
+                                                             char *work) {
+    using kcore::kb3::u16;
+    __shared__ u16 lds[b3::Shape<kcore::kb3::BM, kcore::kb3::BN>::LDS_ELEMS];
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];            // (item, L, i-tile, k-tile | side << 16)
+        const GKItem &it = items[e.item];
+        const int side = e.c >> 16, ktile = e.c & 0xffff;
+        const long lb = e.a % it.nL;
+        const long off = lb * it.nao * it.ldr;
+        const u16 *fthi = (const u16 *)(work + (size_t)it.b3[side][0]);
+        const u16 *ftlo = (const u16 *)(work + (size_t)it.b3[side][1]);
+        u16 *rhi = (u16 *)(work + (size_t)it.b3[side][2]) + off;
+        u16 *rlo = (u16 *)(work + (size_t)it.b3[side][3]) + off;
+        const double *row = it.cderi + (long)e.a * it.ld;
+        // a phase holds only mapped or only unmapped items (one instance
+        // each keeps the register budget of the single-item kernel)
+        kcore::kb3::step1_tile<MAPPED>(row, it.pmap, it.nao, fthi, ftlo, it.ldft, it.nocc, rhi, rlo, it.ldr,
+                                       e.b * kcore::kb3::BM, ktile * kcore::kb3::BN, lds);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(G_THREADS, 2) void gk_step2_b3(const GKItem *items, const Tile *tiles, long n,
+                                                             char *work) {
+    using kcore::kb3::u16;
+    __shared__ u16 lds[b3::Shape<kcore::kb3::BM, kcore::kb3::BN>::LDS_ELEMS];
+    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+        const Tile e = tiles[t];            // (item, tile index, split, pass)
+        const GKItem &it = items[e.item];
+        const long L0 = (long)e.c * it.nL;
+        const long nLb = min(it.nL, it.naux - L0);
+        int ti, tj;
+        if (it.fr == nullptr) {
+            kcore::tri_tile(e.a, ti, tj);
+        } else {
+            ti = e.a / it.ntile;
+            tj = e.a % it.ntile;
+        }
+        const long nsteps = nLb * (it.ldr / b3::BK);
+        const long s0 = nsteps * e.b / it.ksplit, s1 = nsteps * (e.b + 1) / it.ksplit;
+        const int rs = it.fr ? 1 : 0;
+        double *o = (double *)(work + (size_t)it.part) + (long)e.b * it.nao * it.nao;
+        kcore::kb3::step2_tile((const u16 *)(work + (size_t)it.b3[0][2]), (const u16 *)(work + (size_t)it.b3[0][3]),
+                               (const u16 *)(work + (size_t)it.b3[rs][2]), (const u16 *)(work + (size_t)it.b3[rs][3]),
+                               it.nao, it.ldr, (long)it.nao * it.ldr, ti * kcore::S2_B, tj * kcore::S2_B, s0, s1, o,
+                               e.c > 0, lds);
         __syncthreads();
     }
 }
@@ -201,16 +293,19 @@ int persistent_grid(long ntiles, int per_cu) {
 }  // namespace
 
 extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int with_j, int with_k, int prec,
-                                       int bn, mdf_grouped_plan **out, size_t *work_bytes) {
+                                       int bn, size_t work_limit, mdf_grouped_plan **out, size_t *work_bytes) {
     MDF_CHECK_ARG(items && nitems >= 0 && out && work_bytes, "mdf_grouped_plan_create: null argument");
-    MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32, "mdf_grouped_plan_create: precision %d", prec);
+    MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32 || prec == MDF_PREC_BF16X3,
+                  "mdf_grouped_plan_create: precision %d", prec);
     MDF_CHECK_ARG(bn == 0 || bn == 64 || bn == 128 || bn == 160, "mdf_grouped_plan_create: bn %d not in {0,64,128,160}", bn);
     const size_t elem = prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double);
     mdf_grouped_plan *p = new mdf_grouped_plan();
     p->with_j = with_j;
     p->with_k = with_k;
     p->prec = prec;
-    std::vector<Tile> j1, jr1, j2, jr2, k1, k2, kf;
+    std::vector<Tile> j1, jr1, j2, jr2, k0, k1, k2, kf;
+    std::vector<int> kgrid;             // step-2 output tiles per K item
+    if (work_limit == 0) work_limit = MDF_GROUPED_DEFAULT_LIMIT;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off = align_up(off + bytes, 256); return o; };
     int maxocc = 0;
@@ -255,7 +350,10 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                 return MDF_EINVAL;
             }
             const bool two = x.fr != nullptr;
-            kcore::KPlan kp = kcore::plan_k(x.naux, x.nao, x.nocc, two, x.naux);
+            const bool b3 = prec == MDF_PREC_BF16X3;
+            const long nL = b3 ? kcore::block_rows(x.naux, x.nao, (int)kcore::kb3::pad32(x.nocc), two, 4)
+                               : kcore::block_rows(x.naux, x.nao, x.nocc, two, elem);
+            kcore::KPlan kp = kcore::plan_k(x.naux, x.nao, x.nocc, two, nL);
             for (int d = 0; d < x.n_dm; d++) {
                 GKItem g{};
                 g.cderi = x.cderi; g.ld = x.ld; g.naux = x.naux; g.pmap = x.pair_map;
@@ -263,27 +361,111 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                 g.fr = two ? x.fr + (long)d * x.nao * x.nocc : nullptr;
                 g.vk = x.vk + (long)d * x.nao * x.nao;
                 g.nao = x.nao; g.nocc = x.nocc; g.ksplit = kp.ksplit; g.ntile = kp.ntile;
+                g.nL = kp.nL;
                 g.vec = (x.nocc % (16 / (int)elem)) == 0;
-                g.rl = (void *)take(kp.r_words * elem);
-                g.rr = two ? (void *)take(kp.r_words * elem) : nullptr;
-                g.part = (double *)take(kp.part_words * sizeof(double));
-                const int ki = (int)p->hk.size();
-                const int nkt = (int)cdiv(x.nocc, p->bn), nit = (int)cdiv(x.nao, 128);
-                for (int side = 0; side < (two ? 2 : 1); side++)
-                    for (long L = 0; L < x.naux; L++)
-                        for (int it = 0; it < nit; it++)
-                            for (int kt = 0; kt < nkt; kt++) k1.push_back({ki, (int)L, it, kt | (side << 16)});
-                for (int s = 0; s < kp.ksplit; s++)
-                    for (int tl = 0; tl < kp.ntiles_grid; tl++) k2.push_back({ki, tl, s, 0});
-                for (long e = 0; e < cdiv((long)x.nao * x.nao, RED_CHUNK); e++) kf.push_back({ki, (int)e, 0, 0});
+                // workspace offsets are assigned per wave below; keep the sizes
+                if (b3) {
+                    const kcore::kb3::Layout y = kcore::kb3::layout(x.nao, x.nocc, kp.nL, kp.ksplit, two);
+                    g.ldr = y.ldr;
+                    g.ldft = y.ldft;
+                    g.rl = (void *)0;
+                    g.part = (double *)y.total;
+                } else {
+                    g.rl = (void *)(align_up(kp.r_words * elem, 256));
+                    g.part = (double *)(align_up(kp.part_words * sizeof(double), 256));
+                }
+                g.rr = nullptr;
                 p->hk.push_back(g);
+                kgrid.push_back(kp.ntiles_grid);
             }
         }
     }
+    // K waves: consecutive items whose blocks and partials fit work_limit
+    // (at least one item per wave); every wave's workspace starts at 0
+    size_t kmax = 0;
+    p->kwaves = 0;
+    for (size_t w0 = 0; w0 < p->hk.size();) {
+        size_t used = 0, w1 = w0;
+        while (w1 < p->hk.size()) {
+            const GKItem &g = p->hk[w1];
+            const size_t need = (g.fr ? 2 : 1) * (size_t)g.rl + (size_t)g.part;
+            if (w1 > w0 && used + need > work_limit) break;
+            used += need;
+            w1++;
+        }
+        size_t o = 0;
+        long maxpass = 0;
+        for (size_t i = w0; i < w1; i++) {
+            GKItem &g = p->hk[i];
+            if (prec == MDF_PREC_BF16X3) {
+                const kcore::kb3::Layout y = kcore::kb3::layout(g.nao, g.nocc, g.nL, g.ksplit, g.fr != nullptr);
+                for (int side = 0; side < 2; side++) {
+                    const kcore::kb3::Side q = kcore::kb3::side_ptrs((char *)o, y, g.fr ? side : 0);
+                    g.b3[side][0] = q.fthi; g.b3[side][1] = q.ftlo; g.b3[side][2] = q.rhi; g.b3[side][3] = q.rlo;
+                }
+                g.part = (double *)o;
+                o += y.total;
+            } else {
+                const size_t rb = (size_t)g.rl, pb = (size_t)g.part;
+                g.rl = (void *)o; o += rb;
+                if (g.fr) { g.rr = (void *)o; o += rb; }
+                g.part = (double *)o; o += pb;
+            }
+            maxpass = std::max(maxpass, cdiv(g.naux, g.nL));
+        }
+        kmax = std::max(kmax, o);
+        if (prec == MDF_PREC_BF16X3) {
+            const long s0 = k0.size();
+            for (size_t i = w0; i < w1; i++) {
+                const GKItem &g = p->hk[i];
+                for (int side = 0; side < (g.fr ? 2 : 1); side++)
+                    for (long e = 0; e < cdiv((long)g.nocc * g.ldft, 1024L); e++) k0.push_back({(int)i, (int)e, side, 0});
+            }
+            p->kphases.push_back({0, s0, (long)k0.size() - s0});
+        }
+        for (long b = 0; b < maxpass; b++) {
+          // split-BF16: unmapped items' step-1 tiles, then mapped items' ones
+          for (int mp = 0; mp < (prec == MDF_PREC_BF16X3 ? 2 : 1); mp++) {
+            const long s1 = k1.size();
+            for (size_t i = w0; i < w1; i++) {
+                const GKItem &g = p->hk[i];
+                const long L0 = b * g.nL;
+                if (L0 >= g.naux) continue;
+                if (prec == MDF_PREC_BF16X3 && (g.pmap != nullptr) != (mp == 1)) continue;
+                const long L1 = std::min(L0 + g.nL, g.naux);
+                const int nkt = prec == MDF_PREC_BF16X3 ? (int)cdiv(g.ldr, (long)kcore::kb3::BN)
+                                                        : (int)cdiv(g.nocc, p->bn);
+                const int nit = (int)cdiv(g.nao, 128);
+                for (int side = 0; side < (g.fr ? 2 : 1); side++)
+                    for (long L = L0; L < L1; L++)
+                        for (int it = 0; it < nit; it++)
+                            for (int kt = 0; kt < nkt; kt++) k1.push_back({(int)i, (int)L, it, kt | (side << 16)});
+            }
+            p->kphases.push_back({mp ? 4 : 1, s1, (long)k1.size() - s1});
+          }
+            const long s2 = k2.size();
+            for (size_t i = w0; i < w1; i++) {
+                const GKItem &g = p->hk[i];
+                if (b * g.nL >= g.naux) continue;
+                for (int s = 0; s < g.ksplit; s++)
+                    for (int tl = 0; tl < kgrid[i]; tl++) k2.push_back({(int)i, tl, s, (int)b});
+            }
+            p->kphases.push_back({2, s2, (long)k2.size() - s2});
+        }
+        const long sf = kf.size();
+        for (size_t i = w0; i < w1; i++) {
+            const GKItem &g = p->hk[i];
+            for (long e = 0; e < cdiv((long)g.nao * g.nao, RED_CHUNK); e++) kf.push_back({(int)i, (int)e, 0, 0});
+        }
+        p->kphases.push_back({3, sf, (long)kf.size() - sf});
+        p->kwaves++;
+        w0 = w1;
+    }
+    off = std::max(off, kmax);
     p->nj = (int)p->hj.size();
     p->nk = (int)p->hk.size();
     p->t_j1 = j1.size(); p->t_jr1 = jr1.size(); p->t_j2 = j2.size(); p->t_jr2 = jr2.size();
-    p->t_k1 = k1.size(); p->t_k2 = k2.size(); p->t_kf = kf.size();
+    p->t_k0 = k0.size(); p->t_k1 = k1.size(); p->t_k2 = k2.size(); p->t_kf = kf.size();
     p->work_bytes = off + 256;
     // device block: items then tables
     size_t dsz = 0;
@@ -293,6 +475,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     const size_t o_j2 = dtake(j2.size() * sizeof(Tile)), o_jr2 = dtake(jr2.size() * sizeof(Tile));
     const size_t o_k1 = dtake(k1.size() * sizeof(Tile)), o_k2 = dtake(k2.size() * sizeof(Tile));
     const size_t o_kf = dtake(kf.size() * sizeof(Tile));
+    const size_t o_k0 = dtake(k0.size() * sizeof(Tile));
     std::vector<char> host(dsz + 256, 0);
     auto put = [&](size_t o, const void *src, size_t bytes) { if (bytes) memcpy(host.data() + o, src, bytes); };
     put(o_j, p->hj.data(), p->hj.size() * sizeof(GJItem));
@@ -304,6 +487,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     put(o_k1, k1.data(), k1.size() * sizeof(Tile));
     put(o_k2, k2.data(), k2.size() * sizeof(Tile));
     put(o_kf, kf.data(), kf.size() * sizeof(Tile));
+    put(o_k0, k0.data(), k0.size() * sizeof(Tile));
     hipError_t e = hipMalloc(&p->dev, host.size());
     if (e != hipSuccess) {
         set_error("mdf_grouped_plan_create: hipMalloc(%zu) failed: %s", host.size(), hipGetErrorString(e));
@@ -322,18 +506,19 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     p->j1 = (Tile *)(d + o_j1); p->jr1 = (Tile *)(d + o_jr1); p->j2 = (Tile *)(d + o_j2);
     p->jr2 = (Tile *)(d + o_jr2); p->k1 = (Tile *)(d + o_k1); p->k2 = (Tile *)(d + o_k2);
     p->kf = (Tile *)(d + o_kf);
+    p->k0 = (Tile *)(d + o_k0);
     *out = p;
     *work_bytes = p->work_bytes;
     return MDF_OK;
 }
 
 template <int BN>
-static void launch_gk1(const mdf_grouped_plan *p, char *w, hipStream_t st) {
-    const int g = persistent_grid(p->t_k1, 2);
+static void launch_gk1(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
+    const int g = persistent_grid(n, 2);
     if (p->prec == MDF_PREC_FP32)
-        hipLaunchKernelGGL((gk_step1<BN, float>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, p->k1, p->t_k1, w);
+        hipLaunchKernelGGL((gk_step1<BN, float>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
     else
-        hipLaunchKernelGGL((gk_step1<BN, double>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, p->k1, p->t_k1, w);
+        hipLaunchKernelGGL((gk_step1<BN, double>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
 }
 
 extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work_bytes, void *stream) {
@@ -359,21 +544,48 @@ extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work
         MDF_LAUNCHED("gj_reduce2");
     }
     if (p->with_k && p->nk) {
-        switch (p->bn) {
-        case 160: launch_gk1<160>(p, w, st); break;
-        case 128: launch_gk1<128>(p, w, st); break;
-        default: launch_gk1<64>(p, w, st); break;
+        const bool b3 = p->prec == MDF_PREC_BF16X3;
+        for (const Phase &ph : p->kphases) {
+            if (ph.count == 0) continue;
+            if (ph.kind == 0) {
+                hipLaunchKernelGGL(gk_split_ft, dim3(persistent_grid(ph.count, 4)), dim3(256), 0, st, p->kitems,
+                                   p->k0 + ph.start, ph.count, w);
+                MDF_LAUNCHED("gk_split_ft");
+            } else if ((ph.kind == 1 && b3) || ph.kind == 4) {
+                if (ph.kind == 4)
+                    hipLaunchKernelGGL(gk_step1_b3<true>, dim3(persistent_grid(ph.count, 2)), dim3(G_THREADS), 0,
+                                       st, p->kitems, p->k1 + ph.start, ph.count, w);
+                else
+                    hipLaunchKernelGGL(gk_step1_b3<false>, dim3(persistent_grid(ph.count, 2)), dim3(G_THREADS), 0,
+                                       st, p->kitems, p->k1 + ph.start, ph.count, w);
+                MDF_LAUNCHED("gk_step1_b3");
+            } else if (ph.kind == 2 && b3) {
+                hipLaunchKernelGGL(gk_step2_b3, dim3(persistent_grid(ph.count, 2)), dim3(G_THREADS), 0, st,
+                                   p->kitems, p->k2 + ph.start, ph.count, w);
+                MDF_LAUNCHED("gk_step2_b3");
+            } else if (ph.kind == 1) {
+                const Tile *t = p->k1 + ph.start;
+                switch (p->bn) {
+                case 160: launch_gk1<160>(p, t, ph.count, w, st); break;
+                case 128: launch_gk1<128>(p, t, ph.count, w, st); break;
+                default: launch_gk1<64>(p, t, ph.count, w, st); break;
+                }
+                MDF_LAUNCHED("gk_step1");
+            } else if (ph.kind == 2) {
+                const int g2 = persistent_grid(ph.count, 2);
+                if (p->prec == MDF_PREC_FP32)
+                    hipLaunchKernelGGL(gk_step2<float>, dim3(g2), dim3(G_THREADS), 0, st, p->kitems,
+                                       p->k2 + ph.start, ph.count, w);
+                else
+                    hipLaunchKernelGGL(gk_step2<double>, dim3(g2), dim3(G_THREADS), 0, st, p->kitems,
+                                       p->k2 + ph.start, ph.count, w);
+                MDF_LAUNCHED("gk_step2");
+            } else {
+                hipLaunchKernelGGL(gk_finalize, dim3(persistent_grid(ph.count, 4)), dim3(256), 0, st, p->kitems,
+                                   p->kf + ph.start, ph.count, w);
+                MDF_LAUNCHED("gk_finalize");
+            }
         }
-        MDF_LAUNCHED("gk_step1");
-        const int g2 = persistent_grid(p->t_k2, 2);
-        if (p->prec == MDF_PREC_FP32)
-            hipLaunchKernelGGL(gk_step2<float>, dim3(g2), dim3(G_THREADS), 0, st, p->kitems, p->k2, p->t_k2, w);
-        else
-            hipLaunchKernelGGL(gk_step2<double>, dim3(g2), dim3(G_THREADS), 0, st, p->kitems, p->k2, p->t_k2, w);
-        MDF_LAUNCHED("gk_step2");
-        hipLaunchKernelGGL(gk_finalize, dim3(persistent_grid(p->t_kf, 4)), dim3(256), 0, st, p->kitems, p->kf,
-                           p->t_kf, w);
-        MDF_LAUNCHED("gk_finalize");
     }
     return MDF_OK;
 }
@@ -389,5 +601,9 @@ extern "C" int mdf_grouped_plan_info(const mdf_grouped_plan *p, long *tiles, int
     MDF_CHECK_ARG(p && tiles && ntiles >= 7, "mdf_grouped_plan_info: bad arguments");
     tiles[0] = p->t_j1; tiles[1] = p->t_jr1; tiles[2] = p->t_j2; tiles[3] = p->t_jr2;
     tiles[4] = p->t_k1; tiles[5] = p->t_k2; tiles[6] = p->t_kf;
+    if (ntiles >= 9) {
+        tiles[7] = p->kwaves;
+        tiles[8] = (long)p->kphases.size();
+    }
     return MDF_OK;
 }

@@ -68,6 +68,41 @@ __global__ __launch_bounds__(256) void k_finalize(
                    min(e0 + 256, (long)nao * nao));
 }
 
+// ---- split-BF16 (prec 2)
+__global__ __launch_bounds__(256) void k_split_ft(const double *__restrict__ F, long ldf, int nao, int nocc,
+                                                  b3::u16 *hi, b3::u16 *lo, long ldft) {
+    const long e0 = (long)blockIdx.x * 1024;
+    kb3::split_ft_range(F, ldf, nao, nocc, hi, lo, ldft, e0, min(e0 + 1024, (long)nocc * ldft));
+}
+
+template <bool MAPPED>
+__global__ __launch_bounds__(G_THREADS, 2) void k_step1_b3(
+    const double *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao, const b3::u16 *fthi,
+    const b3::u16 *ftlo, long ldft, int nocc, b3::u16 *Rhi, b3::u16 *Rlo, long ldr, long strideL) {
+    __shared__ b3::u16 lds[b3::Shape<kb3::BM, kb3::BN>::LDS_ELEMS];
+    const long z = blockIdx.z;
+    kb3::step1_tile<MAPPED>(cderi + z * ld, pmap, nao, fthi, ftlo, ldft, nocc, Rhi + z * strideL,
+                            Rlo + z * strideL, ldr, blockIdx.y * kb3::BM, blockIdx.x * kb3::BN, lds);
+}
+
+template <bool SYM>
+__global__ __launch_bounds__(G_THREADS, 2) void k_step2_b3(
+    const b3::u16 *lhi, const b3::u16 *llo, const b3::u16 *rhi, const b3::u16 *rlo, int nao, long ldr,
+    long strideL, int nL, int ntile, double *__restrict__ part, long part_stride, int beta) {
+    __shared__ b3::u16 lds[b3::Shape<kb3::BM, kb3::BN>::LDS_ELEMS];
+    int ti, tj;
+    if (SYM) {
+        tri_tile(blockIdx.x, ti, tj);
+    } else {
+        ti = blockIdx.x / ntile;
+        tj = blockIdx.x % ntile;
+    }
+    const long nsteps = (long)nL * (ldr / b3::BK);
+    const long s0 = nsteps * blockIdx.y / gridDim.y, s1 = nsteps * (blockIdx.y + 1) / gridDim.y;
+    kb3::step2_tile(lhi, llo, rhi, rlo, nao, ldr, strideL, ti * S2_B, tj * S2_B, s0, s1,
+                    part + (long)blockIdx.y * part_stride, beta, lds);
+}
+
 template <int BN, typename T>
 void launch_step1(dim3 g, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
                   const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
@@ -143,16 +178,74 @@ int run_k(const KPlan &p, const double *cderi, long naux, long ld, const int *pa
     return MDF_OK;
 }
 
+// The block loop of mdf_get_k on the split-BF16 engine.
+int run_k_b3(const KPlan &p, const double *cderi, long naux, long ld, const int *pair_map, int nao,
+             const double *fl, const double *fr, int nocc, long ldf, double *vk, long ldvk, int accumulate,
+             void *work, hipStream_t st) {
+    const bool two = fr != nullptr;
+    const kb3::Layout y = kb3::layout(nao, nocc, p.nL, p.ksplit, two);
+    double *part = (double *)work;
+    kb3::Side sd[2] = {kb3::side_ptrs((char *)work, y, 0), kb3::side_ptrs((char *)work, y, two ? 1 : 0)};
+    const long strideL = (long)nao * y.ldr, part_stride = (long)nao * nao;
+    const long nft = (long)nocc * y.ldft;
+    for (int side = 0; side < (two ? 2 : 1); side++) {
+        hipLaunchKernelGGL(k_split_ft, dim3((unsigned)cdiv(nft, 1024L)), dim3(256), 0, st, side ? fr : fl, ldf, nao,
+                           nocc, sd[side].fthi, sd[side].ftlo, y.ldft);
+        MDF_LAUNCHED("k_split_ft");
+    }
+    for (long L0 = 0; L0 < naux; L0 += p.nL) {
+        const int nL = (int)(naux - L0 < p.nL ? naux - L0 : p.nL);
+        const double *blk = cderi + L0 * ld;
+        dim3 g1((unsigned)cdiv(y.ldr, (long)kb3::BN), (unsigned)cdiv(nao, kb3::BM), (unsigned)nL);
+        for (int side = 0; side < (two ? 2 : 1); side++) {
+            const kb3::Side &q = sd[side];
+            if (pair_map)
+                hipLaunchKernelGGL(k_step1_b3<true>, g1, dim3(G_THREADS), 0, st, blk, ld, pair_map, nao, q.fthi,
+                                   q.ftlo, y.ldft, nocc, q.rhi, q.rlo, y.ldr, strideL);
+            else
+                hipLaunchKernelGGL(k_step1_b3<false>, g1, dim3(G_THREADS), 0, st, blk, ld, pair_map, nao, q.fthi,
+                                   q.ftlo, y.ldft, nocc, q.rhi, q.rlo, y.ldr, strideL);
+            MDF_LAUNCHED("k_step1_b3");
+        }
+        dim3 g2((unsigned)p.ntiles_grid, (unsigned)p.ksplit);
+        const kb3::Side &a = sd[0], &b = sd[two ? 1 : 0];
+        if (two)
+            hipLaunchKernelGGL(k_step2_b3<false>, g2, dim3(G_THREADS), 0, st, a.rhi, a.rlo, b.rhi, b.rlo, nao, y.ldr,
+                               strideL, nL, p.ntile, part, part_stride, (int)(L0 > 0));
+        else
+            hipLaunchKernelGGL(k_step2_b3<true>, g2, dim3(G_THREADS), 0, st, a.rhi, a.rlo, b.rhi, b.rlo, nao, y.ldr,
+                               strideL, nL, p.ntile, part, part_stride, (int)(L0 > 0));
+        MDF_LAUNCHED("k_step2_b3");
+    }
+    long nt = (long)nao * nao;
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)cdiv(nt, 256L)), dim3(256), 0, st,
+                       part, p.ksplit, part_stride, nao, two ? 0 : 1, vk, ldvk, accumulate);
+    MDF_LAUNCHED("k_finalize");
+    return MDF_OK;
+}
+
 }  // namespace
 
 static size_t k_workspace(const KPlan &p, bool two, size_t elem) {
     return align_up(p.part_words, 64) * sizeof(double) + (two ? 2 : 1) * align_up(p.r_words, 64) * elem;
 }
 
+// Aux rows per pass and workspace of one call for any precision.
+static long k_rows_rule(long naux, int nao, int nocc, bool two, int prec) {
+    if (prec == MDF_PREC_BF16X3) return block_rows(naux, nao, (int)kb3::pad32(nocc), two, 4);
+    return block_rows(naux, nao, nocc, two, prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double));
+}
+static size_t k_bytes(long naux, int nao, int nocc, bool two, int prec, long rows) {
+    KPlan p = plan_k(naux, nao, nocc, two, rows);
+    if (prec == MDF_PREC_BF16X3) return kb3::layout(nao, nocc, p.nL, p.ksplit, two).total;
+    return k_workspace(p, two, prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double));
+}
+
 extern "C" size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_factors, int prec,
                                       long max_block_rows) {
-    KPlan p = plan_k(naux, nao, nocc, two_factors != 0, max_block_rows);
-    return k_workspace(p, two_factors != 0, prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double));
+    long rows = k_rows_rule(naux, nao, nocc, two_factors != 0, prec);
+    if (max_block_rows > 0 && max_block_rows < rows) rows = max_block_rows;
+    return k_bytes(naux, nao, nocc, two_factors != 0, prec, rows);
 }
 
 extern "C" int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *pair_map,
@@ -162,7 +255,8 @@ extern "C" int mdf_get_k(const double *cderi, long naux, long npairs, long ld, c
     MDF_CHECK_ARG(naux >= 0 && nao >= 0 && nocc >= 0, "mdf_get_k: bad sizes");
     MDF_CHECK_ARG(cderi && fl && vk, "mdf_get_k: null pointer");
     MDF_CHECK_ARG(ld >= npairs && ldf >= nocc && ldvk >= nao, "mdf_get_k: leading dimension too small");
-    MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32, "mdf_get_k: unknown precision %d", prec);
+    MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32 || prec == MDF_PREC_BF16X3,
+                  "mdf_get_k: unknown precision %d", prec);
     MDF_CHECK_ARG(pair_map || npairs == (long)nao * (nao + 1) / 2,
                   "mdf_get_k: without a pair map the cderi must hold the full packed triangle "
                   "(npairs=%ld, nao=%d)", npairs, nao);
@@ -175,18 +269,25 @@ extern "C" int mdf_get_k(const double *cderi, long naux, long npairs, long ld, c
         return MDF_OK;
     }
     MDF_CHECK_ARG(work != nullptr, "mdf_get_k: workspace is null");
-    const size_t elem = prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double);
-    KPlan p0 = plan_k(naux, nao, nocc, two, 1);
-    size_t need1 = k_workspace(p0, two, elem);
+    size_t need1 = k_bytes(naux, nao, nocc, two, prec, 1);
     if (work_bytes < need1) {
         set_error("mdf_get_k: workspace %zu bytes < %zu required", work_bytes, need1);
         return MDF_EWORKSPACE;
     }
-    size_t per_row = (size_t)nao * nocc * elem * (two ? 2 : 1);
-    size_t part_bytes = align_up(p0.part_words, 64) * sizeof(double);
-    long rows = (long)((work_bytes - part_bytes) / per_row);
+    // passes of the shape rule's aux rows (the grouped launch cuts the same
+    // way); shorter when the workspace is smaller
+    long rows = k_rows_rule(naux, nao, nocc, two, prec);
+    if (k_bytes(naux, nao, nocc, two, prec, rows) > work_bytes) {
+        long lo = 1, hi = rows;          // largest rows that fit
+        while (lo < hi) {
+            const long mid = (lo + hi + 1) / 2;
+            if (k_bytes(naux, nao, nocc, two, prec, mid) <= work_bytes) lo = mid; else hi = mid - 1;
+        }
+        rows = lo;
+    }
     KPlan p = plan_k(naux, nao, nocc, two, rows);
-    while (p.nL > 1 && k_workspace(p, two, elem) > work_bytes) p = plan_k(naux, nao, nocc, two, p.nL - 1);
+    if (prec == MDF_PREC_BF16X3)
+        return run_k_b3(p, cderi, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
     if (prec == MDF_PREC_FP32)
         return run_k<float>(p, cderi, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
     return run_k<double>(p, cderi, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);

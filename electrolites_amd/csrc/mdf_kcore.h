@@ -5,10 +5,12 @@
 //     R[L, i, k] = sum_j A_L[i, j] F[j, k]                 (step 1)
 //     K[i, j]   += sum_{L,k} R_l[L, i, k] R_r[L, j, k]     (step 2)
 #pragma once
+#include <cstdlib>
 #include <type_traits>
 
 #include "mdf_common.h"
 #include "mdf_gemm.h"
+#include "mdf_b3.h"
 
 namespace mdf {
 namespace kcore {
@@ -260,6 +262,29 @@ inline int pick_ksplit(int ntiles_grid) {
     return best;
 }
 
+// Aux rows per pass: the transformed blocks of one pass (R_l and R_r) stay
+// within K_BLOCK_BYTES.  A function of the item's shape only, so the single
+// entry and the grouped launch cut an item into the same passes.
+// The environment variable MDF_K_BLOCK_BYTES overrides the bound (read on
+// every plan, so tests can force several passes on small items).
+constexpr size_t K_BLOCK_BYTES = 8ull << 30;
+
+inline size_t k_block_bytes() {
+    const char *e = getenv("MDF_K_BLOCK_BYTES");
+    if (e && *e) {
+        const long long v = atoll(e);
+        if (v > 0) return (size_t)v;
+    }
+    return K_BLOCK_BYTES;
+}
+
+inline long block_rows(long naux, int nao, int nocc, bool two, size_t elem) {
+    const size_t per_row = (size_t)nao * nocc * elem * (two ? 2 : 1);
+    long r = per_row ? (long)(k_block_bytes() / per_row) : naux;
+    if (r > naux) r = naux;
+    return r < 1 ? 1 : r;
+}
+
 inline KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows) {
     KPlan p;
     p.ntile = (nao + S2_B - 1) / S2_B;
@@ -272,6 +297,228 @@ inline KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows) {
     p.part_words = (size_t)p.ksplit * nao * nao;
     return p;
 }
+
+// ------------------------------------------------- split-BF16 (prec 2)
+// The same two steps on the three-term split-BF16 engine (mdf_b3.h).  The
+// factors are pre-split into transposed planes FT[k][j] (j contiguous,
+// zero-padded to ldft = a multiple of 32 >= nao); R is stored split, as two
+// planes of [L][i][k] bfloat16 with k zero-padded to ldr = a multiple of 32
+// >= nocc, so step 2 copies it into LDS without conversion.
+namespace kb3 {
+
+using b3::u16;
+constexpr int BM = 128, BN = 128;
+// step 2 adds its FP32 accumulators into the FP64 partial in global memory
+// every S2_FLUSH K-steps (S2_FLUSH * 32 reduction terms)
+constexpr int S2_FLUSH = 64;
+
+__host__ __device__ inline long pad32(long n) { return (n + 31) / 32 * 32; }
+
+// A operand: rows i0..i0+127 of the unpacked A_L; K-step kt covers
+// j = 32 kt .. 32 kt + 31.
+template <bool MAPPED>
+struct PackedA32 {
+    using reg_t = double;
+    static constexpr int N = 16;
+    const double *row;
+    const int *pmap;
+    int nao, i0;
+    int upper;
+    __device__ __forceinline__ void load(int kt, double (&r)[N]) {
+        const int t = threadIdx.x, j0 = kt * b3::BK;
+        if (!MAPPED && j0 >= i0 + 127) {
+            // every j > every i: (i, j) at tri(j) + i, contiguous in i
+            upper = 1;
+            const int j = j0 + (t >> 3), ib = i0 + (t & 7) * 16;
+            const double *src = row + tri(j) + ib;
+#pragma unroll
+            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? src[v] : 0.;
+            return;
+        }
+        upper = 0;
+        const int i = i0 + (t >> 1), jb = j0 + (t & 1) * 16;
+        if (!MAPPED && i0 >= j0 + b3::BK - 1) {
+            // every i >= every j: contiguous in j
+            const double *src = row + tri(i) + jb;
+#pragma unroll
+            for (int v = 0; v < N; v++) r[v] = (i < nao) ? src[v] : 0.;
+            return;
+        }
+#pragma unroll
+        for (int v = 0; v < N; v++) {
+            const int j = jb + v;
+            double x = 0.;
+            if (i < nao && j < nao) {
+                if (MAPPED) {
+                    const int p = pmap[(long)i * nao + j];
+                    x = p >= 0 ? row[p] : 0.;
+                } else {
+                    x = row[i >= j ? tri(i) + j : tri(j) + i];
+                }
+            }
+            r[v] = x;
+        }
+    }
+    __device__ __forceinline__ void store(const double (&r)[N], u16 *hi, u16 *lo) const {
+        const int t = threadIdx.x;
+        if (upper) {
+            const int jj = t >> 3, ib = (t & 7) * 16;
+#pragma unroll
+            for (int v = 0; v < N; v++) {
+                u16 h, l;
+                b3::split((float)r[v], h, l);
+                hi[(ib + v) * b3::PK + jj] = h;
+                lo[(ib + v) * b3::PK + jj] = l;
+            }
+        } else {
+            b3::put16(r, hi, lo, t >> 1, (t & 1) * 16);
+        }
+    }
+};
+
+// B operand of step 1: rows k0..k0+127 of the split FT planes.
+struct FTLoader {
+    using reg_t = uint4;
+    static constexpr int N = b3::PlaneRows128::N;
+    const u16 *hi, *lo;
+    long ld;
+    int k0, nocc;
+    __device__ __forceinline__ void load(int kt, uint4 (&r)[N]) const {
+        const int t = threadIdx.x, k = k0 + (t >> 1);
+        const long off = (long)k * ld + kt * b3::BK + (t & 1) * 16;
+        const bool ok = k < nocc;
+        b3::PlaneRows128::fetch(ok ? hi + off : hi, ok ? lo + off : lo, ok, r);
+    }
+    __device__ __forceinline__ void store(const uint4 (&r)[N], u16 *h, u16 *l) const {
+        b3::PlaneRows128::put(r, h, l);
+    }
+};
+
+// One 128 x 128 tile of step 1 for the cderi row `row`: rows i0.., columns
+// k0.. of the split R planes (row stride ldr; columns nocc..ldr-1 get zeros).
+template <bool MAPPED>
+__device__ __forceinline__ void step1_tile(const double *__restrict__ row, const int *__restrict__ pmap, int nao,
+                                           const u16 *fthi, const u16 *ftlo, long ldft, int nocc, u16 *Rhi,
+                                           u16 *Rlo, long ldr, int i0, int k0, u16 *lds) {
+    PackedA32<MAPPED> la{row, pmap, nao, i0, 0};
+    FTLoader lb{fthi, ftlo, ldft, k0, nocc};
+    b3::Acc32<b3::Shape<BM, BN>::MI, b3::Shape<BM, BN>::NI> acc;
+    acc.zero();
+    b3::mainloop<BM, BN>(la, lb, (nao + b3::BK - 1) / b3::BK, lds, acc);
+    gemm_epilogue_layout<BM, BN, float>(acc, [&](int r, int c, float v) {
+        const int i = i0 + r, k = k0 + c;
+        if (i < nao && k < ldr) {
+            u16 h, l;
+            b3::split(v, h, l);
+            Rhi[(long)i * ldr + k] = h;
+            Rlo[(long)i * ldr + k] = l;
+        }
+    });
+}
+
+// Rows r0..r0+127 of the R planes; K-step kt of this split is
+// (L, kc) = divmod(s0 + kt, ldr / 32), advanced in place by load().
+struct RLoader {
+    using reg_t = uint4;
+    static constexpr int N = b3::PlaneRows128::N;
+    const u16 *hi, *lo;
+    long strideL, ldr;
+    int r0, nao, kchunks;
+    long L;
+    int kc;
+    __device__ __forceinline__ void load(int, uint4 (&r)[N]) {
+        const int t = threadIdx.x, row = r0 + (t >> 1);
+        const bool ok = row < nao;
+        const long off = L * strideL + (long)row * ldr + kc * b3::BK + (t & 1) * 16;
+        b3::PlaneRows128::fetch(ok ? hi + off : hi, ok ? lo + off : lo, ok, r);
+        if (++kc == kchunks) {
+            kc = 0;
+            ++L;
+        }
+    }
+    __device__ __forceinline__ void store(const uint4 (&r)[N], u16 *h, u16 *l) const {
+        b3::PlaneRows128::put(r, h, l);
+    }
+};
+
+// One 128 x 128 output tile (i0, j0) of step 2 over the K-steps [s0, s1);
+// writes (beta = 0) or adds (beta = 1) into the FP64 partial o.
+__device__ __forceinline__ void step2_tile(const u16 *lhi, const u16 *llo, const u16 *rhi, const u16 *rlo, int nao,
+                                           long ldr, long strideL, int i0, int j0, long s0, long s1,
+                                           double *__restrict__ o, int beta, u16 *lds) {
+    const int kchunks = (int)(ldr / b3::BK);
+    const long Ls = s0 / kchunks;
+    const int kcs = (int)(s0 - Ls * kchunks);
+    RLoader la{lhi, llo, strideL, ldr, i0, nao, kchunks, Ls, kcs};
+    RLoader lb{rhi, rlo, strideL, ldr, j0, nao, kchunks, Ls, kcs};
+    b3::Acc32<b3::Shape<BM, BN>::MI, b3::Shape<BM, BN>::NI> acc;
+    bool first = true;
+    long s = s0;
+    do {
+        const int n = (int)min((long)S2_FLUSH, s1 - s);
+        acc.zero();
+        b3::mainloop<BM, BN>(la, lb, n, lds, acc);
+        const bool add = beta || !first;
+        gemm_epilogue_layout<BM, BN, float>(acc, [&](int r, int c, float v) {
+            const int i = i0 + r, j = j0 + c;
+            if (i < nao && j < nao) {
+                double *dst = o + (long)i * nao + j;
+                *dst = add ? *dst + (double)v : (double)v;
+            }
+        });
+        first = false;
+        s += n;
+    } while (s < s1);
+}
+
+// FT[k][j] = F[j][k] split into two planes, j zero-padded to ldft.
+__device__ __forceinline__ void split_ft_range(const double *__restrict__ F, long ldf, int nao, int nocc, u16 *hi,
+                                               u16 *lo, long ldft, long e0, long e1) {
+    for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const int k = (int)(e / ldft), j = (int)(e % ldft);
+        const double x = j < nao ? F[(long)j * ldf + k] : 0.;
+        u16 h, l;
+        b3::split((float)x, h, l);
+        hi[e] = h;
+        lo[e] = l;
+    }
+}
+
+// Workspace layout of one item (bytes from its base): step-2 partials, then
+// per factor side the FT planes and the R planes.
+struct Layout {
+    long ldr, ldft;
+    size_t ft_elems, r_elems;        // per plane
+    size_t part_bytes, side_bytes, total;
+};
+
+inline Layout layout(int nao, int nocc, long nL, int ksplit, bool two) {
+    Layout y;
+    y.ldr = pad32(nocc);
+    y.ldft = pad32(nao);
+    y.ft_elems = (size_t)nocc * y.ldft;
+    y.r_elems = (size_t)nL * nao * y.ldr;
+    y.part_bytes = align_up((size_t)ksplit * nao * nao * sizeof(double), 256);
+    y.side_bytes = align_up(2 * y.ft_elems * sizeof(u16), 256) + align_up(2 * y.r_elems * sizeof(u16), 256);
+    y.total = y.part_bytes + (two ? 2 : 1) * y.side_bytes;
+    return y;
+}
+
+// pointers of one side (0 left, 1 right) inside an item's workspace
+struct Side {
+    u16 *fthi, *ftlo, *rhi, *rlo;
+};
+inline Side side_ptrs(char *base, const Layout &y, int side) {
+    char *b = base + y.part_bytes + side * y.side_bytes;
+    Side s;
+    s.fthi = (u16 *)b;
+    s.ftlo = s.fthi + y.ft_elems;
+    s.rhi = (u16 *)(b + align_up(2 * y.ft_elems * sizeof(u16), 256));
+    s.rlo = s.rhi + y.r_elems;
+    return s;
+}
+
+}  // namespace kb3
 
 }  // namespace kcore
 }  // namespace mdf

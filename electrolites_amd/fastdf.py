@@ -214,21 +214,24 @@ def _vj(cderi, lay, dms3, hermi):
     return vj
 
 
-PRECISIONS = {'fp64': 0, 'fp32': 1}
+PRECISIONS = {'fp64': 0, 'fp32': 1, 'bf16x3': 2}
 
 
 def _vk(cderi, lay, fl, fr, precision='fp64'):
     """``vk[n] = sum_L (A_L fl[n])(A_L fr[n])^T`` on the HIP path.
-    ``precision`` 'fp64' or 'fp32' (FP32 MFMA for both steps)."""
+    ``precision`` 'fp64', 'fp32' (FP32 MFMA for both steps) or 'bf16x3'
+    (split-BF16 three-term MFMA for both steps, FP32 accumulation)."""
     t = _dev.torch()
     n, nao, nocc = fl.shape
     naux, npairs = cderi.shape
     prec = PRECISIONS[precision]
     vk = t.empty((n, nao, nao), dtype=t.float64, device=fl.device)
     two = fr is not None
-    elem = 4 if prec == 1 else 8
+    elem = 8 if prec == 0 else 4
     per_row = nao * nocc * elem * (2 if two else 1)
     rows = max(1, min(naux, K_WORKSPACE_BYTES // max(per_row, 1)))
+    # the library cuts naux into passes by shape (the grouped launch cuts the
+    # same way); K_WORKSPACE_BYTES can only shorten them
     nbytes = _lib.query('mdf_get_k_workspace', naux, nao, nocc, int(two), prec, rows)
     work = _dev.WORKSPACE.get(nbytes)
     for i in range(n):
@@ -236,7 +239,7 @@ def _vk(cderi, lay, fl, fr, precision='fp64'):
         _lib.call('mdf_get_k', _dev.ptr(cderi), naux, npairs, cderi.stride(0),
                   _dev.ptr(lay.map), nao, _dev.ptr(fl[i]),
                   _dev.ptr(fr[i]) if two else None, nocc, nocc, prec,
-                  _dev.ptr(vk[i]), nao, 0, _dev.ptr(work), work.numel(),
+                  _dev.ptr(vk[i]), nao, 0, _dev.ptr(work), nbytes,
                   _dev.stream_ptr())
     return vk
 

@@ -47,13 +47,16 @@ def _water_frame():
 
 
 def water_cluster_atoms(n, seed=0, side=None, density=1.0, min_oo=2.6,
-                        li=False, max_tries=1000000):
+                        li=False, max_tries=1000000, max_resample=10000):
     """Atoms ``[(symbol, xyz_angstrom)]`` of an ``n``-water cluster."""
+    if li and n < 2:
+        raise ValueError('a Li+ at the O centroid needs at least 2 waters '
+                         '(with one it would sit on the oxygen)')
     if side is None:
         side = cube_side(n, density)
     rng = np.random.default_rng(seed)
     frame = _water_frame()
-    while True:
+    for _ in range(max_resample):
         opos = []
         tries = 0
         while len(opos) < n:
@@ -84,6 +87,8 @@ def water_cluster_atoms(n, seed=0, side=None, density=1.0, min_oo=2.6,
                 np.min(np.linalg.norm(np.array(hpos) - c, axis=1)) < 1.2):
             continue
         return atoms + [('Li', tuple(c))]
+    raise RuntimeError(f'no placement of Li+ among {n} waters in {max_resample} '
+                       'resamples (O-Li >= 1.8 A, H-Li >= 1.2 A)')
 
 
 def water_cluster(n, seed=0, side=None, density=1.0, li=False,

@@ -16,10 +16,13 @@ set that is not available offline; the converged energy does not depend on
 the guess).
 
 Precision schedule (``k_precision``): 'fp64' runs every cycle in FP64;
-'mixed' builds K (step 1, the occupied transform, and step 2) on FP32 MFMA
-while the DIIS error is above ``switch_err`` (1e-5), then restarts DIIS and
-finishes in FP64.  J, the metric, the diagonalisation and the energy sums are
-FP64 throughout.  The per-cycle precision is recorded in ``history``.
+'mixed' builds K (step 1, the occupied transform, and step 2) in the
+``low_precision`` mode -- 'fp32' (FP32 MFMA) or 'bf16x3' (split-BF16
+three-term MFMA), FP32 accumulation either way -- while the DIIS error is
+above ``switch_err`` (1e-5), then restarts DIIS and finishes in FP64; 'fp32'
+and 'bf16x3' stay in that mode throughout.  J, the metric, the
+diagonalisation and the energy sums are FP64 throughout.  The per-cycle
+precision is recorded in ``history``.
 """
 import math
 
@@ -33,10 +36,13 @@ from .df import DF
 
 class RHF:
     def __init__(self, mol, conv_tol=1e-10, conv_tol_grad=None, max_cycle=100, diis_space=8,
-                 df=None, verbose=0, k_precision='fp64', switch_err=1e-5):
-        if k_precision not in ('fp64', 'fp32', 'mixed'):
+                 df=None, verbose=0, k_precision='fp64', switch_err=1e-5, low_precision='fp32'):
+        if k_precision not in ('fp64', 'fp32', 'bf16x3', 'mixed'):
             raise ValueError(f'k_precision {k_precision!r}')
+        if low_precision not in ('fp32', 'bf16x3'):
+            raise ValueError(f'low_precision {low_precision!r}')
         self.k_precision = k_precision
+        self.low_precision = low_precision
         self.switch_err = switch_err
         self.mol = mol
         self.conv_tol = conv_tol
@@ -113,7 +119,8 @@ class RHF:
         self.converged = False
         self.done = False
         self.cycles = 0
-        self.prec = 'fp64' if self.k_precision == 'fp64' else 'fp32'
+        self.prec = {'fp64': 'fp64', 'fp32': 'fp32', 'bf16x3': 'bf16x3',
+                     'mixed': self.low_precision}[self.k_precision]
         return self
 
     def absorb(self, vj, vk):

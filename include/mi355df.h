@@ -25,11 +25,12 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 2
+#define MDF_ABI_VERSION 3
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
 #define MDF_PREC_FP32 1
+#define MDF_PREC_BF16X3 2
 
 /* ---- library / device ------------------------------------------------- */
 int mdf_abi_version(void);
@@ -61,11 +62,17 @@ int mdf_get_j(const double *cderi, long naux, long npairs, long ld,
  *             when the cderi holds the full packed triangle
  *   fl, fr    (nao, nocc) density factors, row stride ldf
  *   vk        (nao, nao) out, row stride ldvk; accumulate != 0 adds.
- * prec: MDF_PREC_FP64, or MDF_PREC_FP32 (both steps on FP32 MFMA with FP32
- *   accumulation, R stored in FP32; inputs and vk stay FP64) for the
- *   mixed-precision SCF cycles -- the reference's only narrow mode is FP32
- *   *storage* widened before arithmetic (electrolites/_dffp32.py:166-196).
- * max_block_rows bounds the aux rows transformed per pass (workspace size).
+ * prec: MDF_PREC_FP64, MDF_PREC_FP32 (both steps on FP32 MFMA with FP32
+ *   accumulation, R stored in FP32) or MDF_PREC_BF16X3 (both steps on BF16
+ *   MFMA with each operand split into hi + lo bfloat16 and three product
+ *   terms, FP32 accumulation, R stored split) for the mixed-precision SCF
+ *   cycles; inputs, the cross-split sums and vk stay FP64.  The reference's
+ *   only narrow mode is FP32 *storage* widened before arithmetic
+ *   (electrolites/_dffp32.py:166-196).
+ * The aux rows are transformed in passes of at most 8 GiB of transformed
+ * blocks (a function of naux, nao, nocc, the factor count and prec only);
+ * max_block_rows > 0 lowers that bound for the workspace query, and a
+ * smaller workspace makes mdf_get_k use shorter passes.
  */
 size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_factors, int prec,
                            long max_block_rows);
@@ -85,10 +92,14 @@ int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *p
  *   (n_dm, nao, nao) out (K).
  * mdf_grouped_plan_create flattens the batch into tile tables kept in device
  * memory the plan owns (it allocates; the runs do not) and reports the
- * workspace a run needs; bn = 0 picks the step-1 tile width.
+ * workspace a run needs; bn = 0 picks the step-1 tile width.  K runs in
+ * waves of consecutive items whose transformed blocks and partials fit
+ * work_limit bytes together (0: 32 GiB; a wave holds at least one item), so
+ * the workspace is about max(work_limit, the largest item's need).
  * mdf_grouped_jk runs the batch as persistent launches only (capturable in a
  * HIP graph).  Each item is cut and computed exactly as the single-item
- * entries cut it, so its J/K are bit-identical alone and in a batch. */
+ * entries cut it (mdf_get_k given at least mdf_get_k_workspace(..., 0)
+ * bytes), so its J/K are bit-identical alone and in a batch. */
 typedef struct {
     const double *cderi;
     long ld;
@@ -108,10 +119,11 @@ typedef struct {
 
 typedef struct mdf_grouped_plan mdf_grouped_plan;
 int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int with_j, int with_k, int prec,
-                            int bn, mdf_grouped_plan **plan, size_t *work_bytes);
+                            int bn, size_t work_limit, mdf_grouped_plan **plan, size_t *work_bytes);
 int mdf_grouped_jk(const mdf_grouped_plan *plan, void *work, size_t work_bytes, void *stream);
 int mdf_grouped_plan_destroy(mdf_grouped_plan *plan);
-/* tiles[0..6]: J pass-1, J reduce-1, J pass-2, J reduce-2, K step-1, K step-2, K finalize */
+/* tiles[0..6]: J pass-1, J reduce-1, J pass-2, J reduce-2, K step-1, K step-2, K finalize;
+ * with ntiles >= 9 also tiles[7] = K waves, tiles[8] = K launches */
 int mdf_grouped_plan_info(const mdf_grouped_plan *plan, long *tiles, int ntiles);
 
 /* ---- pair-vector helpers ----------------------------------------------

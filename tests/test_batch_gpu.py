@@ -39,7 +39,7 @@ def make_batch(seed=0):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
 def test_grouped_equals_single(precision):
     import torch
     from electrolites_amd import batch, fastdf
@@ -100,3 +100,50 @@ def test_graph_replay():
             assert torch.equal(e.vj_pairs, rj), rep
             assert torch.equal(e.vk, rk), rep
     g.close()
+
+
+def _entries(dfs, dms):
+    import torch
+    from electrolites_amd import batch, fastdf
+    entries = []
+    for df, dm in zip(dfs, dms):
+        cd = fastdf._cderi_of(df)
+        nao = dm.shape[-1]
+        lay = fastdf._layout(df, cd, nao)
+        dms3 = torch.from_numpy(np.asarray(dm).reshape(-1, nao, nao)).cuda()
+        fl, fr = fastdf.factorize_dm(dm, 1)
+        d = fastdf._sparse_dm(dms3, lay, 1)
+        entries.append(batch.Entry(cd, nao, dms3.shape[0], fl.shape[-1], dm_pairs=d, fl=fl, fr=fr,
+                                   pair_map=lay.map))
+    return entries
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
+def test_passes_and_waves(precision, monkeypatch):
+    """Items cut into several aux passes (a small pass bound) and run in one
+    wave or one wave per item: K equals the single entry's bit for bit."""
+    import torch
+    from electrolites_amd import batch, fastdf
+    monkeypatch.setenv('MDF_K_BLOCK_BYTES', str(300_000))
+    dfs, dms = make_batch(4)
+    ref = []
+    for df, dm in zip(dfs, dms):
+        df.k_precision = precision
+        ref.append(fastdf.get_jk(df, dm, hermi=1))
+    nk = sum(np.asarray(dm).reshape(-1, dm.shape[-1], dm.shape[-1]).shape[0] for dm in dms)
+    for limit, waves in ((0, 1), (1, nk)):
+        entries = _entries(dfs, dms)
+        g = batch.GroupedJK(entries, precision=precision, tune=False, work_limit=limit)
+        info = g.tiles()
+        assert info['k_waves'] == waves
+        assert info['k_launches'] > 3 * waves, 'expected several aux passes'
+        g.run()
+        torch.cuda.synchronize()
+        for e, (rj, rk) in zip(entries, ref):
+            assert np.array_equal(e.vk.cpu().numpy().reshape(rk.shape), rk), (limit, precision)
+        g.close()
+    if precision == 'fp64':
+        for df, dm, (rj, rk) in zip(dfs, dms, ref):
+            _, ok = O.get_jk(df._cderi[0].cpu().numpy(), df._cderi_idx[0], np.asarray(dm))
+            assert np.abs(rk - ok).max() <= 1e-11 * np.abs(ok).max()

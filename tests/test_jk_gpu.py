@@ -185,19 +185,31 @@ def test_device_in_device_out(water2):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('nao,naux,nocc', [(64, 130, 17), (200, 611, 90)])
-def test_fp32_k_error(nao, naux, nocc):
-    """FP32 MFMA exchange: within FP32 rounding of the FP64 result
-    (relative 1e-5 bar; measured error reported by the mixed SCF)."""
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+@pytest.mark.parametrize('nao,naux,nocc,mode,screened', [(64, 130, 17, 0, False), (200, 611, 90, 0, False),
+                                                         (200, 300, 45, 1, False), (170, 260, 33, 0, True),
+                                                         (150, 200, 40, 1, True)])
+def test_low_precision_k_error(precision, nao, naux, nocc, mode, screened):
+    """FP32 and split-BF16 exchange against the FP64 oracle: one factor
+    (mo-tagged) and two factors (untagged), full and screened pair sets,
+    shapes that reach every A-operand case (below, above and across the
+    diagonal).  Bar: 1e-5 relative to max |K| (FP32-level rounding; the split
+    keeps 16 significand bits per operand and three of the four products)."""
     from electrolites_amd import fastdf
-    rng = np.random.default_rng(nao + 7)
+    rng = np.random.default_rng(nao + 7 + 100 * mode)
     pa, _ = O.pair_addresses(nao)
+    if screened:
+        rows, cols = np.divmod(pa, nao)
+        pa = pa[(rng.random(len(pa)) > .25) | (rows == cols)]
     cderi = rng.standard_normal((naux, len(pa))) * .1
     c = rand_psd(nao, nocc, rng)
-    dm = fastdf.tag_array(2 * c @ c.T, mo_coeff=c, mo_occ=np.full(nocc, 2.))
+    dm = 2 * c @ c.T
+    if mode == 0:
+        dm = fastdf.tag_array(dm, mo_coeff=c, mo_occ=np.full(nocc, 2.))
     df = FakeDF(cderi, pa, nao)
-    df.k_precision = 'fp32'
-    _, vk32 = fastdf.get_jk(df, dm, hermi=1)
-    rk = O.get_k_factor(cderi, pa, c * np.sqrt(2.))
-    err = rel(vk32, rk)
+    df.k_precision = precision
+    _, vk = fastdf.get_jk(df, dm, hermi=1)
+    rk = O.get_k(cderi, pa, np.asarray(dm))
+    err = rel(vk, rk)
+    print(f'{precision} mode {mode} screened {screened}: K rel err {err:.2e}')
     assert 1e-12 < err < 1e-5, err

@@ -42,14 +42,15 @@ def test_rhf_energy_matches_oracle(mol):
 
 
 @pytest.mark.gpu
-def test_mixed_precision_rhf(mol):
-    """FP32 K until the DIIS error drops below 1e-5, FP64 finishing cycles:
-    the energy matches the all-FP64 run to 1e-8 Eh."""
+@pytest.mark.parametrize('low', ['fp32', 'bf16x3'])
+def test_mixed_precision_rhf(mol, low):
+    """FP32 or split-BF16 K until the DIIS error drops below 1e-5, FP64
+    finishing cycles: the energy matches the all-FP64 run to 1e-8 Eh."""
     from electrolites_amd.scf import RHF
     e64 = RHF(mol, conv_tol=1e-10).kernel()
-    mf = RHF(mol, conv_tol=1e-10, k_precision='mixed')
+    mf = RHF(mol, conv_tol=1e-10, k_precision='mixed', low_precision=low)
     e = mf.kernel()
     assert mf.converged
     precs = [h['k_precision'] for h in mf.history]
-    assert precs[0] == 'fp32' and precs[-1] == 'fp64'
+    assert precs[0] == low and precs[-1] == 'fp64'
     assert abs(e - e64) < 1e-8, (e, e64)

@@ -9,7 +9,7 @@ import pytest
 def _mols():
     from electrolites_amd.gto import water_cluster
     return [water_cluster(2, seed=11, side=5.0), water_cluster(3, seed=12, side=5.5, li=True),
-            water_cluster(1, seed=13, side=4.0, li=True), water_cluster(4, seed=14, side=6.0),
+            water_cluster(1, seed=13, side=4.0), water_cluster(4, seed=14, side=6.0),
             water_cluster(2, seed=15, side=5.0, li=True)]
 
 
@@ -27,6 +27,9 @@ def test_footprint_and_streams():
     s = serve.Server(hbm_budget=1e6)
     with pytest.raises(ValueError, match='does not fit'):
         s.submit(serve.Request(mols[0]))
+    from electrolites_amd.gto import water_cluster
+    with pytest.raises(ValueError, match='at least 2 waters'):
+        water_cluster(1, li=True)
 
 
 @pytest.mark.gpu
@@ -41,10 +44,12 @@ def test_alone_equals_mix(precision):
     pending = list(reqs)
     peak = 0
     while pending or srv.queue or srv.active:
-        if pending and srv.ticks % 2 == 0:
+        # a new request every other tick, or at once when the server drained
+        if pending and (srv.ticks % 2 == 0 or not (srv.queue or srv.active)):
             srv.submit(pending.pop(0))
         srv.step()
         peak = max(peak, len(srv.active))
+        assert srv.ticks < 2000
     srv.close()
     assert peak >= 3, 'requests did not overlap'
     for r in reqs:
