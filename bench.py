@@ -49,6 +49,9 @@ def parse():
     ap.add_argument('--rates', default='0.5,2,8', help='serve: Poisson rates (req/s)')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--graphs', action='store_true', help='serve: replay grouped runs as HIP graphs')
+    ap.add_argument('--copy-probe', type=int, default=0,
+                    help='getjk: first run a plain device copy of this many MiB (the reference '
+                         'kernel that calibrates HBM byte counters)')
     return ap.parse_args()
 
 
@@ -306,6 +309,13 @@ def main():
 
     dfobj = DF()
     torch.cuda.synchronize()
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe << 17, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f'copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written')
+        del src, dst
     log('cderi resident; warmup')
 
     def one(ev):

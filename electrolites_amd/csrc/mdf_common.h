@@ -22,6 +22,15 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 template <typename T>
 inline T cdiv(T a, T b) { return (a + b - 1) / b; }
 
+// XCD-aware tile order.  The 8 XCDs of an MI355X each have their own L2 and
+// receive workgroups round-robin by workgroup id.  xcd_tile() maps the id w
+// of a grid of g workgroups (g a multiple of 8) to a tile index so that each
+// XCD takes a contiguous run of tiles: neighbouring tiles, which read the
+// same operand rows, then meet in one L2 instead of eight.
+constexpr int N_XCD = 8;
+__host__ __device__ inline long xcd_tile(long w, long g) { return (w % N_XCD) * (g / N_XCD) + w / N_XCD; }
+__host__ __device__ inline long round_xcd(long g) { return (g + N_XCD - 1) / N_XCD * N_XCD; }
+
 }  // namespace mdf
 
 #define MDF_CHECK_ARG(cond, ...)            \

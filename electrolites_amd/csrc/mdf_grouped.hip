@@ -152,10 +152,16 @@ __global__ __launch_bounds__(jcore::J_THREADS) void gj_reduce2(const GJItem *ite
     }
 }
 
+// Step-1 tiles are visited in XCD runs (xcd_tile order): each XCD takes a
+// contiguous eighth of the table, so the k-tiles of one A tile share an L2.
+#define XCD_RUN_LOOP(t, n)                                                                   \
+    const long xr_per = gridDim.x / N_XCD, xr_x = blockIdx.x % N_XCD, xr_j = blockIdx.x / N_XCD; \
+    for (long t = (n) * xr_x / N_XCD + xr_j; t < (n) * (xr_x + 1) / N_XCD; t += xr_per)
+
 template <int BN, typename T>
 __global__ __launch_bounds__(G_THREADS) void gk_step1(const GKItem *items, const Tile *tiles, long n, char *work) {
     __shared__ T lds[TileShape<128, BN>::LDS_ELEMS];
-    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+    XCD_RUN_LOOP(t, n) {
         const Tile e = tiles[t];            // (item, L, i-tile, k-tile | right-factor flag << 16)
         const GKItem &it = items[e.item];
         const bool right = (e.c >> 16) != 0;
@@ -224,7 +230,7 @@ __global__ __launch_bounds__(G_THREADS, 2) void gk_step1_b3(const GKItem *items,
                                                              char *work) {
     using kcore::kb3::u16;
     __shared__ u16 lds[b3::Shape<kcore::kb3::BM, kcore::kb3::BN>::LDS_ELEMS];
-    for (long t = blockIdx.x; t < n; t += gridDim.x) {
+    XCD_RUN_LOOP(t, n) {
         const Tile e = tiles[t];            // (item, L, i-tile, k-tile | side << 16)
         const GKItem &it = items[e.item];
         const int side = e.c >> 16, ktile = e.c & 0xffff;
@@ -287,7 +293,7 @@ int persistent_grid(long ntiles, int per_cu) {
     if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
     long g = (long)cu * per_cu;
     if (g > ntiles) g = ntiles;
-    return (int)(g > 0 ? g : 1);
+    return (int)round_xcd(g > 0 ? g : 1);
 }
 
 }  // namespace

@@ -30,14 +30,21 @@ using namespace mdf::kcore;
 
 namespace {
 
+// 1-D grid of round_xcd(nkt * nit * nL) workgroups in XCD order; tile
+// (k-tile fastest, then i-tile, then aux row) so the k-tiles of one A tile
+// share an L2.
 template <int BN, bool MAPPED, typename T>
 __global__ __launch_bounds__(G_THREADS) void k_step1(
     const double *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao,
     const double *__restrict__ F, long ldf, int nocc, T *__restrict__ R, long ldr,
-    long strideL) {
+    long strideL, int nkt, int nit, long ntiles) {
     __shared__ T lds[TileShape<128, BN>::LDS_ELEMS];
-    step1_tile<BN, MAPPED, T>(cderi + (long)blockIdx.z * ld, pmap, nao, F, ldf, nocc,
-                              R + (long)blockIdx.z * strideL, ldr, blockIdx.y * 128, blockIdx.x * BN, lds);
+    const long x = xcd_tile(blockIdx.x, gridDim.x);
+    if (x >= ntiles) return;
+    const int kt = (int)(x % nkt), it = (int)((x / nkt) % nit);
+    const long L = x / ((long)nkt * nit);
+    step1_tile<BN, MAPPED, T>(cderi + L * ld, pmap, nao, F, ldf, nocc, R + L * strideL, ldr, it * 128, kt * BN,
+                              lds);
 }
 
 template <bool SYM, bool VEC, typename T>
@@ -78,11 +85,15 @@ __global__ __launch_bounds__(256) void k_split_ft(const double *__restrict__ F, 
 template <bool MAPPED>
 __global__ __launch_bounds__(G_THREADS, 2) void k_step1_b3(
     const double *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao, const b3::u16 *fthi,
-    const b3::u16 *ftlo, long ldft, int nocc, b3::u16 *Rhi, b3::u16 *Rlo, long ldr, long strideL) {
+    const b3::u16 *ftlo, long ldft, int nocc, b3::u16 *Rhi, b3::u16 *Rlo, long ldr, long strideL, int nkt,
+    int nit, long ntiles) {
     __shared__ b3::u16 lds[b3::Shape<kb3::BM, kb3::BN>::LDS_ELEMS];
-    const long z = blockIdx.z;
+    const long x = xcd_tile(blockIdx.x, gridDim.x);
+    if (x >= ntiles) return;
+    const int kt = (int)(x % nkt), it = (int)((x / nkt) % nit);
+    const long z = x / ((long)nkt * nit);
     kb3::step1_tile<MAPPED>(cderi + z * ld, pmap, nao, fthi, ftlo, ldft, nocc, Rhi + z * strideL,
-                            Rlo + z * strideL, ldr, blockIdx.y * kb3::BM, blockIdx.x * kb3::BN, lds);
+                            Rlo + z * strideL, ldr, it * kb3::BM, kt * kb3::BN, lds);
 }
 
 template <bool SYM>
@@ -104,24 +115,26 @@ __global__ __launch_bounds__(G_THREADS, 2) void k_step2_b3(
 }
 
 template <int BN, typename T>
-void launch_step1(dim3 g, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
+void launch_step1(int nL, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
                   const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
+    const int nkt = (int)cdiv(nocc, BN), nit = (int)cdiv(nao, 128);
+    const long ntiles = (long)nkt * nit * nL;
+    const dim3 g((unsigned)round_xcd(ntiles));
     if (pmap)
         hipLaunchKernelGGL((k_step1<BN, true, T>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
-                           nocc, r, ldr, strideL);
+                           nocc, r, ldr, strideL, nkt, nit, ntiles);
     else
         hipLaunchKernelGGL((k_step1<BN, false, T>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
-                           nocc, r, ldr, strideL);
+                           nocc, r, ldr, strideL, nkt, nit, ntiles);
 }
 
 template <typename T>
 int step1(const KPlan &p, int nL, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
           const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
-    dim3 g((unsigned)cdiv(nocc, p.bn), (unsigned)cdiv(nao, 128), (unsigned)nL);
     switch (p.bn) {
-    case 160: launch_step1<160, T>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    case 128: launch_step1<128, T>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    default: launch_step1<64, T>(g, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 160: launch_step1<160, T>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 128: launch_step1<128, T>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    default: launch_step1<64, T>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     }
     MDF_LAUNCHED("k_step1");
     return MDF_OK;
@@ -196,15 +209,17 @@ int run_k_b3(const KPlan &p, const double *cderi, long naux, long ld, const int 
     for (long L0 = 0; L0 < naux; L0 += p.nL) {
         const int nL = (int)(naux - L0 < p.nL ? naux - L0 : p.nL);
         const double *blk = cderi + L0 * ld;
-        dim3 g1((unsigned)cdiv(y.ldr, (long)kb3::BN), (unsigned)cdiv(nao, kb3::BM), (unsigned)nL);
+        const int nkt = (int)cdiv(y.ldr, (long)kb3::BN), nit = (int)cdiv(nao, kb3::BM);
+        const long ntiles = (long)nkt * nit * nL;
+        const dim3 g1((unsigned)round_xcd(ntiles));
         for (int side = 0; side < (two ? 2 : 1); side++) {
             const kb3::Side &q = sd[side];
             if (pair_map)
                 hipLaunchKernelGGL(k_step1_b3<true>, g1, dim3(G_THREADS), 0, st, blk, ld, pair_map, nao, q.fthi,
-                                   q.ftlo, y.ldft, nocc, q.rhi, q.rlo, y.ldr, strideL);
+                                   q.ftlo, y.ldft, nocc, q.rhi, q.rlo, y.ldr, strideL, nkt, nit, ntiles);
             else
                 hipLaunchKernelGGL(k_step1_b3<false>, g1, dim3(G_THREADS), 0, st, blk, ld, pair_map, nao, q.fthi,
-                                   q.ftlo, y.ldft, nocc, q.rhi, q.rlo, y.ldr, strideL);
+                                   q.ftlo, y.ldft, nocc, q.rhi, q.rlo, y.ldr, strideL, nkt, nit, ntiles);
             MDF_LAUNCHED("k_step1_b3");
         }
         dim3 g2((unsigned)p.ntiles_grid, (unsigned)p.ksplit);

@@ -327,12 +327,14 @@ struct PackedA32 {
     __device__ __forceinline__ void load(int kt, double (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * b3::BK;
         if (!MAPPED && j0 >= i0 + 127) {
-            // every j > every i: (i, j) at tri(j) + i, contiguous in i
+            // every j > every i: (i, j) at tri(j) + i, contiguous in i.  Thread
+            // t takes row i = i0 + t % 128 and j = j0 + 16 (t / 128) + v: each
+            // load instruction reads 64 consecutive i of one j (coalesced), and
+            // the row lands in LDS as 16 contiguous k (two 16-byte stores).
             upper = 1;
-            const int j = j0 + (t >> 3), ib = i0 + (t & 7) * 16;
-            const double *src = row + tri(j) + ib;
+            const int i = i0 + (t & 127), jb = j0 + (t >> 7) * 16;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? src[v] : 0.;
+            for (int v = 0; v < N; v++) r[v] = (i < nao && jb + v < nao) ? row[tri(jb + v) + i] : 0.;
             return;
         }
         upper = 0;
@@ -361,18 +363,10 @@ struct PackedA32 {
     }
     __device__ __forceinline__ void store(const double (&r)[N], u16 *hi, u16 *lo) const {
         const int t = threadIdx.x;
-        if (upper) {
-            const int jj = t >> 3, ib = (t & 7) * 16;
-#pragma unroll
-            for (int v = 0; v < N; v++) {
-                u16 h, l;
-                b3::split((float)r[v], h, l);
-                hi[(ib + v) * b3::PK + jj] = h;
-                lo[(ib + v) * b3::PK + jj] = l;
-            }
-        } else {
+        if (upper)
+            b3::put16(r, hi, lo, t & 127, (t >> 7) * 16);
+        else
             b3::put16(r, hi, lo, t >> 1, (t & 1) * 16);
-        }
     }
 };
 

@@ -36,13 +36,15 @@ from .df import DF
 
 class RHF:
     def __init__(self, mol, conv_tol=1e-10, conv_tol_grad=None, max_cycle=100, diis_space=8,
-                 df=None, verbose=0, k_precision='fp64', switch_err=1e-5, low_precision='fp32'):
+                 df=None, verbose=0, k_precision='fp64', switch_err=1e-5, low_precision='fp32',
+                 host_diag_max=0):
         if k_precision not in ('fp64', 'fp32', 'bf16x3', 'mixed'):
             raise ValueError(f'k_precision {k_precision!r}')
         if low_precision not in ('fp32', 'bf16x3'):
             raise ValueError(f'low_precision {low_precision!r}')
         self.k_precision = k_precision
         self.low_precision = low_precision
+        self.host_diag_max = host_diag_max
         self.switch_err = switch_err
         self.mol = mol
         self.conv_tol = conv_tol
@@ -86,9 +88,53 @@ class RHF:
         occ = t.full((nocc,), 2., dtype=t.float64, device=C.device)
         return fastdf.tag_array(dm, mo_coeff=co, mo_occ=occ)
 
-    def _diag(self, F):
+    # ------------------------------------------------------------- DIIS
+    def _diis_reset(self):
+        self._diis_n = 0
+
+    def _diis(self, F, err):
+        """Pulay DIIS over the last ``diis_space`` (Fock, error) pairs, kept
+        in device ring buffers; the error overlaps are updated one row per
+        cycle as plain products and sums (a GEMM of shape 8 x nao^2 x 8 or a
+        stack of the history would cost milliseconds per cycle)."""
         t = _dev.torch()
-        e, c = t.linalg.eigh(self.X.T @ F @ self.X)
+        m = self.diis_space
+        if getattr(self, '_fh', None) is None or self._fh.shape[1:] != F.shape:
+            self._fh = t.empty((m,) + tuple(F.shape), dtype=F.dtype, device=F.device)
+            self._eh = t.empty_like(self._fh)
+            self._bm = t.zeros((m, m), dtype=F.dtype, device=F.device)
+        k = self._diis_n % m
+        self._diis_n += 1
+        n = min(self._diis_n, m)
+        self._fh[k].copy_(F)
+        self._eh[k].copy_(err)
+        row = (self._eh[:n] * err).sum(dim=(1, 2))
+        self._bm[k, :n] = row
+        self._bm[:n, k] = row
+        if n < 2:
+            return F
+        B = -t.ones((n + 1, n + 1), dtype=F.dtype, device=F.device)
+        B[n, n] = 0.
+        B[:n, :n] = self._bm[:n, :n]
+        rhs = t.zeros(n + 1, dtype=F.dtype, device=F.device)
+        rhs[n] = -1.
+        coef = t.linalg.solve_ex(B, rhs)[0][:n]           # no host check: no sync
+        return (coef.view(-1, 1, 1) * self._fh[:n]).sum(dim=0)
+
+    def _diag(self, F):
+        """FP64 eigenpairs of ``X^T F X``: on the device, or with LAPACK on
+        the host when the orthogonal dimension is at most ``host_diag_max``
+        (small symmetric eigenproblems are latency-bound on the GPU; a serving
+        loop runs many of them side by side on host cores).  Either way the
+        choice depends on the molecule only."""
+        t = _dev.torch()
+        A = self.X.T @ F @ self.X
+        if A.shape[0] <= self.host_diag_max:
+            e, c = np.linalg.eigh(A.cpu().numpy())
+            e = t.from_numpy(e).to(A.device)
+            c = t.from_numpy(c).to(A.device)
+        else:
+            e, c = t.linalg.eigh(A)
         return e, self.X @ c
 
     # ------------------------------------------------------------- loop
@@ -113,7 +159,7 @@ class RHF:
             _, C = self._diag(self.H)
             dm0 = self._density(C)
         self.dm = dm0
-        self._focks, self._errs = [], []
+        self._diis_reset()
         self._e_old = 0.
         self._final = False
         self.converged = False
@@ -130,42 +176,30 @@ class RHF:
         density."""
         t = _dev.torch()
         F = self.H + vj - .5 * vk
-        e_tot = float(.5 * t.sum(self.dm * (self.H + F))) + self.enuc
+        e_el = .5 * t.sum(self.dm * (self.H + F))          # stays on the device
         if self._final:
-            self.e_tot = e_tot
+            self.e_tot = float(e_el) + self.enuc
             self.done = True
             return
         self.cycles = cycle = self.cycles + 1
         prec = self.prec
         err = F @ self.dm @ self.S
         err = self.X.T @ (err - err.T) @ self.X
-        focks, errs = self._focks, self._errs
-        focks.append(F)
-        errs.append(err)
-        focks, errs = focks[-self.diis_space:], errs[-self.diis_space:]
-        self._focks, self._errs = focks, errs
-        n = len(focks)
-        if n > 1:
-            B = -t.ones((n + 1, n + 1), dtype=t.float64, device=F.device)
-            B[n, n] = 0.
-            E = t.stack([e.reshape(-1) for e in errs])
-            B[:n, :n] = E @ E.T
-            rhs = t.zeros(n + 1, dtype=t.float64, device=F.device)
-            rhs[n] = -1.
-            coef = t.linalg.solve(B, rhs)[:n]
-            F = t.einsum('i,ijk->jk', coef, t.stack(focks))
-        err_max = float(err.abs().max())
+        F = self._diis(F, err)
+        # diagonalise first; the scalars come back in one transfer afterwards
+        self.mo_energy, self.mo_coeff = self._diag(F)
+        e_el, err_max = t.stack([e_el, err.abs().max()]).tolist()
+        e_tot = e_el + self.enuc
         de = e_tot - self._e_old
         self._e_old = e_tot
         self.history.append({'cycle': cycle, 'e_tot': e_tot, 'de': de, 'err': err_max,
                              'k_precision': prec})
         if self.verbose:
             print(f'cycle {cycle}: E = {e_tot:.12f}  dE = {de:.3e}  |err| = {err_max:.3e}')
-        self.mo_energy, self.mo_coeff = self._diag(F)
         self.dm = self._density(self.mo_coeff)
         if prec != 'fp64' and self.k_precision == 'mixed' and err_max < self.switch_err:
             self.prec = 'fp64'              # finishing cycles in FP64, fresh DIIS
-            self._focks, self._errs = [], []
+            self._diis_reset()
         elif abs(de) < self.conv_tol and err_max < self.conv_tol_grad:
             self.converged = True           # (an all-FP32 run converges in FP32)
             self._final = True

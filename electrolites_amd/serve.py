@@ -17,6 +17,21 @@ its J/K item -- and advances all of them one SCF cycle per tick:
    diagonalisation, next density, precision switch); finished requests leave
    and free their bytes for the queue.
 
+Step 3 is latency-bound (small matrices, one FP64 diagonalisation per
+request), so the requests absorb concurrently: each has its own HIP stream
+(default priority) and a worker thread of a pool issues its operations
+there; the torch operators and the library calls release the interpreter
+lock while they run, so the diagonalisations of different requests overlap.
+Orthogonal dimensions up to ``host_diag_max`` are diagonalised by LAPACK on
+host cores (one thread each), larger ones on the GPU.  Each stream waits on
+an event recorded after the grouped J/K launches, and the next tick's
+launches wait on events recorded after every request's absorb.
+
+Admission is asynchronous: an admitted request's cderi and one-electron
+matrices are built by a builder thread on a stream of its own while the
+ticks go on, and the request joins the batch at the first tick after its
+build finished.
+
 Each request runs exactly the cycle sequence of a stand-alone
 :class:`scf.RHF` with the same precision schedule, and its grouped J/K item
 is cut exactly as the single-item entries cut it, so its energy is the same
@@ -27,10 +42,12 @@ electrolites/fastdf.py:805-1026); this is the caller-side batching built on
 the grouped launch.
 """
 import collections
+import concurrent.futures
 import math
 import time
 
 import numpy as np
+from threadpoolctl import threadpool_limits
 
 from . import _dev
 from . import _lib
@@ -95,11 +112,12 @@ class Request:
 class _Slot:
     """An admitted request: its SCF state and its grouped J/K item."""
 
-    def __init__(self, req, conv_tol, max_cycle):
+    def __init__(self, req, conv_tol, max_cycle, host_diag_max=0, low_precision='fp32'):
         t = _dev.torch()
         self.req = req
         self.scf = RHF(req.mol, conv_tol=conv_tol, max_cycle=max_cycle,
-                       k_precision=req.k_precision)
+                       k_precision=req.k_precision, host_diag_max=host_diag_max,
+                       low_precision=low_precision)
         self.scf.build()
         self.scf.start()
         dfobj = self.scf.with_df
@@ -110,14 +128,16 @@ class _Slot:
         self.entry.layout = self.lay
         self.nao = nao
         self._vj = t.empty((1, nao, nao), dtype=t.float64, device=self.cderi.device)
+        self.stream = t.cuda.Stream()
 
     def load_density(self):
-        """Density factors and packed density of the current ``scf.dm``,
-        computed as :func:`fastdf.get_jk` computes them."""
+        """Density factor and packed density of the current ``scf.dm``, the
+        values :func:`fastdf.get_jk` computes for it (the factor of a density
+        tagged with occupied orbitals C and occupations n, all > 0 here, is
+        C sqrt(n), the same device operations), without a host round trip."""
+        t = _dev.torch()
         dm = self.scf.dm
-        fl, fr = fastdf.factorize_dm(dm, 1)
-        assert fr is None and fl.shape == self.entry.fl.shape
-        self.entry.fl.copy_(fl)
+        self.entry.fl[0].copy_(dm.mo_coeff * t.sqrt(dm.mo_occ))
         e = self.entry
         _lib.call('mdf_pack_dm', _dev.ptr(dm), 1, self.nao, self.nao * self.nao,
                   _dev.ptr(self.lay.addr), e.npairs, 1, _dev.ptr(e.dm_pairs), e.npairs,
@@ -136,8 +156,16 @@ class Server:
     ``serve`` drives a timed arrival stream to completion."""
 
     def __init__(self, hbm_budget=HBM_BUDGET, max_batch=64, conv_tol=1e-10, max_cycle=100,
-                 graphs=False):
+                 graphs=False, workers=16, host_diag_max=400, async_build=True,
+                 low_precision='bf16x3'):
         self.hbm_budget = float(hbm_budget)
+        self.workers = workers
+        self.host_diag_max = host_diag_max
+        self.low_precision = low_precision
+        self._pool = concurrent.futures.ThreadPoolExecutor(workers) if workers > 1 else None
+        self._builder = concurrent.futures.ThreadPoolExecutor(1) if async_build else None
+        self._building = []            # (future, request) of admitted requests being built
+        self.timing = collections.Counter()
         self.max_batch = max_batch
         self.conv_tol = conv_tol
         self.max_cycle = max_cycle
@@ -154,19 +182,51 @@ class Server:
     def now(self):
         return time.perf_counter() - self.t0
 
+    @property
+    def busy(self):
+        """Requests queued, being built or in the batch."""
+        return bool(self.queue or self.active or self._building)
+
     def submit(self, req):
         if req.bytes > self.hbm_budget:
             raise ValueError(f'request {req.rid}: {req.bytes / 1e9:.1f} GB does not fit the '
                              f'{self.hbm_budget / 1e9:.1f} GB budget')
         self.queue.append(req)
 
+    def _build(self, req):
+        t = _dev.torch()
+        stream = t.cuda.Stream()
+        with t.cuda.stream(stream):
+            slot = _Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max, self.low_precision)
+            ev = t.cuda.Event()
+            ev.record(stream)
+        return slot, ev
+
     def _admit(self):
-        while self.queue and len(self.active) < self.max_batch and \
+        t = _dev.torch()
+        while self.queue and len(self.active) + len(self._building) < self.max_batch and \
                 self.resident + self.queue[0].bytes <= self.hbm_budget:
             req = self.queue.popleft()
             req.t_admit = self.now()
-            self.active.append(_Slot(req, self.conv_tol, self.max_cycle))
             self.resident += req.bytes
+            if self._builder is None:
+                self.active.append(_Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max,
+                                         self.low_precision))
+            else:
+                self._building.append(self._builder.submit(self._build, req))
+        # requests whose build finished join the batch; when nothing is
+        # active, wait for the first build rather than spin
+        if self._building and not self.active:
+            concurrent.futures.wait(self._building[:1])
+        still = []
+        for f in self._building:
+            if f.done():
+                slot, ev = f.result()
+                t.cuda.current_stream().wait_event(ev)
+                self.active.append(slot)
+            else:
+                still.append(f)
+        self._building = still
 
     def _plan(self, slots, prec):
         key = (prec,) + tuple(id(s) for s in slots)
@@ -182,7 +242,11 @@ class Server:
 
     def step(self):
         """Admit, then advance every active request by one SCF cycle."""
+        t = _dev.torch()
+        t0 = time.perf_counter()
         self._admit()
+        t1 = time.perf_counter()
+        self.timing['admit_s'] += t1 - t0
         if not self.active:
             return 0
         self.ticks += 1
@@ -198,9 +262,32 @@ class Server:
         for key in list(self._plans):
             if key not in live:
                 self._plans.pop(key).close()
+        main = t.cuda.current_stream()
+        jk_done = t.cuda.Event()
+        jk_done.record(main)
+
+        def absorb(s):
+            with t.cuda.stream(s.stream):
+                s.stream.wait_event(jk_done)
+                finished = s.absorb()
+                ev = t.cuda.Event()
+                ev.record(s.stream)
+            return finished, ev
+
+        # host diagonalisations run one LAPACK thread each, side by side
+        with threadpool_limits(1, user_api='blas'):
+            if self._pool is not None and len(self.active) > 1:
+                results = list(self._pool.map(absorb, self.active))
+            else:
+                results = [absorb(s) for s in self.active]
+        for _, ev in results:
+            main.wait_event(ev)
+        self.timing['tick_s'] += time.perf_counter() - t1
         still = []
-        for s in self.active:
-            if s.absorb():
+        retired = False
+        for s, (finished, _) in zip(self.active, results):
+            if finished:
+                retired = True
                 r, scf = s.req, s.scf
                 r.t_done = self.now()
                 r.e_tot, r.cycles, r.converged = scf.e_tot, scf.cycles, scf.converged
@@ -209,6 +296,8 @@ class Server:
                 self.done.append(r)
             else:
                 still.append(s)
+        if retired:
+            t.cuda.synchronize()      # a retired request's buffers were used on two streams
         self.active = still
         self.trace.append((self.now(), len(self.active), len(self.queue), self.resident))
         return len(self.active)
@@ -218,11 +307,11 @@ class Server:
         server clock passes its ``arrival``; returns :func:`summary`."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         self.t0 = time.perf_counter()
-        while pending or self.queue or self.active:
+        while pending or self.busy:
             now = self.now()
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft())
-            if not self.queue and not self.active:
+            if not self.busy:
                 time.sleep(max(0., pending[0].arrival - self.now()))
                 continue
             self.step()
@@ -233,6 +322,12 @@ class Server:
         for g in self._plans.values():
             g.close()
         self._plans.clear()
+        if self._pool is not None:
+            self._pool.shutdown()
+            self._pool = None
+        if self._builder is not None:
+            self._builder.shutdown()
+            self._builder = None
 
 
 def serve_one_at_a_time(requests, conv_tol=1e-10, max_cycle=100, k_precision='fp64'):

@@ -43,9 +43,9 @@ def test_alone_equals_mix(precision):
     reqs = [serve.Request(m, rid=i, k_precision=precision) for i, m in enumerate(mols)]
     pending = list(reqs)
     peak = 0
-    while pending or srv.queue or srv.active:
+    while pending or srv.busy:
         # a new request every other tick, or at once when the server drained
-        if pending and (srv.ticks % 2 == 0 or not (srv.queue or srv.active)):
+        if pending and (srv.ticks % 2 == 0 or not srv.busy):
             srv.submit(pending.pop(0))
         srv.step()
         peak = max(peak, len(srv.active))
@@ -57,13 +57,17 @@ def test_alone_equals_mix(precision):
         alone = serve.Server(conv_tol=1e-10)
         a = serve.Request(r.mol, k_precision=precision)
         alone.serve([a])
-        mf = RHF(r.mol, conv_tol=1e-10, k_precision=precision)
+        # the stand-alone driver with the server's diagonalisation choice
+        mf = RHF(r.mol, conv_tol=1e-10, k_precision=precision, host_diag_max=srv.host_diag_max,
+                 low_precision=srv.low_precision)
         e = mf.kernel()
         assert abs(r.e_tot - a.e_tot) <= 1e-12, (r.rid, r.e_tot, a.e_tot)
         assert abs(r.e_tot - e) <= 1e-12, (r.rid, r.e_tot, e)
         assert r.cycles == a.cycles == mf.cycles
+        # the plain driver (device diagonalisation): the same SCF solution
+        assert abs(r.e_tot - RHF(r.mol, conv_tol=1e-10, k_precision=precision).kernel()) <= 1e-9
         if precision == 'mixed':
-            assert 'fp32' in [h['k_precision'] for h in r.history]
+            assert srv.low_precision in [h['k_precision'] for h in r.history]
             e64 = RHF(r.mol, conv_tol=1e-10).kernel()
             assert abs(r.e_tot - e64) <= 1e-8
 
