@@ -41,10 +41,14 @@ __device__ __forceinline__ void split(float x, u16 &hi, u16 &lo) {
     lo = bf16_rne(x - bf16_f(hi));
 }
 
-template <int BM, int BN>
+// A workgroup of WGM x WGN wavefronts computes a BM x BN tile; each
+// wavefront a (BM / WGM) x (BN / WGN) sub-tile of MI x NI MFMA blocks.
+template <int BM, int BN, int WGM = 2, int WGN = 2>
 struct Shape {
-    static constexpr int WM = BM / 2, WN = BN / 2;
+    static constexpr int THREADS = 64 * WGM * WGN;
+    static constexpr int WM = BM / WGM, WN = BN / WGN;
     static constexpr int MI = WM / 16, NI = WN / 16;
+    static_assert(MI * 16 == WM && NI * 16 == WN, "wave tile must be a multiple of 16");
     static constexpr int A_PLANE = BM * PK, B_PLANE = BN * PK;    // u16 elements
     static constexpr int STAGE = 2 * A_PLANE + 2 * B_PLANE;
     static constexpr int LDS_ELEMS = 2 * STAGE;
@@ -61,12 +65,12 @@ __device__ __forceinline__ float4_t mma(bf16x8_t a, bf16x8_t b, float4_t c) {
 //   using reg_t; static constexpr int N;
 //   void load(int kt, reg_t (&r)[N]);                 // global -> registers
 //   void store(const reg_t (&r)[N], u16 *hi, u16 *lo) const;   // -> the two planes
-template <int BM, int BN, class LA, class LB>
+template <int BM, int BN, int WGM, int WGN, class LA, class LB>
 __device__ __forceinline__ void mainloop(LA &la, LB &lb, int nk, u16 *lds,
-                                         Acc32<Shape<BM, BN>::MI, Shape<BM, BN>::NI> &acc) {
-    using S = Shape<BM, BN>;
+                                         Acc32<Shape<BM, BN, WGM, WGN>::MI, Shape<BM, BN, WGM, WGN>::NI> &acc) {
+    using S = Shape<BM, BN, WGM, WGN>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WGM, wn = wave / WGM;
     const int aoff = (wm * S::WM + (lane & 15)) * PK + 8 * (lane >> 4);
     const int boff = 2 * S::A_PLANE + (wn * S::WN + (lane & 15)) * PK + 8 * (lane >> 4);
     typename LA::reg_t ra[LA::N];
@@ -113,19 +117,43 @@ __device__ __forceinline__ void mainloop(LA &la, LB &lb, int nk, u16 *lds,
     }
 }
 
-// Writes 16 values (k = kb .. kb + 15 of LDS row `row`) into both planes.
-template <typename V>
-__device__ __forceinline__ void put16(const V (&x)[16], u16 *hi, u16 *lo, int row, int kb) {
-    uint4 h[2], l[2];
+// Visits every accumulator element with its tile-local (row, col) (the FP32
+// MFMA result layout).
+template <int BM, int BN, int WGM, int WGN, class AccT, class F>
+__device__ __forceinline__ void epilogue(const AccT &acc, F &&f) {
+    using S = Shape<BM, BN, WGM, WGN>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave % WGM, wn = wave / WGM;
+#pragma unroll
+    for (int mi = 0; mi < S::MI; mi++)
+#pragma unroll
+        for (int ni = 0; ni < S::NI; ni++) {
+            const int c = wn * S::WN + ni * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; v++) f(wm * S::WM + mi * 16 + 4 * (lane >> 4) + v, c, acc.v[mi][ni][v]);
+        }
+}
+
+// Writes NV (8 or 16) values (k = kb .. kb + NV - 1 of LDS row `row`) into
+// both planes as 16-byte stores.
+template <int NV, typename V>
+__device__ __forceinline__ void put(const V (&x)[NV], u16 *hi, u16 *lo, int row, int kb) {
+    static_assert(NV % 8 == 0, "put: whole 16-byte groups");
+    uint4 h[NV / 8], l[NV / 8];
     u16 *hs = reinterpret_cast<u16 *>(h), *ls = reinterpret_cast<u16 *>(l);
 #pragma unroll
-    for (int v = 0; v < 16; v++) split((float)x[v], hs[v], ls[v]);
+    for (int v = 0; v < NV; v++) split((float)x[v], hs[v], ls[v]);
     uint4 *dh = reinterpret_cast<uint4 *>(hi + row * PK + kb);
     uint4 *dl = reinterpret_cast<uint4 *>(lo + row * PK + kb);
-    dh[0] = h[0];
-    dh[1] = h[1];
-    dl[0] = l[0];
-    dl[1] = l[1];
+#pragma unroll
+    for (int g = 0; g < NV / 8; g++) {
+        dh[g] = h[g];
+        dl[g] = l[g];
+    }
+}
+template <typename V>
+__device__ __forceinline__ void put16(const V (&x)[16], u16 *hi, u16 *lo, int row, int kb) {
+    put<16>(x, hi, lo, row, kb);
 }
 
 // Two already-split planes P[row][k] (row stride ld elements, k contiguous,
@@ -157,6 +185,52 @@ struct PlaneRows128 {
         dh[1] = r[1];
         dl[0] = r[2];
         dl[1] = r[3];
+    }
+};
+
+// ROWS rows of two split planes (row r0 + r at hi/lo + r * ld + koff, 32
+// values per row) for a workgroup of THREADS threads: unit u = t + THREADS q
+// copies half u % 2 (16 values) of row u / 2.  Rows at or beyond nrows read
+// zero.
+template <int ROWS, int THREADS>
+struct PlaneRowsN {
+    static constexpr int UNITS = 2 * ROWS;
+    static constexpr int Q = (UNITS + THREADS - 1) / THREADS;
+    using reg_t = uint4;
+    static constexpr int N = 4 * Q;
+    __device__ __forceinline__ static void fetch(const u16 *hi, const u16 *lo, long ld, long koff, int r0,
+                                                 int nrows, uint4 (&r)[N]) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int u = threadIdx.x + THREADS * q, row = u >> 1;
+            const bool ok = u < UNITS && r0 + row < nrows;
+            const long off = (long)(r0 + row) * ld + koff + (u & 1) * 16;
+            if (ok) {
+                const uint4 *h = reinterpret_cast<const uint4 *>(hi + off);
+                const uint4 *l = reinterpret_cast<const uint4 *>(lo + off);
+                r[4 * q] = h[0];
+                r[4 * q + 1] = h[1];
+                r[4 * q + 2] = l[0];
+                r[4 * q + 3] = l[1];
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) r[4 * q + v] = make_uint4(0, 0, 0, 0);
+            }
+        }
+    }
+    __device__ __forceinline__ static void put(const uint4 (&r)[N], u16 *hi, u16 *lo) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int u = threadIdx.x + THREADS * q, row = u >> 1, kb = (u & 1) * 16;
+            if (u < UNITS) {
+                uint4 *dh = reinterpret_cast<uint4 *>(hi + row * PK + kb);
+                uint4 *dl = reinterpret_cast<uint4 *>(lo + row * PK + kb);
+                dh[0] = r[4 * q];
+                dh[1] = r[4 * q + 1];
+                dl[0] = r[4 * q + 2];
+                dl[1] = r[4 * q + 3];
+            }
+        }
     }
 };
 

@@ -60,18 +60,21 @@ struct GKItem {
     // split-BF16 (prec 2): workspace offsets of [side][FT hi, FT lo, R hi, R lo]
     void *b3[2][4];
     long ldr, ldft;
+    int bn3;                // step-1 tile width (kb3::pick_bn(ldr))
 };
 
 struct Tile {
     int item, a, b, c;
 };
 
-// A launch of the K sequence: kind 0 = factor split (prec 2), 1 = step 1
-// (4 = step 1 of items with a pair map, prec 2), 2 = step 2, 3 = finalize,
-// over tiles [start, start + count) of that kind's table.
+// A launch of the K sequence: kind 0 = factor split (prec 2), 1 = step 1,
+// 2 = step 2, 3 = finalize, over tiles [start, start + count) of that kind's
+// table.  Split-BF16 step-1 launches hold items of one tile width `bn` and
+// one `mapped` flag each.
 struct Phase {
     int kind;
     long start, count;
+    int bn, mapped;
 };
 
 }  // namespace
@@ -225,11 +228,12 @@ __global__ __launch_bounds__(256) void gk_split_ft(const GKItem *items, const Ti
     }
 }
 
-template <bool MAPPED>
-__global__ __launch_bounds__(G_THREADS, 2) void gk_step1_b3(const GKItem *items, const Tile *tiles, long n,
-                                                             char *work) {
+template <bool MAPPED, int BN>
+__global__ __launch_bounds__(kcore::kb3::Step1Cfg<BN>::THREADS) void gk_step1_b3(const GKItem *items,
+                                                                                 const Tile *tiles, long n,
+                                                                                 char *work) {
     using kcore::kb3::u16;
-    __shared__ u16 lds[b3::Shape<kcore::kb3::BM, kcore::kb3::BN>::LDS_ELEMS];
+    __shared__ u16 lds[kcore::kb3::Step1Cfg<BN>::S::LDS_ELEMS];
     XCD_RUN_LOOP(t, n) {
         const Tile e = tiles[t];            // (item, L, i-tile, k-tile | side << 16)
         const GKItem &it = items[e.item];
@@ -243,8 +247,8 @@ __global__ __launch_bounds__(G_THREADS, 2) void gk_step1_b3(const GKItem *items,
         const double *row = it.cderi + (long)e.a * it.ld;
         // a phase holds only mapped or only unmapped items (one instance
         // each keeps the register budget of the single-item kernel)
-        kcore::kb3::step1_tile<MAPPED>(row, it.pmap, it.nao, fthi, ftlo, it.ldft, it.nocc, rhi, rlo, it.ldr,
-                                       e.b * kcore::kb3::BM, ktile * kcore::kb3::BN, lds);
+        kcore::kb3::step1_tile<MAPPED, BN>(row, it.pmap, it.nao, fthi, ftlo, it.ldft, it.nocc, rhi, rlo, it.ldr,
+                                           e.b * kcore::kb3::BM, ktile * BN, lds);
         __syncthreads();
     }
 }
@@ -252,7 +256,7 @@ __global__ __launch_bounds__(G_THREADS, 2) void gk_step1_b3(const GKItem *items,
 __global__ __launch_bounds__(G_THREADS, 2) void gk_step2_b3(const GKItem *items, const Tile *tiles, long n,
                                                              char *work) {
     using kcore::kb3::u16;
-    __shared__ u16 lds[b3::Shape<kcore::kb3::BM, kcore::kb3::BN>::LDS_ELEMS];
+    __shared__ u16 lds[b3::Shape<kcore::S2_B, kcore::S2_B>::LDS_ELEMS];
     for (long t = blockIdx.x; t < n; t += gridDim.x) {
         const Tile e = tiles[t];            // (item, tile index, split, pass)
         const GKItem &it = items[e.item];
@@ -374,6 +378,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                     const kcore::kb3::Layout y = kcore::kb3::layout(x.nao, x.nocc, kp.nL, kp.ksplit, two);
                     g.ldr = y.ldr;
                     g.ldft = y.ldft;
+                    g.bn3 = kcore::kb3::pick_bn(y.ldr);
                     g.rl = (void *)0;
                     g.part = (double *)y.total;
                 } else {
@@ -427,27 +432,29 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                 for (int side = 0; side < (g.fr ? 2 : 1); side++)
                     for (long e = 0; e < cdiv((long)g.nocc * g.ldft, 1024L); e++) k0.push_back({(int)i, (int)e, side, 0});
             }
-            p->kphases.push_back({0, s0, (long)k0.size() - s0});
+            p->kphases.push_back({0, s0, (long)k0.size() - s0, 0, 0});
         }
+        const bool b3 = prec == MDF_PREC_BF16X3;
         for (long b = 0; b < maxpass; b++) {
-          // split-BF16: unmapped items' step-1 tiles, then mapped items' ones
-          for (int mp = 0; mp < (prec == MDF_PREC_BF16X3 ? 2 : 1); mp++) {
+          // split-BF16: one step-1 launch per (pair map or not, tile width)
+          for (int cls = 0; cls < (b3 ? 6 : 1); cls++) {
+            static const int kBN3[3] = {128, 256, 320};
+            const int mp = cls / 3, cbn = b3 ? kBN3[cls % 3] : p->bn;
             const long s1 = k1.size();
             for (size_t i = w0; i < w1; i++) {
                 const GKItem &g = p->hk[i];
                 const long L0 = b * g.nL;
                 if (L0 >= g.naux) continue;
-                if (prec == MDF_PREC_BF16X3 && (g.pmap != nullptr) != (mp == 1)) continue;
+                if (b3 && ((g.pmap != nullptr) != (mp == 1) || g.bn3 != cbn)) continue;
                 const long L1 = std::min(L0 + g.nL, g.naux);
-                const int nkt = prec == MDF_PREC_BF16X3 ? (int)cdiv(g.ldr, (long)kcore::kb3::BN)
-                                                        : (int)cdiv(g.nocc, p->bn);
+                const int nkt = b3 ? (int)cdiv(g.ldr, (long)cbn) : (int)cdiv(g.nocc, p->bn);
                 const int nit = (int)cdiv(g.nao, 128);
                 for (int side = 0; side < (g.fr ? 2 : 1); side++)
                     for (long L = L0; L < L1; L++)
                         for (int it = 0; it < nit; it++)
                             for (int kt = 0; kt < nkt; kt++) k1.push_back({(int)i, (int)L, it, kt | (side << 16)});
             }
-            p->kphases.push_back({mp ? 4 : 1, s1, (long)k1.size() - s1});
+            p->kphases.push_back({1, s1, (long)k1.size() - s1, cbn, mp});
           }
             const long s2 = k2.size();
             for (size_t i = w0; i < w1; i++) {
@@ -456,14 +463,14 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                 for (int s = 0; s < g.ksplit; s++)
                     for (int tl = 0; tl < kgrid[i]; tl++) k2.push_back({(int)i, tl, s, (int)b});
             }
-            p->kphases.push_back({2, s2, (long)k2.size() - s2});
+            p->kphases.push_back({2, s2, (long)k2.size() - s2, 0, 0});
         }
         const long sf = kf.size();
         for (size_t i = w0; i < w1; i++) {
             const GKItem &g = p->hk[i];
             for (long e = 0; e < cdiv((long)g.nao * g.nao, RED_CHUNK); e++) kf.push_back({(int)i, (int)e, 0, 0});
         }
-        p->kphases.push_back({3, sf, (long)kf.size() - sf});
+        p->kphases.push_back({3, sf, (long)kf.size() - sf, 0, 0});
         p->kwaves++;
         w0 = w1;
     }
@@ -527,6 +534,14 @@ static void launch_gk1(const mdf_grouped_plan *p, const Tile *tiles, long n, cha
         hipLaunchKernelGGL((gk_step1<BN, double>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
 }
 
+template <bool MAPPED, int BN>
+static void launch_gk1_b3(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
+    // one workgroup per CU (the 256- and 320-wide tiles hold 120-140 KB of LDS)
+    const int per_cu = BN == 128 ? 2 : 1;
+    hipLaunchKernelGGL((gk_step1_b3<MAPPED, BN>), dim3(persistent_grid(n, per_cu)),
+                       dim3(kcore::kb3::Step1Cfg<BN>::THREADS), 0, st, p->kitems, tiles, n, w);
+}
+
 extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work_bytes, void *stream) {
     MDF_CHECK_ARG(p && (work || p->work_bytes <= 256), "mdf_grouped_jk: null argument");
     if (work_bytes < p->work_bytes) {
@@ -557,13 +572,16 @@ extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work
                 hipLaunchKernelGGL(gk_split_ft, dim3(persistent_grid(ph.count, 4)), dim3(256), 0, st, p->kitems,
                                    p->k0 + ph.start, ph.count, w);
                 MDF_LAUNCHED("gk_split_ft");
-            } else if ((ph.kind == 1 && b3) || ph.kind == 4) {
-                if (ph.kind == 4)
-                    hipLaunchKernelGGL(gk_step1_b3<true>, dim3(persistent_grid(ph.count, 2)), dim3(G_THREADS), 0,
-                                       st, p->kitems, p->k1 + ph.start, ph.count, w);
-                else
-                    hipLaunchKernelGGL(gk_step1_b3<false>, dim3(persistent_grid(ph.count, 2)), dim3(G_THREADS), 0,
-                                       st, p->kitems, p->k1 + ph.start, ph.count, w);
+            } else if (ph.kind == 1 && b3) {
+                const Tile *t1 = p->k1 + ph.start;
+                switch (ph.bn * 2 + ph.mapped) {
+                case 640: launch_gk1_b3<false, 320>(p, t1, ph.count, w, st); break;
+                case 641: launch_gk1_b3<true, 320>(p, t1, ph.count, w, st); break;
+                case 512: launch_gk1_b3<false, 256>(p, t1, ph.count, w, st); break;
+                case 513: launch_gk1_b3<true, 256>(p, t1, ph.count, w, st); break;
+                case 257: launch_gk1_b3<true, 128>(p, t1, ph.count, w, st); break;
+                default: launch_gk1_b3<false, 128>(p, t1, ph.count, w, st); break;
+                }
                 MDF_LAUNCHED("gk_step1_b3");
             } else if (ph.kind == 2 && b3) {
                 hipLaunchKernelGGL(gk_step2_b3, dim3(persistent_grid(ph.count, 2)), dim3(G_THREADS), 0, st,

@@ -82,25 +82,40 @@ __global__ __launch_bounds__(256) void k_split_ft(const double *__restrict__ F, 
     kb3::split_ft_range(F, ldf, nao, nocc, hi, lo, ldft, e0, min(e0 + 1024, (long)nocc * ldft));
 }
 
-template <bool MAPPED>
-__global__ __launch_bounds__(G_THREADS, 2) void k_step1_b3(
+template <bool MAPPED, int BN>
+__global__ __launch_bounds__(kb3::Step1Cfg<BN>::THREADS) void k_step1_b3(
     const double *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao, const b3::u16 *fthi,
     const b3::u16 *ftlo, long ldft, int nocc, b3::u16 *Rhi, b3::u16 *Rlo, long ldr, long strideL, int nkt,
     int nit, long ntiles) {
-    __shared__ b3::u16 lds[b3::Shape<kb3::BM, kb3::BN>::LDS_ELEMS];
+    __shared__ b3::u16 lds[kb3::Step1Cfg<BN>::S::LDS_ELEMS];
     const long x = xcd_tile(blockIdx.x, gridDim.x);
     if (x >= ntiles) return;
     const int kt = (int)(x % nkt), it = (int)((x / nkt) % nit);
     const long z = x / ((long)nkt * nit);
-    kb3::step1_tile<MAPPED>(cderi + z * ld, pmap, nao, fthi, ftlo, ldft, nocc, Rhi + z * strideL,
-                            Rlo + z * strideL, ldr, it * kb3::BM, kt * kb3::BN, lds);
+    kb3::step1_tile<MAPPED, BN>(cderi + z * ld, pmap, nao, fthi, ftlo, ldft, nocc, Rhi + z * strideL,
+                                Rlo + z * strideL, ldr, it * kb3::BM, kt * BN, lds);
+}
+
+template <int BN>
+void launch_step1_b3(int nL, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
+                     const b3::u16 *fthi, const b3::u16 *ftlo, long ldft, int nocc, b3::u16 *rhi, b3::u16 *rlo,
+                     long ldr, long strideL) {
+    const int nkt = (int)cdiv(ldr, (long)BN), nit = (int)cdiv(nao, kb3::BM);
+    const long ntiles = (long)nkt * nit * nL;
+    const dim3 g((unsigned)round_xcd(ntiles)), b(kb3::Step1Cfg<BN>::THREADS);
+    if (pmap)
+        hipLaunchKernelGGL((k_step1_b3<true, BN>), g, b, 0, st, blk, ld, pmap, nao, fthi, ftlo, ldft, nocc, rhi,
+                           rlo, ldr, strideL, nkt, nit, ntiles);
+    else
+        hipLaunchKernelGGL((k_step1_b3<false, BN>), g, b, 0, st, blk, ld, pmap, nao, fthi, ftlo, ldft, nocc, rhi,
+                           rlo, ldr, strideL, nkt, nit, ntiles);
 }
 
 template <bool SYM>
 __global__ __launch_bounds__(G_THREADS, 2) void k_step2_b3(
     const b3::u16 *lhi, const b3::u16 *llo, const b3::u16 *rhi, const b3::u16 *rlo, int nao, long ldr,
     long strideL, int nL, int ntile, double *__restrict__ part, long part_stride, int beta) {
-    __shared__ b3::u16 lds[b3::Shape<kb3::BM, kb3::BN>::LDS_ELEMS];
+    __shared__ b3::u16 lds[b3::Shape<S2_B, S2_B>::LDS_ELEMS];
     int ti, tj;
     if (SYM) {
         tri_tile(blockIdx.x, ti, tj);
@@ -209,17 +224,23 @@ int run_k_b3(const KPlan &p, const double *cderi, long naux, long ld, const int 
     for (long L0 = 0; L0 < naux; L0 += p.nL) {
         const int nL = (int)(naux - L0 < p.nL ? naux - L0 : p.nL);
         const double *blk = cderi + L0 * ld;
-        const int nkt = (int)cdiv(y.ldr, (long)kb3::BN), nit = (int)cdiv(nao, kb3::BM);
-        const long ntiles = (long)nkt * nit * nL;
-        const dim3 g1((unsigned)round_xcd(ntiles));
+        const int bn = kb3::pick_bn(y.ldr);
         for (int side = 0; side < (two ? 2 : 1); side++) {
             const kb3::Side &q = sd[side];
-            if (pair_map)
-                hipLaunchKernelGGL(k_step1_b3<true>, g1, dim3(G_THREADS), 0, st, blk, ld, pair_map, nao, q.fthi,
-                                   q.ftlo, y.ldft, nocc, q.rhi, q.rlo, y.ldr, strideL, nkt, nit, ntiles);
-            else
-                hipLaunchKernelGGL(k_step1_b3<false>, g1, dim3(G_THREADS), 0, st, blk, ld, pair_map, nao, q.fthi,
-                                   q.ftlo, y.ldft, nocc, q.rhi, q.rlo, y.ldr, strideL, nkt, nit, ntiles);
+            switch (bn) {
+            case 320:
+                launch_step1_b3<320>(nL, st, blk, ld, pair_map, nao, q.fthi, q.ftlo, y.ldft, nocc, q.rhi, q.rlo,
+                                     y.ldr, strideL);
+                break;
+            case 256:
+                launch_step1_b3<256>(nL, st, blk, ld, pair_map, nao, q.fthi, q.ftlo, y.ldft, nocc, q.rhi, q.rlo,
+                                     y.ldr, strideL);
+                break;
+            default:
+                launch_step1_b3<128>(nL, st, blk, ld, pair_map, nao, q.fthi, q.ftlo, y.ldft, nocc, q.rhi, q.rlo,
+                                     y.ldr, strideL);
+                break;
+            }
             MDF_LAUNCHED("k_step1_b3");
         }
         dim3 g2((unsigned)p.ntiles_grid, (unsigned)p.ksplit);

@@ -307,7 +307,7 @@ inline KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows) {
 namespace kb3 {
 
 using b3::u16;
-constexpr int BM = 128, BN = 128;
+constexpr int BM = 128;               // step-1 rows per tile (step 2: S2_B x S2_B)
 // step 2 adds its FP32 accumulators into the FP64 partial in global memory
 // every S2_FLUSH K-steps (S2_FLUSH * 32 reduction terms)
 constexpr int S2_FLUSH = 64;
@@ -315,11 +315,12 @@ constexpr int S2_FLUSH = 64;
 __host__ __device__ inline long pad32(long n) { return (n + 31) / 32 * 32; }
 
 // A operand: rows i0..i0+127 of the unpacked A_L; K-step kt covers
-// j = 32 kt .. 32 kt + 31.
-template <bool MAPPED>
+// j = 32 kt .. 32 kt + 31.  THREADS threads, N = 4096 / THREADS values each.
+template <bool MAPPED, int THREADS>
 struct PackedA32 {
     using reg_t = double;
-    static constexpr int N = 16;
+    static constexpr int N = 128 * b3::BK / THREADS;
+    static constexpr int TPR = b3::BK / N;           // threads per row (lower layout)
     const double *row;
     const int *pmap;
     int nao, i0;
@@ -328,17 +329,17 @@ struct PackedA32 {
         const int t = threadIdx.x, j0 = kt * b3::BK;
         if (!MAPPED && j0 >= i0 + 127) {
             // every j > every i: (i, j) at tri(j) + i, contiguous in i.  Thread
-            // t takes row i = i0 + t % 128 and j = j0 + 16 (t / 128) + v: each
+            // t takes row i = i0 + t % 128 and j = j0 + N (t / 128) + v: each
             // load instruction reads 64 consecutive i of one j (coalesced), and
-            // the row lands in LDS as 16 contiguous k (two 16-byte stores).
+            // the row lands in LDS as N contiguous k (16-byte stores).
             upper = 1;
-            const int i = i0 + (t & 127), jb = j0 + (t >> 7) * 16;
+            const int i = i0 + (t & 127), jb = j0 + (t >> 7) * N;
 #pragma unroll
             for (int v = 0; v < N; v++) r[v] = (i < nao && jb + v < nao) ? row[tri(jb + v) + i] : 0.;
             return;
         }
         upper = 0;
-        const int i = i0 + (t >> 1), jb = j0 + (t & 1) * 16;
+        const int i = i0 + t / TPR, jb = j0 + (t % TPR) * N;
         if (!MAPPED && i0 >= j0 + b3::BK - 1) {
             // every i >= every j: contiguous in j
             const double *src = row + tri(i) + jb;
@@ -364,42 +365,63 @@ struct PackedA32 {
     __device__ __forceinline__ void store(const double (&r)[N], u16 *hi, u16 *lo) const {
         const int t = threadIdx.x;
         if (upper)
-            b3::put16(r, hi, lo, t & 127, (t >> 7) * 16);
+            b3::put<N>(r, hi, lo, t & 127, (t >> 7) * N);
         else
-            b3::put16(r, hi, lo, t >> 1, (t & 1) * 16);
+            b3::put<N>(r, hi, lo, t / TPR, (t % TPR) * N);
     }
 };
 
-// B operand of step 1: rows k0..k0+127 of the split FT planes.
+// B operand of step 1: rows k0..k0+BN-1 of the split FT planes.
+template <int BN, int THREADS>
 struct FTLoader {
+    using P = b3::PlaneRowsN<BN, THREADS>;
     using reg_t = uint4;
-    static constexpr int N = b3::PlaneRows128::N;
+    static constexpr int N = P::N;
     const u16 *hi, *lo;
     long ld;
     int k0, nocc;
     __device__ __forceinline__ void load(int kt, uint4 (&r)[N]) const {
-        const int t = threadIdx.x, k = k0 + (t >> 1);
-        const long off = (long)k * ld + kt * b3::BK + (t & 1) * 16;
-        const bool ok = k < nocc;
-        b3::PlaneRows128::fetch(ok ? hi + off : hi, ok ? lo + off : lo, ok, r);
+        P::fetch(hi, lo, ld, (long)kt * b3::BK, k0, nocc, r);
     }
-    __device__ __forceinline__ void store(const uint4 (&r)[N], u16 *h, u16 *l) const {
-        b3::PlaneRows128::put(r, h, l);
-    }
+    __device__ __forceinline__ void store(const uint4 (&r)[N], u16 *h, u16 *l) const { P::put(r, h, l); }
 };
 
-// One 128 x 128 tile of step 1 for the cderi row `row`: rows i0.., columns
+// Step-1 tile widths: 128 columns on 2 x 2 wavefronts, 256 or 320 on 2 x 4
+// (one workgroup then covers up to 320 occupied columns, so each cderi tile
+// is read once per factor instead of once per 128 columns).
+template <int BN>
+struct Step1Cfg {
+    static constexpr int WGN = BN == 128 ? 2 : 4;
+    static constexpr int THREADS = 128 * WGN;
+    using S = b3::Shape<BM, BN, 2, WGN>;
+};
+
+inline int pick_bn(long ldr) {
+    int best = 128;
+    long pad = -1;
+    for (int bn : {128, 256, 320}) {
+        const long p = (ldr + bn - 1) / bn * bn - ldr;
+        if (pad < 0 || p < pad || (p == pad && bn > best)) {
+            best = bn;
+            pad = p;
+        }
+    }
+    return best;
+}
+
+// One 128 x BN tile of step 1 for the cderi row `row`: rows i0.., columns
 // k0.. of the split R planes (row stride ldr; columns nocc..ldr-1 get zeros).
-template <bool MAPPED>
+template <bool MAPPED, int BN>
 __device__ __forceinline__ void step1_tile(const double *__restrict__ row, const int *__restrict__ pmap, int nao,
                                            const u16 *fthi, const u16 *ftlo, long ldft, int nocc, u16 *Rhi,
                                            u16 *Rlo, long ldr, int i0, int k0, u16 *lds) {
-    PackedA32<MAPPED> la{row, pmap, nao, i0, 0};
-    FTLoader lb{fthi, ftlo, ldft, k0, nocc};
-    b3::Acc32<b3::Shape<BM, BN>::MI, b3::Shape<BM, BN>::NI> acc;
+    using C = Step1Cfg<BN>;
+    PackedA32<MAPPED, C::THREADS> la{row, pmap, nao, i0, 0};
+    FTLoader<BN, C::THREADS> lb{fthi, ftlo, ldft, k0, nocc};
+    b3::Acc32<C::S::MI, C::S::NI> acc;
     acc.zero();
-    b3::mainloop<BM, BN>(la, lb, (nao + b3::BK - 1) / b3::BK, lds, acc);
-    gemm_epilogue_layout<BM, BN, float>(acc, [&](int r, int c, float v) {
+    b3::mainloop<BM, BN, 2, C::WGN>(la, lb, (nao + b3::BK - 1) / b3::BK, lds, acc);
+    b3::epilogue<BM, BN, 2, C::WGN>(acc, [&](int r, int c, float v) {
         const int i = i0 + r, k = k0 + c;
         if (i < nao && k < ldr) {
             u16 h, l;
@@ -445,15 +467,15 @@ __device__ __forceinline__ void step2_tile(const u16 *lhi, const u16 *llo, const
     const int kcs = (int)(s0 - Ls * kchunks);
     RLoader la{lhi, llo, strideL, ldr, i0, nao, kchunks, Ls, kcs};
     RLoader lb{rhi, rlo, strideL, ldr, j0, nao, kchunks, Ls, kcs};
-    b3::Acc32<b3::Shape<BM, BN>::MI, b3::Shape<BM, BN>::NI> acc;
+    b3::Acc32<b3::Shape<S2_B, S2_B>::MI, b3::Shape<S2_B, S2_B>::NI> acc;
     bool first = true;
     long s = s0;
     do {
         const int n = (int)min((long)S2_FLUSH, s1 - s);
         acc.zero();
-        b3::mainloop<BM, BN>(la, lb, n, lds, acc);
+        b3::mainloop<S2_B, S2_B, 2, 2>(la, lb, n, lds, acc);
         const bool add = beta || !first;
-        gemm_epilogue_layout<BM, BN, float>(acc, [&](int r, int c, float v) {
+        b3::epilogue<S2_B, S2_B, 2, 2>(acc, [&](int r, int c, float v) {
             const int i = i0 + r, j = j0 + c;
             if (i < nao && j < nao) {
                 double *dst = o + (long)i * nao + j;
