@@ -59,6 +59,24 @@ def log(msg):
     print(f'[bench] {msg}', file=sys.stderr, flush=True)
 
 
+def measured_traffic(nwater):
+    """HBM bytes per call of the K and J builds from the committed counter
+    summary (two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE, scaled by
+    a plain copy of known size); None when no summary covers this workload."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                        f'hbm_traffic_h2o{nwater}.md')
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith('bench-traffic-'):
+                    key, val = line.split(':', 1)
+                    out[key.strip()[len('bench-traffic-'):]] = float(val)
+    except OSError:
+        pass
+    return out
+
+
 def cpu_baseline(mol_nao, naux, npairs, nocc, args):
     """CPU restatement (oracle leg) on a sample of aux rows, scaled."""
     from oracle import oracle as O
@@ -366,6 +384,7 @@ def main():
 
     S = naux * npairs * 8
     bj = 2 * S + 16 * (npairs + naux)
+    traffic = measured_traffic(args.nwater) if args.nocc == 320 else {}
     fk = 2. * naux * nao * nao * nocc + 1. * naux * nocc * nao * (nao + 1)
     out = {
         'metric': 'get_jk (J+K) calls/hr, (H2O)64 def2-SVP FP64, single density',
@@ -388,12 +407,12 @@ def main():
         'cderi_build': build,
         'roofline': {'bound': 'mfma', 'achieved': fk / tk / 1e12,
                      'peak': FP64_MFMA_PEAK / 1e12, 'unit': 'TFLOP/s',
-                     'frac': fk / tk / FP64_MFMA_PEAK, 'traffic': None,
+                     'frac': fk / tk / FP64_MFMA_PEAK, 'traffic': traffic.get('k'),
                      'kernel': 'mdf_get_k (k_step1 + k_step2 + k_finalize)',
                      'algorithmic': fk},
         'roofline_j': {'bound': 'hbm', 'achieved': bj / tj / 1e9,
                        'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
-                       'frac': bj / tj / HBM_PEAK, 'traffic': None,
+                       'frac': bj / tj / HBM_PEAK, 'traffic': traffic.get('j'),
                        'kernel': 'mdf_get_j (j_rho_partial + j_vj_pass)',
                        'algorithmic': bj},
     }

@@ -174,6 +174,26 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None):
     return cderi, (pair_addr, diag)
 
 
+def tile_fill(mol, tile=128):
+    """Fraction of the lower-triangle ``tile x tile`` AO blocks that hold at
+    least one shell pair kept by the screening (``ints.PAIR_SCREEN``).  The
+    reference switches step 1 to its block-sparse tiled form only when this
+    fill is at most 0.75 (electrolites/fastdf.py:566-574, ``TILE_MAX_FILL``
+    :235); the dense clusters measured here have fill 1.  Host-only."""
+    bound = getattr(mol, '_mdf_pair_bound', None)
+    if bound is None:
+        bound = ints.pair_bound(mol.ao)
+        mol._mdf_pair_bound = bound
+    kept = np.tril(bound > ints.PAIR_SCREEN)
+    blk = np.asarray(mol.ao.ao_loc)[:-1] // tile
+    nt = (mol.nao + tile - 1) // tile
+    hit = np.zeros((nt, nt), bool)
+    I, J = np.nonzero(kept)
+    hit[blk[I], blk[J]] = True
+    hit |= hit.T
+    return float(hit[np.tril_indices(nt)].mean())
+
+
 def _cholesky_eri(intopt, omega=None, use_gpu_memory=None):
     """``df._cholesky_eri``'s signature.  ``intopt`` is this package's
     :class:`~electrolites_amd.df.DF` (or anything with a ``mol`` carrying

@@ -106,3 +106,18 @@ def test_library_error_message():
     from electrolites_amd import _lib
     with pytest.raises(_lib.MDFError, match='bad sizes'):
         _lib.call('mdf_get_k', None, -1, 0, 0, None, 0, None, None, 0, 0, 0, None, 0, 0, None, 0, None)
+
+
+def test_tile_fill_dense_configs():
+    """The block-sparse tiled step 1 of the reference (taken only at AO tile
+    fill <= 0.75) does not apply to the dense 1 g/cm^3 clusters: every
+    128 x 128 AO block holds kept shell pairs."""
+    from electrolites_amd import fastdfcderi
+    from electrolites_amd.gto import water_cluster
+    for n in (32, 64):
+        assert fastdfcderi.tile_fill(water_cluster(n, seed=0)) == 1.0
+    # a sparse arrangement (two waters 60 A apart) is not dense
+    from electrolites_amd.gto.mole import Mole
+    far = Mole([('O', (0., 0., 0.)), ('H', (0.76, 0.59, 0.)), ('H', (-0.76, 0.59, 0.)),
+                ('O', (60., 0., 0.)), ('H', (60.76, 0.59, 0.)), ('H', (59.24, 0.59, 0.))])
+    assert fastdfcderi.tile_fill(far, tile=8) < 0.75
