@@ -25,13 +25,27 @@ diagonalisation and the energy sums are FP64 throughout.  The per-cycle
 precision is recorded in ``history``.
 """
 import math
+import threading
 
 import numpy as np
+from threadpoolctl import threadpool_limits
 
 from . import _dev
 from . import fastdf
 from . import ints
 from .df import DF
+
+
+_HOST_LAPACK = threading.Lock()
+
+
+def host_eigh(a):
+    """LAPACK ``eigh`` on one host thread.  The thread count of a threaded
+    BLAS is process-wide and changes the last bits of the result, so every
+    host diagonalisation here runs single-threaded under one lock: the same
+    matrix gives the same eigenpairs whichever thread or server asks."""
+    with _HOST_LAPACK, threadpool_limits(1, user_api='blas'):
+        return np.linalg.eigh(a)
 
 
 class RHF:
@@ -130,7 +144,7 @@ class RHF:
         t = _dev.torch()
         A = self.X.T @ F @ self.X
         if A.shape[0] <= self.host_diag_max:
-            e, c = np.linalg.eigh(A.cpu().numpy())
+            e, c = host_eigh(A.cpu().numpy())
             e = t.from_numpy(e).to(A.device)
             c = t.from_numpy(c).to(A.device)
         else:

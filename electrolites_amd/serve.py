@@ -23,7 +23,10 @@ request), so the requests absorb concurrently: each has its own HIP stream
 there; the torch operators and the library calls release the interpreter
 lock while they run, so the diagonalisations of different requests overlap.
 Orthogonal dimensions up to ``host_diag_max`` are diagonalised by LAPACK on
-host cores (one thread each), larger ones on the GPU.  Each stream waits on
+the host (single-threaded, serialised: :func:`scf.host_eigh`), larger ones
+on the GPU.  ``pipeline=True`` splits the requests into two groups whose
+J/K and absorb phases alternate (measured no faster here: the loop is
+GPU-bound).  Each stream waits on
 an event recorded after the grouped J/K launches, and the next tick's
 launches wait on events recorded after every request's absorb.
 
@@ -47,7 +50,6 @@ import math
 import time
 
 import numpy as np
-from threadpoolctl import threadpool_limits
 
 from . import _dev
 from . import _lib
@@ -129,6 +131,8 @@ class _Slot:
         self.nao = nao
         self._vj = t.empty((1, nao, nao), dtype=t.float64, device=self.cderi.device)
         self.stream = t.cuda.Stream()
+        self.jk_done = None           # event after this slot's launched J/K
+        self.ready = None             # event after this slot's last absorb
 
     def load_density(self):
         """Density factor and packed density of the current ``scf.dm``, the
@@ -157,8 +161,11 @@ class Server:
 
     def __init__(self, hbm_budget=HBM_BUDGET, max_batch=64, conv_tol=1e-10, max_cycle=100,
                  graphs=False, workers=16, host_diag_max=400, async_build=True,
-                 low_precision='bf16x3'):
+                 low_precision='bf16x3', pipeline=False):
         self.hbm_budget = float(hbm_budget)
+        self.pipeline = pipeline
+        self._plan_tick = {}
+        self._weight = [0., 0.]
         self.workers = workers
         self.host_diag_max = host_diag_max
         self.low_precision = low_precision
@@ -210,8 +217,8 @@ class Server:
             req.t_admit = self.now()
             self.resident += req.bytes
             if self._builder is None:
-                self.active.append(_Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max,
-                                         self.low_precision))
+                self._activate(_Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max,
+                                     self.low_precision))
             else:
                 self._building.append(self._builder.submit(self._build, req))
         # requests whose build finished join the batch; when nothing is
@@ -223,13 +230,21 @@ class Server:
             if f.done():
                 slot, ev = f.result()
                 t.cuda.current_stream().wait_event(ev)
-                self.active.append(slot)
+                self._activate(slot)
             else:
                 still.append(f)
         self._building = still
 
+    def _activate(self, slot):
+        """Join the batch, in the lighter of the two pipeline groups (work
+        weighed as nao^3)."""
+        slot.weight = float(slot.nao) ** 3
+        slot.group = 0 if self._weight[0] <= self._weight[1] else 1
+        self._weight[slot.group] += slot.weight
+        self.active.append(slot)
+
     def _plan(self, slots, prec):
-        key = (prec,) + tuple(id(s) for s in slots)
+        key = (prec,) + tuple(sorted(id(s) for s in slots))
         g = self._plans.get(key)
         if g is None:
             g = batch.GroupedJK([s.entry for s in slots], precision=prec, tune=False)
@@ -240,8 +255,61 @@ class Server:
             g.capture_graph()
         return key, g
 
+    def _retire(self, s):
+        r, scf = s.req, s.scf
+        r.t_done = self.now()
+        r.e_tot, r.cycles, r.converged = scf.e_tot, scf.cycles, scf.converged
+        r.history = scf.history
+        self.resident -= r.bytes
+        self.done.append(r)
+
+    def _launch(self, slots):
+        """Grouped J/K of the slots' current densities on the current stream;
+        returns the event that marks its end."""
+        t = _dev.torch()
+        main = t.cuda.current_stream()
+        groups = collections.defaultdict(list)
+        for s in slots:
+            if s.ready is not None:
+                main.wait_event(s.ready)        # its absorb wrote the density
+            s.load_density()
+            groups[s.scf.prec].append(s)
+        for prec, members in sorted(groups.items()):
+            key, g = self._plan(members, prec)
+            self._plan_tick[key] = self.ticks
+            g.run()
+        ev = t.cuda.Event()
+        ev.record(main)
+        return ev
+
+    def _absorb_all(self, slots):
+        """Each slot absorbs its J/K on its own stream from a worker thread;
+        returns the slots that finished."""
+        t = _dev.torch()
+
+        def absorb(s):
+            with t.cuda.stream(s.stream):
+                s.stream.wait_event(s.jk_done)
+                finished = s.absorb()
+                ev = t.cuda.Event()
+                ev.record(s.stream)
+            s.jk_done, s.ready = None, ev
+            return finished
+
+        if self._pool is not None and len(slots) > 1:
+            done = list(self._pool.map(absorb, slots))
+        else:
+            done = [absorb(s) for s in slots]
+        return [s for s, d in zip(slots, done) if d]
+
     def step(self):
-        """Admit, then advance every active request by one SCF cycle."""
+        """Admit, then one tick.  Lockstep (``pipeline=False``): every active
+        request gets its J/K, then absorbs it.  Pipelined: the requests are
+        in two groups; a tick launches one group's J/K and, while the GPU
+        runs it, absorbs the other group's J/K from the previous tick, so the
+        host-side and latency-bound parts of one group overlap the other
+        group's contractions.  Either way each request follows its own
+        stand-alone cycle sequence."""
         t = _dev.torch()
         t0 = time.perf_counter()
         self._admit()
@@ -250,55 +318,28 @@ class Server:
         if not self.active:
             return 0
         self.ticks += 1
-        groups = collections.defaultdict(list)
-        for s in self.active:
-            s.load_density()
-            groups[s.scf.prec].append(s)
-        live = set()
-        for prec, slots in sorted(groups.items()):
-            key, g = self._plan(slots, prec)
-            live.add(key)
-            g.run()
-        for key in list(self._plans):
-            if key not in live:
-                self._plans.pop(key).close()
-        main = t.cuda.current_stream()
-        jk_done = t.cuda.Event()
-        jk_done.record(main)
-
-        def absorb(s):
-            with t.cuda.stream(s.stream):
-                s.stream.wait_event(jk_done)
-                finished = s.absorb()
-                ev = t.cuda.Event()
-                ev.record(s.stream)
-            return finished, ev
-
-        # host diagonalisations run one LAPACK thread each, side by side
-        with threadpool_limits(1, user_api='blas'):
-            if self._pool is not None and len(self.active) > 1:
-                results = list(self._pool.map(absorb, self.active))
-            else:
-                results = [absorb(s) for s in self.active]
-        for _, ev in results:
-            main.wait_event(ev)
+        if self.pipeline:
+            g = self.ticks % 2
+            launch = [s for s in self.active if s.group == g and s.jk_done is None]
+            absorb = [s for s in self.active if s.group != g and s.jk_done is not None]
+        else:
+            launch = absorb = [s for s in self.active if s.jk_done is None]
+        if launch:
+            ev = self._launch(launch)
+            for s in launch:
+                s.jk_done = ev
+        finished = self._absorb_all(absorb) if absorb else []
+        for key in [k for k, tk in self._plan_tick.items() if tk < self.ticks - 2]:
+            self._plans.pop(key).close()
+            del self._plan_tick[key]
         self.timing['tick_s'] += time.perf_counter() - t1
-        still = []
-        retired = False
-        for s, (finished, _) in zip(self.active, results):
-            if finished:
-                retired = True
-                r, scf = s.req, s.scf
-                r.t_done = self.now()
-                r.e_tot, r.cycles, r.converged = scf.e_tot, scf.cycles, scf.converged
-                r.history = scf.history
-                self.resident -= r.bytes
-                self.done.append(r)
-            else:
-                still.append(s)
-        if retired:
-            t.cuda.synchronize()      # a retired request's buffers were used on two streams
-        self.active = still
+        for s in finished:
+            s.stream.synchronize()
+            self._retire(s)
+            self._weight[s.group] -= s.weight
+        if finished:
+            gone = set(map(id, finished))
+            self.active = [s for s in self.active if id(s) not in gone]
         self.trace.append((self.now(), len(self.active), len(self.queue), self.resident))
         return len(self.active)
 
