@@ -82,46 +82,71 @@ int launch_gemm(int M, int N, int K, double alpha, const double *A, long lda, co
 }
 
 // Cholesky of one diagonal block (n <= 128, lower, in place) by one
-// workgroup, then its inverse into D (ldd = 128).  A non-positive pivot
+// workgroup of 1024 threads, then its inverse into D (ldd = 128).  Both
+// triangles live in LDS, packed (i(i+1)/2 + j, 2 x 66 KB), so the 128
+// dependent steps of each phase touch LDS only.  A non-positive pivot
 // records (global index + 1) in *info and continues with pivot 1.
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double *A, long lda, int n, int k0, double *D,
-                                                         int *info) {
+constexpr int DIAG_THREADS = 1024;
+constexpr int DIAG_PARTS = DIAG_THREADS / LB;     // partial sums per inverse entry
+
+__device__ __forceinline__ int ptri(int i) { return i * (i + 1) / 2; }
+
+__global__ __launch_bounds__(DIAG_THREADS) void potrf_diag_kernel(double *A, long lda, int n, int k0,
+                                                                  double *D, int *info) {
+    __shared__ double Lp[LB * (LB + 1) / 2];
+    __shared__ double Xp[LB * (LB + 1) / 2];
+    __shared__ double red[DIAG_PARTS][LB];
     const int t = threadIdx.x;
+    for (int e = t; e < n * n; e += DIAG_THREADS) {
+        const int i = e / n, j = e % n;
+        if (j <= i) Lp[ptri(i) + j] = A[(long)i * lda + j];
+    }
+    // right-looking factorisation
     for (int k = 0; k < n; k++) {
         __syncthreads();
-        const double d = A[(long)k * lda + k];
+        const double d = Lp[ptri(k) + k];
         const bool bad = !(d > 0.);
         const double piv = bad ? 1. : sqrt(d);
         __syncthreads();
         if (t == 0) {
-            A[(long)k * lda + k] = piv;
+            Lp[ptri(k) + k] = piv;
             if (bad && info) atomicCAS(info, 0, k0 + k + 1);
         }
-        for (int i = k + 1 + t; i < n; i += 256) A[(long)i * lda + k] /= piv;
+        for (int i = k + 1 + t; i < n; i += DIAG_THREADS) Lp[ptri(i) + k] /= piv;
         __syncthreads();
         const int m = n - k - 1;
-        for (long e = t; e < (long)m * m; e += 256) {
-            const int i = k + 1 + (int)(e / m), j = k + 1 + (int)(e % m);
-            if (j <= i) A[(long)i * lda + j] -= A[(long)i * lda + k] * A[(long)j * lda + k];
+        for (int e = t; e < m * m; e += DIAG_THREADS) {
+            const int i = k + 1 + e / m, j = k + 1 + e % m;
+            if (j <= i) Lp[ptri(i) + j] -= Lp[ptri(i) + k] * Lp[ptri(j) + k];
         }
     }
     __syncthreads();
-    // zero the strict upper triangle of the block, so the factor is clean
-    for (long e = t; e < (long)n * n; e += 256) {
-        const int i = (int)(e / n), j = (int)(e % n);
-        if (j > i) A[(long)i * lda + j] = 0.;
-    }
-    // inverse, one column per thread: X[j][j] = 1/L[j][j], forward substitution
-    for (int j = t; j < LB; j += 256) {
-        for (int i = 0; i < LB; i++) {
-            double x = 0.;
-            if (j < n && i < n && i >= j) {
-                double s = i == j ? 1. : 0.;
-                for (int k = j; k < i; k++) s -= A[(long)i * lda + k] * D[(long)k * LB + j];
-                x = s / A[(long)i * lda + i];
-            }
-            D[(long)i * LB + j] = x;
+    // inverse by forward substitution, one row per step:
+    // X[i][j] = (delta_ij - sum_{j<=k<i} L[i][k] X[k][j]) / L[i][i], the sum
+    // split over DIAG_PARTS threads per column and added in a fixed order
+    const int j = t % LB, part = t / LB;
+    for (int i = 0; i < n; i++) {
+        double s = 0.;
+        if (j < i)
+            for (int k = j + part; k < i; k += DIAG_PARTS) s += Lp[ptri(i) + k] * Xp[ptri(k) + j];
+        red[part][j] = s;
+        __syncthreads();
+        if (part == 0 && j <= i) {
+            double acc = 0.;
+#pragma unroll
+            for (int q = 0; q < DIAG_PARTS; q++) acc += red[q][j];
+            Xp[ptri(i) + j] = ((i == j ? 1. : 0.) - acc) / Lp[ptri(i) + i];
         }
+        __syncthreads();
+    }
+    // the factor back (strict upper triangle of the block zeroed) and D
+    for (int e = t; e < n * n; e += DIAG_THREADS) {
+        const int i = e / n, c = e % n;
+        A[(long)i * lda + c] = c <= i ? Lp[ptri(i) + c] : 0.;
+    }
+    for (int e = t; e < LB * LB; e += DIAG_THREADS) {
+        const int i = e / LB, c = e % LB;
+        D[(long)i * LB + c] = (i < n && c <= i) ? Xp[ptri(i) + c] : 0.;
     }
 }
 
@@ -165,7 +190,7 @@ extern "C" int mdf_potrf(double *A, long n, long lda, int *info, void *work, siz
         const int kb = (int)(n - k0 < LB ? n - k0 : LB);
         double *Akk = A + k0 * lda + k0;
         double *Dk = D + (k0 / LB) * LB * LB;
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, Akk, lda, kb, (int)k0, Dk, info);
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(DIAG_THREADS), 0, st, Akk, lda, kb, (int)k0, Dk, info);
         MDF_LAUNCHED("potrf_diag_kernel");
         const long m2 = n - k0 - kb;
         if (m2 <= 0) break;
