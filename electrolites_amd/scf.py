@@ -37,13 +37,44 @@ from .df import DF
 
 
 _HOST_LAPACK = threading.Lock()
+_single_blas = 0          # number of open single_thread_blas contexts
+_single_lim = None
 
 
-def host_eigh(a):
+class single_thread_blas:
+    """Context: BLAS/LAPACK at one thread for the whole process (the first
+    context to open sets it, the last to close restores it), so that
+    :func:`host_eigh` calls with ``concurrent=True`` from many threads run
+    side by side while one is open."""
+
+    def __enter__(self):
+        global _single_blas, _single_lim
+        with _HOST_LAPACK:
+            if _single_blas == 0:
+                _single_lim = threadpool_limits(1, user_api='blas')
+            _single_blas += 1
+        return self
+
+    def __exit__(self, *exc):
+        global _single_blas, _single_lim
+        with _HOST_LAPACK:
+            _single_blas -= 1
+            if _single_blas == 0:
+                _single_lim.restore_original_limits()
+                _single_lim = None
+
+
+def host_eigh(a, concurrent=False):
     """LAPACK ``eigh`` on one host thread.  The thread count of a threaded
     BLAS is process-wide and changes the last bits of the result, so every
-    host diagonalisation here runs single-threaded under one lock: the same
-    matrix gives the same eigenpairs whichever thread or server asks."""
+    host diagonalisation here is single-threaded: with ``concurrent`` the
+    caller guarantees an open :class:`single_thread_blas` context and calls
+    run side by side; otherwise they are serialised under one lock.  The same
+    matrix gives the same eigenpairs either way."""
+    if concurrent:
+        if not _single_blas:
+            raise RuntimeError('host_eigh(concurrent=True) outside single_thread_blas')
+        return np.linalg.eigh(a)
     with _HOST_LAPACK, threadpool_limits(1, user_api='blas'):
         return np.linalg.eigh(a)
 
@@ -59,6 +90,7 @@ class RHF:
         self.k_precision = k_precision
         self.low_precision = low_precision
         self.host_diag_max = host_diag_max
+        self.concurrent_diag = False      # set by a caller that holds single_thread_blas
         self.switch_err = switch_err
         self.mol = mol
         self.conv_tol = conv_tol
@@ -144,7 +176,7 @@ class RHF:
         t = _dev.torch()
         A = self.X.T @ F @ self.X
         if A.shape[0] <= self.host_diag_max:
-            e, c = host_eigh(A.cpu().numpy())
+            e, c = host_eigh(A.cpu().numpy(), self.concurrent_diag)
             e = t.from_numpy(e).to(A.device)
             c = t.from_numpy(c).to(A.device)
         else:

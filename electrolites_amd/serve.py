@@ -55,7 +55,7 @@ from . import _dev
 from . import _lib
 from . import batch
 from . import fastdf
-from .scf import RHF
+from .scf import RHF, single_thread_blas
 
 #: default resident-byte budget (of the 288 GB of HBM, leaving headroom for
 #: workspaces and the framework's allocator)
@@ -120,6 +120,8 @@ class _Slot:
         self.scf = RHF(req.mol, conv_tol=conv_tol, max_cycle=max_cycle,
                        k_precision=req.k_precision, host_diag_max=host_diag_max,
                        low_precision=low_precision)
+        # the server constructs and absorbs slots inside single_thread_blas
+        self.scf.concurrent_diag = True
         self.scf.build()
         self.scf.start()
         dfobj = self.scf.with_df
@@ -203,7 +205,7 @@ class Server:
     def _build(self, req):
         t = _dev.torch()
         stream = t.cuda.Stream()
-        with t.cuda.stream(stream):
+        with t.cuda.stream(stream), single_thread_blas():
             slot = _Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max, self.low_precision)
             ev = t.cuda.Event()
             ev.record(stream)
@@ -217,8 +219,9 @@ class Server:
             req.t_admit = self.now()
             self.resident += req.bytes
             if self._builder is None:
-                self._activate(_Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max,
-                                     self.low_precision))
+                with single_thread_blas():
+                    self._activate(_Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max,
+                                         self.low_precision))
             else:
                 self._building.append(self._builder.submit(self._build, req))
         # requests whose build finished join the batch; when nothing is
@@ -296,10 +299,11 @@ class Server:
             s.jk_done, s.ready = None, ev
             return finished
 
-        if self._pool is not None and len(slots) > 1:
-            done = list(self._pool.map(absorb, slots))
-        else:
-            done = [absorb(s) for s in slots]
+        with single_thread_blas():
+            if self._pool is not None and len(slots) > 1:
+                done = list(self._pool.map(absorb, slots))
+            else:
+                done = [absorb(s) for s in slots]
         return [s for s, d in zip(slots, done) if d]
 
     def step(self):
