@@ -382,6 +382,36 @@ def main():
     del vk64
     tk32, k32_err = low['fp32']
 
+    # FP32 storage (the FASTDF_FP32 mode): the same tensor rounded to FP32,
+    # widened to FP64 in the kernels
+    c32 = cderi.float()
+
+    class DF32:
+        _cderi = {torch.cuda.current_device(): c32}
+        _cderi_idx = None
+
+        def build(self):
+            pass
+
+    df32 = DF32()
+    vj64 = fastdf.get_jk(dfobj, dm, hermi=1, with_k=False)[0]
+    fastdf.get_jk(df32, dm, hermi=1, with_k=False)
+    e3 = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    e3[0].record()
+    vj32 = fastdf.get_jk(df32, dm, hermi=1, with_k=False)[0]
+    e3[1].record()
+    vk32s = fastdf._vk(c32, fastdf._layout(df32, c32, nao), fl, fr, 'fp64')
+    e3[2].record()
+    torch.cuda.synchronize()
+    fp32_store = {'j_ms': e3[0].elapsed_time(e3[1]), 'k_ms': e3[1].elapsed_time(e3[2]),
+                  'cderi_GB': naux * npairs * 4 / 1e9,
+                  'j_rel_err_vs_fp64_store': float((vj32 - vj64).abs().max() / vj64.abs().max())}
+    bj32 = naux * npairs * 8 + 16 * (npairs + naux)
+    fp32_store['j_hbm_frac'] = bj32 / (fp32_store['j_ms'] * 1e-3) / HBM_PEAK
+    log(f'FP32 storage: J {fp32_store["j_ms"]:.2f} ms ({100 * fp32_store["j_hbm_frac"]:.1f} % HBM), '
+        f'K {fp32_store["k_ms"]:.1f} ms')
+    del c32, vj32, vk32s, vj64, df32
+
     S = naux * npairs * 8
     bj = 2 * S + 16 * (npairs + naux)
     traffic = measured_traffic(args.nwater) if args.nocc == 320 else {}
@@ -405,6 +435,7 @@ def main():
                       'k_fp32_rel_err_vs_fp64': k32_err, 'k_bf16x3': low['bf16x3'][0] * 1e3,
                       'k_bf16x3_rel_err_vs_fp64': low['bf16x3'][1]},
         'cderi_build': build,
+        'fp32_storage': fp32_store,
         'roofline': {'bound': 'mfma', 'achieved': fk / tk / 1e12,
                      'peak': FP64_MFMA_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': fk / tk / FP64_MFMA_PEAK, 'traffic': traffic.get('k'),

@@ -9,7 +9,8 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, 'libmi355df.so')
-ABI_VERSION = 3
+ABI_VERSION = 4
+DTYPE_F64, DTYPE_F32 = 0, 1
 PREC_FP64 = 0
 PREC_FP32 = 1
 
@@ -26,13 +27,14 @@ _SIGS = {
     'mdf_device_info': (I, [ctypes.POINTER(I), ctypes.POINTER(ctypes.c_longlong),
                             ctypes.c_char_p, I]),
     'mdf_get_j_workspace': (SZ, [L, L, I]),
-    'mdf_get_j': (I, [P, L, L, L, P, I, L, P, L, P, L, I, P, SZ, P]),
+    'mdf_get_j': (I, [P, I, L, L, L, P, I, L, P, L, P, L, I, P, SZ, P]),
     'mdf_get_k_workspace': (SZ, [L, I, I, I, I, L]),
-    'mdf_get_k': (I, [P, L, L, L, P, I, P, P, I, L, I, P, L, I, P, SZ, P]),
+    'mdf_get_k': (I, [P, I, L, L, L, P, I, P, P, I, L, I, P, L, I, P, SZ, P]),
     'mdf_pack_dm': (I, [P, I, I, L, P, L, I, P, L, P]),
     'mdf_unpack_pairs': (I, [P, I, L, P, L, I, P, L, P]),
     'mdf_build_pair_map': (I, [P, L, I, P, P]),
     'mdf_fill_zero': (I, [P, L, P]),
+    'mdf_cast_f32': (I, [P, L, L, L, P, L, P]),
     'mdf_int3c2e': (I, [P, P, P, P, P, P, L, P, L, P]),
     'mdf_int2c2e': (I, [P, P, P, P, L, P]),
     'mdf_int1e': (I, [P, I, P, P, I, P, P, P, P, P, P]),

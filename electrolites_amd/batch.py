@@ -29,14 +29,15 @@ class JKItem(ctypes.Structure):
     _fields_ = [('cderi', P), ('ld', ctypes.c_long), ('naux', ctypes.c_long),
                 ('npairs', ctypes.c_long), ('nao', ctypes.c_int), ('pair_map', P),
                 ('n_dm', ctypes.c_int), ('dm_pairs', P), ('rho', P), ('vj_pairs', P),
-                ('fl', P), ('fr', P), ('nocc', ctypes.c_int), ('vk', P)]
+                ('fl', P), ('fr', P), ('nocc', ctypes.c_int), ('vk', P), ('cderi_dtype', ctypes.c_int)]
 
 
 
 class Entry:
     """One molecule's device buffers for a grouped run.
 
-    ``cderi`` (naux, npairs) float64; ``pair_map`` int32 (nao*nao) or None;
+    ``cderi`` (naux, npairs) float64 or float32 (FP32 storage, widened on
+    load; one storage type per batch); ``pair_map`` int32 (nao*nao) or None;
     ``dm_pairs`` (n_dm, npairs); ``fl``/``fr`` (n_dm, nao, nocc); outputs
     ``rho`` (n_dm, naux), ``vj_pairs`` (n_dm, npairs), ``vk`` (n_dm, nao, nao)
     are allocated here unless given.
@@ -60,7 +61,7 @@ class Entry:
         p = lambda x: None if x is None else x.data_ptr()
         return JKItem(self.cderi.data_ptr(), self.cderi.stride(0), self.naux, self.npairs, self.nao,
                       p(self.pair_map), self.n_dm, p(self.dm_pairs), p(self.rho), p(self.vj_pairs),
-                      p(self.fl), p(self.fr), self.nocc, p(self.vk))
+                      p(self.fl), p(self.fr), self.nocc, p(self.vk), fastdf._dtype_code(self.cderi))
 
     def signature(self):
         return (self.nao, self.naux, self.npairs, self.n_dm, self.nocc, self.fr is not None)

@@ -37,7 +37,7 @@ constexpr int GJ_NDM = 8;   // densities per molecule in one grouped J pass
 
 // Per-molecule J work on the device.
 struct GJItem {
-    const double *cderi;
+    const void *cderi;      // double or float (the plan's storage type)
     long ld, naux, npairs;
     const double *d;        // (ndm, npairs)
     double *rho, *vj;       // (ndm, naux), (ndm, npairs)
@@ -48,7 +48,7 @@ struct GJItem {
 
 // Per-(molecule, density) K work on the device.
 struct GKItem {
-    const double *cderi;
+    const void *cderi;
     long ld, naux;
     const int *pmap;
     const double *fl, *fr;  // (nao, nocc); fr NULL -> symmetric
@@ -81,6 +81,7 @@ struct Phase {
 
 struct mdf_grouped_plan {
     int with_j, with_k, prec, bn;
+    int dtype;                          // cderi storage type of every item
     int nj, nk;                         // items
     long t_j1, t_jr1, t_j2, t_jr2;      // tile counts
     long t_k1, t_k2, t_kf;
@@ -102,18 +103,21 @@ namespace {
 constexpr long RED_CHUNK = 4096;    // elements per reduction tile
 constexpr size_t MDF_GROUPED_DEFAULT_LIMIT = 32ull << 30;   // K wave workspace
 
+template <typename S>
 __global__ __launch_bounds__(jcore::J_THREADS) void gj_pass1(const GJItem *items, const Tile *tiles, long n,
                                                              char *work) {
+    constexpr int W = 16 / sizeof(S);       // values per 16-byte vector
     for (long t = blockIdx.x; t < n; t += gridDim.x) {
         const Tile e = tiles[t];
         const GJItem &it = items[e.item];
         double *part = (double *)(work + (size_t)it.part1);
-        if (it.vec == 2)
-            jcore::rho_partial_tile<GJ_NDM, 2>(it.cderi, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part,
-                                               e.a, e.b);
+        const S *L = (const S *)it.cderi;
+        if (it.vec == W)
+            jcore::rho_partial_tile<GJ_NDM, W, S>(L, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part, e.a,
+                                                  e.b);
         else
-            jcore::rho_partial_tile<GJ_NDM, 1>(it.cderi, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part,
-                                               e.a, e.b);
+            jcore::rho_partial_tile<GJ_NDM, 1, S>(L, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part, e.a,
+                                                  e.b);
     }
 }
 
@@ -128,19 +132,22 @@ __global__ __launch_bounds__(jcore::J_THREADS) void gj_reduce1(const GJItem *ite
     }
 }
 
+template <typename S>
 __global__ __launch_bounds__(jcore::J_THREADS) void gj_pass2(const GJItem *items, const Tile *tiles, long n,
                                                              char *work) {
+    constexpr int W = 16 / sizeof(S);
     for (long t = blockIdx.x; t < n; t += gridDim.x) {
         const Tile e = tiles[t];
         const GJItem &it = items[e.item];
         const long r0 = (long)e.b * it.rows_per_group, r1 = min(r0 + it.rows_per_group, it.naux);
         double *out = (double *)(work + (size_t)it.part2) + (long)e.b * it.ndm * it.npairs;
-        if (it.vec == 2)
-            jcore::vj_pass_tile<GJ_NDM, 2>(it.cderi, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out,
-                                           it.npairs, e.a);
+        const S *L = (const S *)it.cderi;
+        if (it.vec == W)
+            jcore::vj_pass_tile<GJ_NDM, W, S>(L, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out, it.npairs,
+                                              e.a);
         else
-            jcore::vj_pass_tile<GJ_NDM, 1>(it.cderi, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out,
-                                           it.npairs, e.a);
+            jcore::vj_pass_tile<GJ_NDM, 1, S>(L, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out, it.npairs,
+                                              e.a);
     }
 }
 
@@ -161,7 +168,7 @@ __global__ __launch_bounds__(jcore::J_THREADS) void gj_reduce2(const GJItem *ite
     const long xr_per = gridDim.x / N_XCD, xr_x = blockIdx.x % N_XCD, xr_j = blockIdx.x / N_XCD; \
     for (long t = (n) * xr_x / N_XCD + xr_j; t < (n) * (xr_x + 1) / N_XCD; t += xr_per)
 
-template <int BN, typename T>
+template <int BN, typename T, typename Src>
 __global__ __launch_bounds__(G_THREADS) void gk_step1(const GKItem *items, const Tile *tiles, long n, char *work) {
     __shared__ T lds[TileShape<128, BN>::LDS_ELEMS];
     XCD_RUN_LOOP(t, n) {
@@ -172,13 +179,13 @@ __global__ __launch_bounds__(G_THREADS) void gk_step1(const GKItem *items, const
         const double *F = right ? it.fr : it.fl;
         const long lb = e.a % it.nL;        // row within the pass
         T *R = (T *)(work + (size_t)(right ? it.rr : it.rl)) + lb * it.nao * it.nocc;
-        const double *row = it.cderi + (long)e.a * it.ld;
+        const Src *row = (const Src *)it.cderi + (long)e.a * it.ld;
         if (it.pmap)
-            kcore::step1_tile<BN, true, T>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
-                                           ktile * BN, lds);
+            kcore::step1_tile<BN, true, T, Src>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
+                                                ktile * BN, lds);
         else
-            kcore::step1_tile<BN, false, T>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
-                                            ktile * BN, lds);
+            kcore::step1_tile<BN, false, T, Src>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
+                                                 ktile * BN, lds);
         __syncthreads();
     }
 }
@@ -228,7 +235,7 @@ __global__ __launch_bounds__(256) void gk_split_ft(const GKItem *items, const Ti
     }
 }
 
-template <bool MAPPED, int BN>
+template <bool MAPPED, int BN, typename Src>
 __global__ __launch_bounds__(kcore::kb3::Step1Cfg<BN>::THREADS) void gk_step1_b3(const GKItem *items,
                                                                                  const Tile *tiles, long n,
                                                                                  char *work) {
@@ -244,10 +251,10 @@ __global__ __launch_bounds__(kcore::kb3::Step1Cfg<BN>::THREADS) void gk_step1_b3
         const u16 *ftlo = (const u16 *)(work + (size_t)it.b3[side][1]);
         u16 *rhi = (u16 *)(work + (size_t)it.b3[side][2]) + off;
         u16 *rlo = (u16 *)(work + (size_t)it.b3[side][3]) + off;
-        const double *row = it.cderi + (long)e.a * it.ld;
+        const Src *row = (const Src *)it.cderi + (long)e.a * it.ld;
         // a phase holds only mapped or only unmapped items (one instance
         // each keeps the register budget of the single-item kernel)
-        kcore::kb3::step1_tile<MAPPED, BN>(row, it.pmap, it.nao, fthi, ftlo, it.ldft, it.nocc, rhi, rlo, it.ldr,
+        kcore::kb3::step1_tile<MAPPED, BN, Src>(row, it.pmap, it.nao, fthi, ftlo, it.ldft, it.nocc, rhi, rlo, it.ldr,
                                            e.b * kcore::kb3::BM, ktile * BN, lds);
         __syncthreads();
     }
@@ -308,11 +315,18 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32 || prec == MDF_PREC_BF16X3,
                   "mdf_grouped_plan_create: precision %d", prec);
     MDF_CHECK_ARG(bn == 0 || bn == 64 || bn == 128 || bn == 160, "mdf_grouped_plan_create: bn %d not in {0,64,128,160}", bn);
+    for (int m = 0; m < nitems; m++)
+        MDF_CHECK_ARG(items[m].cderi_dtype == items[0].cderi_dtype &&
+                          (items[m].cderi_dtype == MDF_DTYPE_F64 || items[m].cderi_dtype == MDF_DTYPE_F32),
+                      "mdf_grouped_plan_create: item %d cderi dtype %d (one storage type per plan)", m,
+                      items[m].cderi_dtype);
     const size_t elem = prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double);
     mdf_grouped_plan *p = new mdf_grouped_plan();
     p->with_j = with_j;
     p->with_k = with_k;
     p->prec = prec;
+    p->dtype = nitems ? items[0].cderi_dtype : MDF_DTYPE_F64;
+    const int wvec = p->dtype == MDF_DTYPE_F32 ? 4 : 2;     // values per 16-byte vector
     std::vector<Tile> j1, jr1, j2, jr2, k0, k1, k2, kf;
     std::vector<int> kgrid;             // step-2 output tiles per K item
     if (work_limit == 0) work_limit = MDF_GROUPED_DEFAULT_LIMIT;
@@ -338,8 +352,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
             GJItem g{};
             g.cderi = x.cderi; g.ld = x.ld; g.naux = x.naux; g.npairs = x.npairs;
             g.d = x.dm_pairs; g.rho = x.rho; g.vj = x.vj_pairs; g.ndm = x.n_dm;
-            g.vec = ((x.ld % 2 == 0) && (x.npairs % 2 == 0) && ((uintptr_t)x.cderi % 16 == 0) &&
-                     ((uintptr_t)x.dm_pairs % 16 == 0)) ? 2 : 1;
+            g.vec = ((x.ld % wvec == 0) && ((uintptr_t)x.cderi % 16 == 0)) ? wvec : 1;
             jcore::JPlan pl = jcore::plan_j(x.naux, x.npairs, x.n_dm, g.vec);
             g.nchunk = pl.nchunk; g.ngroup = pl.ngroup; g.rows_per_group = pl.rows_per_group; g.ncblock = pl.ncblock;
             g.part1 = (double *)take(pl.part1_words * sizeof(double));
@@ -525,21 +538,32 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     return MDF_OK;
 }
 
-template <int BN>
-static void launch_gk1(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
+template <int BN, typename Src>
+static void launch_gk1_s(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
     const int g = persistent_grid(n, 2);
     if (p->prec == MDF_PREC_FP32)
-        hipLaunchKernelGGL((gk_step1<BN, float>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
+        hipLaunchKernelGGL((gk_step1<BN, float, Src>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
     else
-        hipLaunchKernelGGL((gk_step1<BN, double>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
+        hipLaunchKernelGGL((gk_step1<BN, double, Src>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
+}
+
+template <int BN>
+static void launch_gk1(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
+    if (p->dtype == MDF_DTYPE_F32)
+        launch_gk1_s<BN, float>(p, tiles, n, w, st);
+    else
+        launch_gk1_s<BN, double>(p, tiles, n, w, st);
 }
 
 template <bool MAPPED, int BN>
 static void launch_gk1_b3(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
     // one workgroup per CU (the 256- and 320-wide tiles hold 120-140 KB of LDS)
     const int per_cu = BN == 128 ? 2 : 1;
-    hipLaunchKernelGGL((gk_step1_b3<MAPPED, BN>), dim3(persistent_grid(n, per_cu)),
-                       dim3(kcore::kb3::Step1Cfg<BN>::THREADS), 0, st, p->kitems, tiles, n, w);
+    const dim3 g(persistent_grid(n, per_cu)), b(kcore::kb3::Step1Cfg<BN>::THREADS);
+    if (p->dtype == MDF_DTYPE_F32)
+        hipLaunchKernelGGL((gk_step1_b3<MAPPED, BN, float>), g, b, 0, st, p->kitems, tiles, n, w);
+    else
+        hipLaunchKernelGGL((gk_step1_b3<MAPPED, BN, double>), g, b, 0, st, p->kitems, tiles, n, w);
 }
 
 extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work_bytes, void *stream) {
@@ -551,14 +575,22 @@ extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work
     hipStream_t st = as_stream(stream);
     char *w = (char *)work;
     if (p->with_j && p->nj) {
-        hipLaunchKernelGGL(gj_pass1, dim3(persistent_grid(p->t_j1, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
-                           p->j1, p->t_j1, w);
+        if (p->dtype == MDF_DTYPE_F32)
+            hipLaunchKernelGGL(gj_pass1<float>, dim3(persistent_grid(p->t_j1, 4)), dim3(jcore::J_THREADS), 0, st,
+                               p->jitems, p->j1, p->t_j1, w);
+        else
+            hipLaunchKernelGGL(gj_pass1<double>, dim3(persistent_grid(p->t_j1, 4)), dim3(jcore::J_THREADS), 0, st,
+                               p->jitems, p->j1, p->t_j1, w);
         MDF_LAUNCHED("gj_pass1");
         hipLaunchKernelGGL(gj_reduce1, dim3(persistent_grid(p->t_jr1, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
                            p->jr1, p->t_jr1, w);
         MDF_LAUNCHED("gj_reduce1");
-        hipLaunchKernelGGL(gj_pass2, dim3(persistent_grid(p->t_j2, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
-                           p->j2, p->t_j2, w);
+        if (p->dtype == MDF_DTYPE_F32)
+            hipLaunchKernelGGL(gj_pass2<float>, dim3(persistent_grid(p->t_j2, 4)), dim3(jcore::J_THREADS), 0, st,
+                               p->jitems, p->j2, p->t_j2, w);
+        else
+            hipLaunchKernelGGL(gj_pass2<double>, dim3(persistent_grid(p->t_j2, 4)), dim3(jcore::J_THREADS), 0, st,
+                               p->jitems, p->j2, p->t_j2, w);
         MDF_LAUNCHED("gj_pass2");
         hipLaunchKernelGGL(gj_reduce2, dim3(persistent_grid(p->t_jr2, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
                            p->jr2, p->t_jr2, w);

@@ -11,18 +11,30 @@ namespace mdf {
 namespace jcore {
 
 constexpr int J_THREADS = 256;
-constexpr int J_U = 4;        // 16-byte loads per lane per row (pass 1)
+constexpr int J_LANE = 8;     // cderi values per lane per row (pass 1)
 constexpr int J_RB = 32;      // rows per pass-1 tile
 
-template <int VEC>
+// The cderi is stored in S = double or float (the FP32 storage mode: every
+// value is widened to FP64 before arithmetic).  A lane loads VEC values at a
+// time (16 bytes when VEC * sizeof(S) == 16) and J_U = J_LANE / VEC loads per
+// row, so a chunk is 4 * 64 * J_LANE = 2048 columns in either type.
+template <typename S, int VEC>
 struct vec_t;
 template <>
-struct vec_t<1> { using T = double; };
+struct vec_t<double, 1> { using T = double; };
 template <>
-struct vec_t<2> { using T = double2; };
+struct vec_t<double, 2> { using T = double2; };
+template <>
+struct vec_t<float, 1> { using T = float; };
+template <>
+struct vec_t<float, 4> { using T = float4; };
 
-template <int VEC>
-constexpr long chunk_cols() { return 4L * 64 * VEC * J_U; }
+template <typename S, int VEC>
+__device__ __forceinline__ double lane_elem(const typename vec_t<S, VEC>::T &x, int e) {
+    return (double)reinterpret_cast<const S *>(&x)[e];
+}
+
+constexpr long chunk_cols() { return 4L * 64 * J_LANE; }
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -34,12 +46,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 // chunk is held in registers and reused for J_RB rows; each row's partial dot
 // product is reduced across the wavefront and the four wavefronts.
 // part[(chunk * ndm + n) * naux + r]; ndm <= NDM densities are live.
-template <int NDM, int VEC>
-__device__ __forceinline__ void rho_partial_tile(const double *__restrict__ L, long ld, long naux, long npairs,
+template <int NDM, int VEC, typename S = double>
+__device__ __forceinline__ void rho_partial_tile(const S *__restrict__ L, long ld, long naux, long npairs,
                                         const double *__restrict__ d, long ldd, int ndm,
                                         double *__restrict__ part, long chunk, long rblock) {
+    constexpr int J_U = J_LANE / VEC;
     constexpr int WCOLS = 64 * VEC * J_U;
     constexpr int GCOLS = 4 * WCOLS;
+    using V = typename vec_t<S, VEC>::T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long cbase = chunk * GCOLS + wave * WCOLS + lane * VEC;
     const bool full = (chunk + 1) * GCOLS <= npairs;
@@ -60,24 +74,27 @@ __device__ __forceinline__ void rho_partial_tile(const double *__restrict__ L, l
     const int nr = (int)min((long)J_RB, naux - r0);
 
     for (int rr = 0; rr < nr; rr++) {
-        const double *row = L + (r0 + rr) * ld;
+        const S *row = L + (r0 + rr) * ld;
         double acc[NDM];
 #pragma unroll
         for (int n = 0; n < NDM; n++) acc[n] = 0.;
         if (full) {
-            typename vec_t<VEC>::T x[J_U];
+            V x[J_U];
 #pragma unroll
-            for (int u = 0; u < J_U; u++)
-                x[u] = *reinterpret_cast<const typename vec_t<VEC>::T *>(row + cbase + u * 64 * VEC);
+            for (int u = 0; u < J_U; u++) x[u] = *reinterpret_cast<const V *>(row + cbase + u * 64 * VEC);
 #pragma unroll
             for (int u = 0; u < J_U; u++) {
-                if constexpr (VEC == 2) {
+                if constexpr (VEC == 2 && sizeof(S) == 8) {
 #pragma unroll
                     for (int n = 0; n < NDM; n++)
                         acc[n] += x[u].x * dv[n][2 * u] + x[u].y * dv[n][2 * u + 1];
                 } else {
 #pragma unroll
-                    for (int n = 0; n < NDM; n++) acc[n] += x[u] * dv[n][u];
+                    for (int v = 0; v < VEC; v++) {
+                        const double xv = lane_elem<S, VEC>(x[u], v);
+#pragma unroll
+                        for (int n = 0; n < NDM; n++) acc[n] += xv * dv[n][u * VEC + v];
+                    }
                 }
             }
         } else {
@@ -86,7 +103,7 @@ __device__ __forceinline__ void rho_partial_tile(const double *__restrict__ L, l
 #pragma unroll
                 for (int v = 0; v < VEC; v++) {
                     const long c = cbase + u * 64 * VEC + v;
-                    const double xv = c < npairs ? row[c] : 0.;
+                    const double xv = c < npairs ? (double)row[c] : 0.;
 #pragma unroll
                     for (int n = 0; n < NDM; n++) acc[n] += xv * dv[n][u * VEC + v];
                 }
@@ -120,10 +137,11 @@ __device__ __forceinline__ void rho_reduce_range(const double *__restrict__ part
 
 // Pass-2 tile (cblock, rows [r0, r1)): a thread owns VEC consecutive columns.
 // out[n * ldo + c] for n < ndm.
-template <int NDM, int VEC>
-__device__ __forceinline__ void vj_pass_tile(const double *__restrict__ L, long ld, long npairs,
+template <int NDM, int VEC, typename S = double>
+__device__ __forceinline__ void vj_pass_tile(const S *__restrict__ L, long ld, long npairs,
                                     const double *__restrict__ rho, long ldrho, int ndm, long r0, long r1,
                                     double *__restrict__ out, long ldo, long cblock) {
+    using V = typename vec_t<S, VEC>::T;
     const long c = (cblock * J_THREADS + threadIdx.x) * VEC;
     double acc[NDM][VEC];
 #pragma unroll
@@ -131,39 +149,30 @@ __device__ __forceinline__ void vj_pass_tile(const double *__restrict__ L, long 
 #pragma unroll
         for (int v = 0; v < VEC; v++) acc[n][v] = 0.;
     if (c + VEC <= npairs) {
-        const double *p = L + r0 * ld + c;
+        const S *p = L + r0 * ld + c;
         long r = r0;
         for (; r + 8 <= r1; r += 8) {
-            typename vec_t<VEC>::T x[8];
+            V x[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++)
-                x[k] = *reinterpret_cast<const typename vec_t<VEC>::T *>(p + k * ld);
+            for (int k = 0; k < 8; k++) x[k] = *reinterpret_cast<const V *>(p + k * ld);
             p += 8 * ld;
 #pragma unroll
             for (int k = 0; k < 8; k++)
 #pragma unroll
                 for (int n = 0; n < NDM; n++) {
                     const double w = n < ndm ? rho[n * ldrho + r + k] : 0.;
-                    if constexpr (VEC == 2) {
-                        acc[n][0] += w * x[k].x;
-                        acc[n][1] += w * x[k].y;
-                    } else {
-                        acc[n][0] += w * x[k];
-                    }
+#pragma unroll
+                    for (int v = 0; v < VEC; v++) acc[n][v] += w * lane_elem<S, VEC>(x[k], v);
                 }
         }
         for (; r < r1; r++) {
-            const typename vec_t<VEC>::T x = *reinterpret_cast<const typename vec_t<VEC>::T *>(p);
+            const V x = *reinterpret_cast<const V *>(p);
             p += ld;
 #pragma unroll
             for (int n = 0; n < NDM; n++) {
                 const double w = n < ndm ? rho[n * ldrho + r] : 0.;
-                if constexpr (VEC == 2) {
-                    acc[n][0] += w * x.x;
-                    acc[n][1] += w * x.y;
-                } else {
-                    acc[n][0] += w * x;
-                }
+#pragma unroll
+                for (int v = 0; v < VEC; v++) acc[n][v] += w * lane_elem<S, VEC>(x, v);
             }
         }
     } else if (c < npairs) {
@@ -171,7 +180,7 @@ __device__ __forceinline__ void vj_pass_tile(const double *__restrict__ L, long 
 #pragma unroll
             for (int v = 0; v < VEC; v++)
                 if (c + v < npairs) {
-                    const double x = L[r * ld + c + v];
+                    const double x = (double)L[r * ld + c + v];
 #pragma unroll
                     for (int n = 0; n < NDM; n++)
                         if (n < ndm) acc[n][v] += rho[n * ldrho + r] * x;
@@ -207,9 +216,10 @@ struct JPlan {
     size_t part1_words, part2_words;
 };
 
+// vec: cderi values per thread in pass 2 (1, 2 for double; 1, 4 for float)
 inline JPlan plan_j(long naux, long npairs, int ndm, int vec) {
     JPlan p;
-    const long gc = vec == 2 ? chunk_cols<2>() : chunk_cols<1>();
+    const long gc = chunk_cols();
     p.nchunk = (npairs + gc - 1) / gc;
     p.nrblock = (naux + J_RB - 1) / J_RB;
     p.ncblock = (npairs + (long)J_THREADS * vec - 1) / ((long)J_THREADS * vec);

@@ -33,9 +33,9 @@ namespace {
 // 1-D grid of round_xcd(nkt * nit * nL) workgroups in XCD order; tile
 // (k-tile fastest, then i-tile, then aux row) so the k-tiles of one A tile
 // share an L2.
-template <int BN, bool MAPPED, typename T>
+template <int BN, bool MAPPED, typename T, typename Src>
 __global__ __launch_bounds__(G_THREADS) void k_step1(
-    const double *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao,
+    const Src *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao,
     const double *__restrict__ F, long ldf, int nocc, T *__restrict__ R, long ldr,
     long strideL, int nkt, int nit, long ntiles) {
     __shared__ T lds[TileShape<128, BN>::LDS_ELEMS];
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(G_THREADS) void k_step1(
     if (x >= ntiles) return;
     const int kt = (int)(x % nkt), it = (int)((x / nkt) % nit);
     const long L = x / ((long)nkt * nit);
-    step1_tile<BN, MAPPED, T>(cderi + L * ld, pmap, nao, F, ldf, nocc, R + L * strideL, ldr, it * 128, kt * BN,
+    step1_tile<BN, MAPPED, T, Src>(cderi + L * ld, pmap, nao, F, ldf, nocc, R + L * strideL, ldr, it * 128, kt * BN,
                               lds);
 }
 
@@ -82,9 +82,9 @@ __global__ __launch_bounds__(256) void k_split_ft(const double *__restrict__ F, 
     kb3::split_ft_range(F, ldf, nao, nocc, hi, lo, ldft, e0, min(e0 + 1024, (long)nocc * ldft));
 }
 
-template <bool MAPPED, int BN>
+template <bool MAPPED, int BN, typename Src>
 __global__ __launch_bounds__(kb3::Step1Cfg<BN>::THREADS) void k_step1_b3(
-    const double *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao, const b3::u16 *fthi,
+    const Src *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao, const b3::u16 *fthi,
     const b3::u16 *ftlo, long ldft, int nocc, b3::u16 *Rhi, b3::u16 *Rlo, long ldr, long strideL, int nkt,
     int nit, long ntiles) {
     __shared__ b3::u16 lds[kb3::Step1Cfg<BN>::S::LDS_ELEMS];
@@ -92,22 +92,22 @@ __global__ __launch_bounds__(kb3::Step1Cfg<BN>::THREADS) void k_step1_b3(
     if (x >= ntiles) return;
     const int kt = (int)(x % nkt), it = (int)((x / nkt) % nit);
     const long z = x / ((long)nkt * nit);
-    kb3::step1_tile<MAPPED, BN>(cderi + z * ld, pmap, nao, fthi, ftlo, ldft, nocc, Rhi + z * strideL,
+    kb3::step1_tile<MAPPED, BN, Src>(cderi + z * ld, pmap, nao, fthi, ftlo, ldft, nocc, Rhi + z * strideL,
                                 Rlo + z * strideL, ldr, it * kb3::BM, kt * BN, lds);
 }
 
-template <int BN>
-void launch_step1_b3(int nL, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
+template <int BN, typename Src>
+void launch_step1_b3(int nL, hipStream_t st, const Src *blk, long ld, const int *pmap, int nao,
                      const b3::u16 *fthi, const b3::u16 *ftlo, long ldft, int nocc, b3::u16 *rhi, b3::u16 *rlo,
                      long ldr, long strideL) {
     const int nkt = (int)cdiv(ldr, (long)BN), nit = (int)cdiv(nao, kb3::BM);
     const long ntiles = (long)nkt * nit * nL;
     const dim3 g((unsigned)round_xcd(ntiles)), b(kb3::Step1Cfg<BN>::THREADS);
     if (pmap)
-        hipLaunchKernelGGL((k_step1_b3<true, BN>), g, b, 0, st, blk, ld, pmap, nao, fthi, ftlo, ldft, nocc, rhi,
+        hipLaunchKernelGGL((k_step1_b3<true, BN, Src>), g, b, 0, st, blk, ld, pmap, nao, fthi, ftlo, ldft, nocc, rhi,
                            rlo, ldr, strideL, nkt, nit, ntiles);
     else
-        hipLaunchKernelGGL((k_step1_b3<false, BN>), g, b, 0, st, blk, ld, pmap, nao, fthi, ftlo, ldft, nocc, rhi,
+        hipLaunchKernelGGL((k_step1_b3<false, BN, Src>), g, b, 0, st, blk, ld, pmap, nao, fthi, ftlo, ldft, nocc, rhi,
                            rlo, ldr, strideL, nkt, nit, ntiles);
 }
 
@@ -129,27 +129,27 @@ __global__ __launch_bounds__(G_THREADS, 2) void k_step2_b3(
                     part + (long)blockIdx.y * part_stride, beta, lds);
 }
 
-template <int BN, typename T>
-void launch_step1(int nL, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
+template <int BN, typename T, typename Src>
+void launch_step1(int nL, hipStream_t st, const Src *blk, long ld, const int *pmap, int nao,
                   const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
     const int nkt = (int)cdiv(nocc, BN), nit = (int)cdiv(nao, 128);
     const long ntiles = (long)nkt * nit * nL;
     const dim3 g((unsigned)round_xcd(ntiles));
     if (pmap)
-        hipLaunchKernelGGL((k_step1<BN, true, T>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
+        hipLaunchKernelGGL((k_step1<BN, true, T, Src>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
                            nocc, r, ldr, strideL, nkt, nit, ntiles);
     else
-        hipLaunchKernelGGL((k_step1<BN, false, T>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
+        hipLaunchKernelGGL((k_step1<BN, false, T, Src>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
                            nocc, r, ldr, strideL, nkt, nit, ntiles);
 }
 
-template <typename T>
-int step1(const KPlan &p, int nL, hipStream_t st, const double *blk, long ld, const int *pmap, int nao,
+template <typename T, typename Src>
+int step1(const KPlan &p, int nL, hipStream_t st, const Src *blk, long ld, const int *pmap, int nao,
           const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
     switch (p.bn) {
-    case 160: launch_step1<160, T>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    case 128: launch_step1<128, T>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    default: launch_step1<64, T>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 160: launch_step1<160, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 128: launch_step1<128, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    default: launch_step1<64, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     }
     MDF_LAUNCHED("k_step1");
     return MDF_OK;
@@ -177,8 +177,8 @@ int step2(const KPlan &p, int nL, bool two, hipStream_t st, const T *rl, const T
 }
 
 // The block loop of mdf_get_k for element type T.
-template <typename T>
-int run_k(const KPlan &p, const double *cderi, long naux, long ld, const int *pair_map, int nao,
+template <typename T, typename Src>
+int run_k(const KPlan &p, const Src *cderi, long naux, long ld, const int *pair_map, int nao,
           const double *fl, const double *fr, int nocc, long ldf, double *vk, long ldvk, int accumulate,
           void *work, hipStream_t st) {
     const bool two = fr != nullptr;
@@ -189,11 +189,11 @@ int run_k(const KPlan &p, const double *cderi, long naux, long ld, const int *pa
     const long part_stride = (long)nao * nao;
     for (long L0 = 0; L0 < naux; L0 += p.nL) {
         const int nL = (int)(naux - L0 < p.nL ? naux - L0 : p.nL);
-        const double *blk = cderi + L0 * ld;
-        int rc = step1<T>(p, nL, st, blk, ld, pair_map, nao, fl, ldf, nocc, rl, ldr, strideL);
+        const Src *blk = cderi + L0 * ld;
+        int rc = step1<T, Src>(p, nL, st, blk, ld, pair_map, nao, fl, ldf, nocc, rl, ldr, strideL);
         if (rc) return rc;
         if (two) {
-            rc = step1<T>(p, nL, st, blk, ld, pair_map, nao, fr, ldf, nocc, rr, ldr, strideL);
+            rc = step1<T, Src>(p, nL, st, blk, ld, pair_map, nao, fr, ldf, nocc, rr, ldr, strideL);
             if (rc) return rc;
         }
         rc = step2<T>(p, nL, two, st, rl, rr, nao, nocc, ldr, strideL, part, part_stride, L0 > 0);
@@ -207,7 +207,8 @@ int run_k(const KPlan &p, const double *cderi, long naux, long ld, const int *pa
 }
 
 // The block loop of mdf_get_k on the split-BF16 engine.
-int run_k_b3(const KPlan &p, const double *cderi, long naux, long ld, const int *pair_map, int nao,
+template <typename Src>
+int run_k_b3(const KPlan &p, const Src *cderi, long naux, long ld, const int *pair_map, int nao,
              const double *fl, const double *fr, int nocc, long ldf, double *vk, long ldvk, int accumulate,
              void *work, hipStream_t st) {
     const bool two = fr != nullptr;
@@ -223,7 +224,7 @@ int run_k_b3(const KPlan &p, const double *cderi, long naux, long ld, const int 
     }
     for (long L0 = 0; L0 < naux; L0 += p.nL) {
         const int nL = (int)(naux - L0 < p.nL ? naux - L0 : p.nL);
-        const double *blk = cderi + L0 * ld;
+        const Src *blk = cderi + L0 * ld;
         const int bn = kb3::pick_bn(y.ldr);
         for (int side = 0; side < (two ? 2 : 1); side++) {
             const kb3::Side &q = sd[side];
@@ -284,7 +285,7 @@ extern "C" size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_fact
     return k_bytes(naux, nao, nocc, two_factors != 0, prec, rows);
 }
 
-extern "C" int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *pair_map,
+extern "C" int mdf_get_k(const void *cderi, int cderi_dtype, long naux, long npairs, long ld, const int *pair_map,
                          int nao, const double *fl, const double *fr, int nocc, long ldf, int prec,
                          double *vk, long ldvk, int accumulate, void *work, size_t work_bytes,
                          void *stream) {
@@ -293,6 +294,8 @@ extern "C" int mdf_get_k(const double *cderi, long naux, long npairs, long ld, c
     MDF_CHECK_ARG(ld >= npairs && ldf >= nocc && ldvk >= nao, "mdf_get_k: leading dimension too small");
     MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32 || prec == MDF_PREC_BF16X3,
                   "mdf_get_k: unknown precision %d", prec);
+    MDF_CHECK_ARG(cderi_dtype == MDF_DTYPE_F64 || cderi_dtype == MDF_DTYPE_F32, "mdf_get_k: cderi dtype %d",
+                  cderi_dtype);
     MDF_CHECK_ARG(pair_map || npairs == (long)nao * (nao + 1) / 2,
                   "mdf_get_k: without a pair map the cderi must hold the full packed triangle "
                   "(npairs=%ld, nao=%d)", npairs, nao);
@@ -322,9 +325,18 @@ extern "C" int mdf_get_k(const double *cderi, long naux, long npairs, long ld, c
         rows = lo;
     }
     KPlan p = plan_k(naux, nao, nocc, two, rows);
+    if (cderi_dtype == MDF_DTYPE_F32) {
+        const float *c = (const float *)cderi;
+        if (prec == MDF_PREC_BF16X3)
+            return run_k_b3(p, c, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
+        if (prec == MDF_PREC_FP32)
+            return run_k<float>(p, c, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
+        return run_k<double>(p, c, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
+    }
+    const double *c = (const double *)cderi;
     if (prec == MDF_PREC_BF16X3)
-        return run_k_b3(p, cderi, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
+        return run_k_b3(p, c, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
     if (prec == MDF_PREC_FP32)
-        return run_k<float>(p, cderi, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
-    return run_k<double>(p, cderi, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
+        return run_k<float>(p, c, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
+    return run_k<double>(p, c, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
 }

@@ -20,39 +20,39 @@ __device__ __forceinline__ long tri(long i) { return i * (i + 1) / 2; }
 // ---------------------------------------------------------------- step 1
 // A operand of step 1: rows i0..i0+127 of the unpacked A_L, K-step kt covers
 // j = 16 kt .. 16 kt + 15.
-template <bool MAPPED, typename T = double>
+template <bool MAPPED, typename T = double, typename Src = double>
 struct PackedALoader {
-    using reg_t = double;
+    using reg_t = Src;                                   // the cderi storage type
     static constexpr int N = 128 * G_BK / G_THREADS;     // 8
     static constexpr int TPJ = G_THREADS / G_BK;         // threads per j row (upper)
-    const double *row;   // cderi row L
+    const Src *row;      // cderi row L
     const int *pmap;     // MAPPED: pair index of (i, j), -1 if not stored
     int nao, i0;
     int upper;           // layout used by the last load()
-    __device__ __forceinline__ void load(int kt, double (&r)[N]) {
+    __device__ __forceinline__ void load(int kt, Src (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * G_BK;
         if (!MAPPED && j0 >= i0 + 127) {
             // every j > every i: (i, j) at tri(j) + i, contiguous in i
             upper = 1;
             const int j = j0 + t / TPJ, ib = i0 + (t % TPJ) * N;
-            const double *src = row + tri(j) + ib;
+            const Src *src = row + tri(j) + ib;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? src[v] : 0.;
+            for (int v = 0; v < N; v++) r[v] = (j < nao && ib + v < nao) ? src[v] : (Src)0;
             return;
         }
         upper = 0;
         const int i = i0 + (t >> 1), jb = j0 + (t & 1) * N;
         if (!MAPPED && i0 >= j0 + G_BK - 1) {
             // every i >= every j: contiguous in j
-            const double *src = row + tri(i) + jb;
+            const Src *src = row + tri(i) + jb;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (i < nao) ? src[v] : 0.;
+            for (int v = 0; v < N; v++) r[v] = (i < nao) ? src[v] : (Src)0;
             return;
         }
 #pragma unroll
         for (int v = 0; v < N; v++) {
             const int j = jb + v;
-            double x = 0.;
+            Src x = 0;
             if (i < nao && j < nao) {
                 if (MAPPED) {
                     const int p = pmap[(long)i * nao + j];
@@ -64,7 +64,7 @@ struct PackedALoader {
             r[v] = x;
         }
     }
-    __device__ __forceinline__ void store(const double (&r)[N], T *lds, int pitch) const {
+    __device__ __forceinline__ void store(const Src (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;
         if (upper) {
             const int jj = t / TPJ, ib = (t % TPJ) * N;
@@ -80,12 +80,12 @@ struct PackedALoader {
 
 // One 128 x BN tile of step 1 for aux row `row` (the cderi row L): rows
 // i0.., columns (occupied index) k0.. of R_L.
-template <int BN, bool MAPPED, typename T>
-__device__ __forceinline__ void step1_tile(const double *__restrict__ row, const int *__restrict__ pmap, int nao,
+template <int BN, bool MAPPED, typename T, typename Src = double>
+__device__ __forceinline__ void step1_tile(const Src *__restrict__ row, const int *__restrict__ pmap, int nao,
                                   const double *__restrict__ F, long ldf, int nocc, T *__restrict__ Rl, long ldr,
                                   int i0, int k0, T *lds) {
     using S = TileShape<128, BN>;
-    PackedALoader<MAPPED, T> la{row, pmap, nao, i0, 0};
+    PackedALoader<MAPPED, T, Src> la{row, pmap, nao, i0, 0};
     ColsLoader<BN, T, double> lb{F, ldf, k0, nocc, 0, nao};
     Acc<S::MI, S::NI, T> acc;
     acc.zero();
@@ -316,16 +316,16 @@ __host__ __device__ inline long pad32(long n) { return (n + 31) / 32 * 32; }
 
 // A operand: rows i0..i0+127 of the unpacked A_L; K-step kt covers
 // j = 32 kt .. 32 kt + 31.  THREADS threads, N = 4096 / THREADS values each.
-template <bool MAPPED, int THREADS>
+template <bool MAPPED, int THREADS, typename Src = double>
 struct PackedA32 {
-    using reg_t = double;
+    using reg_t = Src;
     static constexpr int N = 128 * b3::BK / THREADS;
     static constexpr int TPR = b3::BK / N;           // threads per row (lower layout)
-    const double *row;
+    const Src *row;
     const int *pmap;
     int nao, i0;
     int upper;
-    __device__ __forceinline__ void load(int kt, double (&r)[N]) {
+    __device__ __forceinline__ void load(int kt, Src (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * b3::BK;
         if (!MAPPED && j0 >= i0 + 127) {
             // every j > every i: (i, j) at tri(j) + i, contiguous in i.  Thread
@@ -335,22 +335,22 @@ struct PackedA32 {
             upper = 1;
             const int i = i0 + (t & 127), jb = j0 + (t >> 7) * N;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (i < nao && jb + v < nao) ? row[tri(jb + v) + i] : 0.;
+            for (int v = 0; v < N; v++) r[v] = (i < nao && jb + v < nao) ? row[tri(jb + v) + i] : (Src)0;
             return;
         }
         upper = 0;
         const int i = i0 + t / TPR, jb = j0 + (t % TPR) * N;
         if (!MAPPED && i0 >= j0 + b3::BK - 1) {
             // every i >= every j: contiguous in j
-            const double *src = row + tri(i) + jb;
+            const Src *src = row + tri(i) + jb;
 #pragma unroll
-            for (int v = 0; v < N; v++) r[v] = (i < nao) ? src[v] : 0.;
+            for (int v = 0; v < N; v++) r[v] = (i < nao) ? src[v] : (Src)0;
             return;
         }
 #pragma unroll
         for (int v = 0; v < N; v++) {
             const int j = jb + v;
-            double x = 0.;
+            Src x = 0;
             if (i < nao && j < nao) {
                 if (MAPPED) {
                     const int p = pmap[(long)i * nao + j];
@@ -362,7 +362,7 @@ struct PackedA32 {
             r[v] = x;
         }
     }
-    __device__ __forceinline__ void store(const double (&r)[N], u16 *hi, u16 *lo) const {
+    __device__ __forceinline__ void store(const Src (&r)[N], u16 *hi, u16 *lo) const {
         const int t = threadIdx.x;
         if (upper)
             b3::put<N>(r, hi, lo, t & 127, (t >> 7) * N);
@@ -411,12 +411,12 @@ inline int pick_bn(long ldr) {
 
 // One 128 x BN tile of step 1 for the cderi row `row`: rows i0.., columns
 // k0.. of the split R planes (row stride ldr; columns nocc..ldr-1 get zeros).
-template <bool MAPPED, int BN>
-__device__ __forceinline__ void step1_tile(const double *__restrict__ row, const int *__restrict__ pmap, int nao,
+template <bool MAPPED, int BN, typename Src = double>
+__device__ __forceinline__ void step1_tile(const Src *__restrict__ row, const int *__restrict__ pmap, int nao,
                                            const u16 *fthi, const u16 *ftlo, long ldft, int nocc, u16 *Rhi,
                                            u16 *Rlo, long ldr, int i0, int k0, u16 *lds) {
     using C = Step1Cfg<BN>;
-    PackedA32<MAPPED, C::THREADS> la{row, pmap, nao, i0, 0};
+    PackedA32<MAPPED, C::THREADS, Src> la{row, pmap, nao, i0, 0};
     FTLoader<BN, C::THREADS> lb{fthi, ftlo, ldft, k0, nocc};
     b3::Acc32<C::S::MI, C::S::NI> acc;
     acc.zero();

@@ -100,6 +100,27 @@ extern "C" int mdf_unpack_pairs(const double *vj_pairs, int n_dm, long ldv, cons
     return MDF_OK;
 }
 
+__global__ __launch_bounds__(256) void cast_f32_kernel(const double *__restrict__ src, long lds, long cols,
+                                                       float *__restrict__ dst, long ldd, long total) {
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const long r = e / cols, c = e % cols;
+        dst[r * ldd + c] = (float)src[r * lds + c];
+    }
+}
+
+extern "C" int mdf_cast_f32(const double *src, long lds, long rows, long cols, float *dst, long ldd,
+                            void *stream) {
+    MDF_CHECK_ARG(rows >= 0 && cols >= 0 && lds >= cols && ldd >= cols, "mdf_cast_f32: bad shape");
+    MDF_CHECK_ARG((src && dst) || rows * cols == 0, "mdf_cast_f32: null pointer");
+    const long total = rows * cols;
+    if (total == 0) return MDF_OK;
+    const long blocks = mdf::cdiv(total, 256L);
+    hipLaunchKernelGGL(cast_f32_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
+                       mdf::as_stream(stream), src, lds, cols, dst, ldd, total);
+    MDF_LAUNCHED("cast_f32_kernel");
+    return MDF_OK;
+}
+
 extern "C" int mdf_fill_zero(double *x, long n, void *stream) {
     MDF_CHECK_ARG(x || n == 0, "mdf_fill_zero: null pointer");
     if (n == 0) return MDF_OK;

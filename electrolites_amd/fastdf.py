@@ -21,9 +21,11 @@ driver lowers it for early cycles.
 when :func:`electrolites_amd.patch` has installed this function.
 
 Every case this module does not cover (a complex density, a range-separated
-``omega``, more than one device, a non-float64 tensor) is *declined*: counted
-in ``stats()`` and handed to the function this one replaced, or, when there is
-none (standalone use), raised as :class:`Declined`.
+``omega``, more than one device, a cderi neither float64 nor float32) is
+*declined*: counted in ``stats()`` and handed to the function this one
+replaced, or, when there is none (standalone use), raised as
+:class:`Declined`.  A float32 cderi (the FP32 storage mode) is covered: its
+values are widened to FP64 before arithmetic.
 """
 import os
 
@@ -205,7 +207,7 @@ def _vj(cderi, lay, dms3, hermi):
     vjp = t.empty((n, npairs), dtype=t.float64, device=dms3.device)
     nbytes = _lib.query('mdf_get_j_workspace', naux, npairs, n)
     work = _dev.WORKSPACE.get(nbytes)
-    _lib.call('mdf_get_j', _dev.ptr(cderi), naux, npairs, cderi.stride(0),
+    _lib.call('mdf_get_j', _dev.ptr(cderi), _dtype_code(cderi), naux, npairs, cderi.stride(0),
               _dev.ptr(d), n, npairs, _dev.ptr(rho), naux, _dev.ptr(vjp),
               npairs, 0, _dev.ptr(work), work.numel(), _dev.stream_ptr())
     vj = t.empty((n, nao, nao), dtype=t.float64, device=dms3.device)
@@ -215,6 +217,18 @@ def _vj(cderi, lay, dms3, hermi):
 
 
 PRECISIONS = {'fp64': 0, 'fp32': 1, 'bf16x3': 2}
+
+
+def _dtype_code(cderi):
+    """Storage type of a cderi for the library: FP64, or FP32 storage whose
+    values are widened to FP64 before arithmetic (the reference's
+    FASTDF_FP32 mode, electrolites/_dffp32.py:166-196)."""
+    t = _dev.torch()
+    if cderi.dtype == t.float64:
+        return _lib.DTYPE_F64
+    if cderi.dtype == t.float32:
+        return _lib.DTYPE_F32
+    raise ValueError(f'cderi dtype {cderi.dtype}')
 
 
 def _vk(cderi, lay, fl, fr, precision='fp64'):
@@ -236,7 +250,7 @@ def _vk(cderi, lay, fl, fr, precision='fp64'):
     work = _dev.WORKSPACE.get(nbytes)
     for i in range(n):
         STATS['k_calls'] += 1
-        _lib.call('mdf_get_k', _dev.ptr(cderi), naux, npairs, cderi.stride(0),
+        _lib.call('mdf_get_k', _dev.ptr(cderi), _dtype_code(cderi), naux, npairs, cderi.stride(0),
                   _dev.ptr(lay.map), nao, _dev.ptr(fl[i]),
                   _dev.ptr(fr[i]) if two else None, nocc, nocc, prec,
                   _dev.ptr(vk[i]), nao, 0, _dev.ptr(work), nbytes,
@@ -280,9 +294,11 @@ def _get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
     if getattr(dfobj, '_cderi', None) is None:
         dfobj.build()
     cderi = _cderi_of(dfobj)
-    if cderi.dtype != t.float64:
+    if cderi.dtype not in (t.float64, t.float32):
         return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
-                             'non-float64 cderi')
+                             'cderi neither float64 nor float32')
+    if cderi.dtype == t.float32:
+        STATS['fp32_store'] = STATS.get('fp32_store', 0) + 1
     if cderi.shape[0] == 0:
         return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
                              'empty cderi slice')

@@ -17,6 +17,12 @@ lower triangle (``p = i(i+1)/2 + j``), with ``cderi_idx = (pair_addr = i*nao + j
 (int32), diag_idx)`` as the reference returns (:446-449, :693).  Shell pairs
 whose primitive overlap bound is below ``ints.PAIR_SCREEN`` are not computed
 and hold zeros.
+
+FP32 storage (the reference's FASTDF_FP32 mode, electrolites/_dffp32.py):
+with ``FASTDF_FP32=1`` in the environment (read at import, as there) or
+``dtype='float32'``, each chunk is computed in FP64 and stored rounded to
+FP32 (``mdf_cast_f32``); J and K widen the values back to FP64 before
+arithmetic.  The tensor takes half the HBM and J half the bytes.
 """
 import os
 
@@ -31,6 +37,8 @@ STATS = {'builds': 0, 'eigen_fallback': 0, 'chunks': 0, 'pairs_screened': 0,
 LINEAR_DEP_THR = 1e-9
 #: device bytes for one chunk of three-centre integrals
 CHUNK_BYTES = int(float(os.environ.get('FASTDFCDERI_CHUNK', 4 << 30)))
+#: store new cderi tensors in single precision
+FP32 = os.environ.get('FASTDF_FP32', '') not in ('', '0')
 
 
 def stats():
@@ -123,30 +131,35 @@ def _chunks(shells, naux, budget):
     return out
 
 
-def cholesky_eri(mol, chunk_bytes=None, timer=None):
+def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None):
     """Build the cderi of ``mol`` (orbital basis ``mol.ao``, fitting basis
     ``mol.aux``) on the current device.  Returns ``(cderi, (pair_addr,
     diag_idx))``.  ``timer`` (a :class:`_Timer`) collects per-stage device
-    times."""
+    times.  ``dtype`` 'float64' or 'float32' (FP32 storage; default from
+    ``FASTDF_FP32``)."""
     _dev.require_gpu()
     t = _dev.torch()
     tm = timer or _NoTimer()
+    fp32 = FP32 if dtype is None else str(dtype).endswith('float32')
     ao, aux = _basis(mol, 'ao'), _basis(mol, 'aux')
     nao, naux = ao.nao, aux.nao
     npairs = nao * (nao + 1) // 2
     linv, coef = decompose_j2c(aux, timer)
     nrow = naux if coef is None else coef.shape[0]
-    cderi = t.empty((nrow, npairs), dtype=t.float64, device=_dev.device())
+    cderi = t.empty((nrow, npairs), dtype=t.float32 if fp32 else t.float64, device=_dev.device())
     bound = getattr(mol, '_mdf_pair_bound', None)
     if bound is None:
         bound = ints.pair_bound(mol.ao)
         mol._mdf_pair_bound = bound
     budget = chunk_bytes or CHUNK_BYTES
+    if fp32:
+        budget //= 2                       # an FP64 output chunk beside the integrals
     chunks = _chunks(mol.ao, naux, budget)
     widest = max(int(mol.ao.ao_loc[I1]) * (int(mol.ao.ao_loc[I1]) + 1) // 2 -
                  int(mol.ao.ao_loc[I0]) * (int(mol.ao.ao_loc[I0]) + 1) // 2
                  for I0, I1 in chunks)
     buf = t.empty(naux * widest, dtype=t.float64, device=_dev.device())
+    out64 = t.empty(nrow * widest, dtype=t.float64, device=_dev.device()) if fp32 else None
     for I0, I1 in chunks:
         tasks = ints.PairTasks(mol.ao, I0, I1, bound)
         STATS['pairs_kept'] += tasks.npairs_kept
@@ -158,14 +171,19 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None):
         ints.int3c2e_chunk(ao, aux, tasks, blk, width)
         tm.mark('int3c2e:1')
         dst = cderi[:, tasks.p0:]
+        # FP64 result straight into the tensor, or into a chunk then rounded
+        out, ldo = (out64, width) if fp32 else (dst, npairs)
         tm.mark('trmm:0')
         if coef is None:
             _lib.call('mdf_trmm', _dev.ptr(linv), naux, naux, _dev.ptr(blk), width, width,
-                      _dev.ptr(dst), npairs, _dev.stream_ptr())
+                      _dev.ptr(out), ldo, _dev.stream_ptr())
         else:
             _lib.call('mdf_gemm', 0, 0, 0, nrow, width, naux, 1., _dev.ptr(coef), naux,
-                      _dev.ptr(blk), width, 0., _dev.ptr(dst), npairs, _dev.stream_ptr())
+                      _dev.ptr(blk), width, 0., _dev.ptr(out), ldo, _dev.stream_ptr())
         tm.mark('trmm:1')
+        if fp32:
+            _lib.call('mdf_cast_f32', _dev.ptr(out64), width, nrow, width, _dev.ptr(dst), npairs,
+                      _dev.stream_ptr())
         STATS['chunks'] += 1
     STATS['builds'] += 1
     rows, cols = np.tril_indices(nao)

@@ -25,12 +25,18 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 3
+#define MDF_ABI_VERSION 4
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
 #define MDF_PREC_FP32 1
 #define MDF_PREC_BF16X3 2
+
+/* storage type of a cderi: FP64, or FP32 storage with every value widened
+ * to FP64 before arithmetic (the reference's FASTDF_FP32 mode,
+ * electrolites/_dffp32.py:166-196; its widened blocks, fastdf.py:611-671) */
+#define MDF_DTYPE_F64 0
+#define MDF_DTYPE_F32 1
 
 /* ---- library / device ------------------------------------------------- */
 int mdf_abi_version(void);
@@ -41,14 +47,14 @@ int mdf_device_info(int *cu_count, long long *hbm_bytes, char *arch, int arch_le
 /* ---- J: Coulomb over the packed cderi -----------------------------------
  * Replaces electrolites/fastdf.py:348-386 (_vj_pass: rho = L.d, vj = L^T.rho)
  * as used by _get_jk (:936-946) and the J-only branch _get_j (:688-745).
- *   cderi     (naux, npairs), row stride ld
+ *   cderi     (naux, npairs), row stride ld, MDF_DTYPE_F64 or _F32 values
  *   dm_pairs  (n_dm, npairs), row stride ldd: the density gathered onto the
  *             pairs with the symmetry factors of fastdf.py:674-686
  *   rho       (n_dm, naux) out, row stride ldrho (the fitted density)
  *   vj_pairs  (n_dm, npairs) out, row stride ldvj; accumulate != 0 adds.
  */
 size_t mdf_get_j_workspace(long naux, long npairs, int n_dm);
-int mdf_get_j(const double *cderi, long naux, long npairs, long ld,
+int mdf_get_j(const void *cderi, int cderi_dtype, long naux, long npairs, long ld,
               const double *dm_pairs, int n_dm, long ldd,
               double *rho, long ldrho, double *vj_pairs, long ldvj, int accumulate,
               void *work, size_t work_bytes, void *stream);
@@ -58,6 +64,7 @@ int mdf_get_j(const double *cderi, long naux, long npairs, long ld,
  * contract('Lij,njk->nikL'), step 2 _syrk / contract('nikL,njkL->nij')) and
  * _fill_from_triangle (:389-402), for one density:
  *   vk = sum_L (A_L fl)(A_L fr)^T,   fr == NULL: fr = fl (symmetric, mode 0)
+ *   cderi     MDF_DTYPE_F64 or _F32 values (FP32 storage is widened on load)
  *   pair_map  (nao*nao) int32 pair index of (i,j) or -1 when screened; NULL
  *             when the cderi holds the full packed triangle
  *   fl, fr    (nao, nocc) density factors, row stride ldf
@@ -76,7 +83,7 @@ int mdf_get_j(const double *cderi, long naux, long npairs, long ld,
  */
 size_t mdf_get_k_workspace(long naux, int nao, int nocc, int two_factors, int prec,
                            long max_block_rows);
-int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *pair_map,
+int mdf_get_k(const void *cderi, int cderi_dtype, long naux, long npairs, long ld, const int *pair_map,
               int nao, const double *fl, const double *fr, int nocc, long ldf, int prec,
               double *vk, long ldvk, int accumulate, void *work, size_t work_bytes,
               void *stream);
@@ -101,7 +108,7 @@ int mdf_get_k(const double *cderi, long naux, long npairs, long ld, const int *p
  * entries cut it (mdf_get_k given at least mdf_get_k_workspace(..., 0)
  * bytes), so its J/K are bit-identical alone and in a batch. */
 typedef struct {
-    const double *cderi;
+    const void *cderi;
     long ld;
     long naux;
     long npairs;
@@ -115,6 +122,7 @@ typedef struct {
     const double *fr;
     int nocc;
     double *vk;
+    int cderi_dtype;      /* MDF_DTYPE_F64 or _F32; one storage type per plan */
 } mdf_jk_item;
 
 typedef struct mdf_grouped_plan mdf_grouped_plan;
@@ -142,6 +150,10 @@ int mdf_unpack_pairs(const double *vj_pairs, int n_dm, long ldv, const int *pair
                      long npairs, int nao, double *out, long ldo, void *stream);
 int mdf_build_pair_map(const int *pair_addr, long npairs, int nao, int *map, void *stream);
 int mdf_fill_zero(double *x, long n, void *stream);
+/* dst[r * ldd + c] = (float)src[r * lds + c] (round to nearest even): the
+ * FP32 store of a cderi chunk, as the reference's FASTDF_FP32 build casts
+ * each FP64 block on its way into the tensor (electrolites/_dffp32.py:67-198). */
+int mdf_cast_f32(const double *src, long lds, long rows, long cols, float *dst, long ldd, void *stream);
 
 /* ---- Gaussian integrals (Rys quadrature, one kernel per angular class) --
  * A basis is a set of contracted shells; all arrays are DEVICE memory:

@@ -147,3 +147,21 @@ def test_passes_and_waves(precision, monkeypatch):
         for df, dm, (rj, rk) in zip(dfs, dms, ref):
             _, ok = O.get_jk(df._cderi[0].cpu().numpy(), df._cderi_idx[0], np.asarray(dm))
             assert np.abs(rk - ok).max() <= 1e-11 * np.abs(ok).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
+def test_grouped_fp32_storage(precision):
+    """A batch of FP32-stored cderis: every item equals its single-entry
+    get_jk bit for bit."""
+    import torch
+    from electrolites_amd import batch, fastdf
+    dfs, dms = make_batch(5)
+    for df in dfs:
+        df._cderi = {0: df._cderi[0].float()}
+    got = batch.batched_get_jk(dfs, dms, hermi=1, precision=precision)
+    for df, dm, (vj, vk) in zip(dfs, dms, got):
+        df.k_precision = precision
+        rj, rk = fastdf.get_jk(df, dm, hermi=1)
+        assert np.array_equal(vj.cpu().numpy(), rj)
+        assert np.array_equal(vk.cpu().numpy(), rk)

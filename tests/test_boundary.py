@@ -89,8 +89,9 @@ def test_standalone_decline_raises(monkeypatch):
 
 
 def test_narrow_tensor_never_falls_through(monkeypatch):
-    """fastdf.py:434-440: a float32 cderi must raise instead of reaching a
-    float64 reader."""
+    """fastdf.py:434-440: a declined call on a float32 cderi must raise
+    instead of reaching a float64 reader (covered calls run the FP32-storage
+    kernels)."""
     import torch
     from electrolites_amd import fastdf
 
@@ -105,7 +106,7 @@ def test_narrow_tensor_never_falls_through(monkeypatch):
 def test_library_error_message():
     from electrolites_amd import _lib
     with pytest.raises(_lib.MDFError, match='bad sizes'):
-        _lib.call('mdf_get_k', None, -1, 0, 0, None, 0, None, None, 0, 0, 0, None, 0, 0, None, 0, None)
+        _lib.call('mdf_get_k', None, 0, -1, 0, 0, None, 0, None, None, 0, 0, 0, None, 0, 0, None, 0, None)
 
 
 def test_tile_fill_dense_configs():

@@ -137,3 +137,18 @@ def test_jk_on_built_cderi(built):
     rj, rk = O.get_jk(ref, pa, np.asarray(dm))
     assert rel(vj, rj) < 1e-11
     assert rel(vk, rk) < 1e-11
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('chunk_bytes', [None, 200000])
+def test_fp32_storage_build(built, chunk_bytes):
+    """The FP32-storage build (FASTDF_FP32) stores exactly the FP64 build's
+    values rounded to nearest FP32, chunk by chunk."""
+    import torch
+    from electrolites_amd import fastdfcderi
+    mol, df, ref, pa = built
+    c64, _ = fastdfcderi.cholesky_eri(mol, chunk_bytes=chunk_bytes)
+    c32, idx = fastdfcderi.cholesky_eri(mol, chunk_bytes=chunk_bytes, dtype='float32')
+    assert c32.dtype == torch.float32 and c32.shape == c64.shape
+    assert torch.equal(c32, c64.float())
+    assert np.array_equal(idx[0], pa.astype(np.int32))

@@ -213,3 +213,36 @@ def test_low_precision_k_error(precision, nao, naux, nocc, mode, screened):
     err = rel(vk, rk)
     print(f'{precision} mode {mode} screened {screened}: K rel err {err:.2e}')
     assert 1e-12 < err < 1e-5, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('mode,screened,n_dm', [(0, False, 1), (1, False, 3), (0, True, 2), (1, True, 1)])
+def test_fp32_storage(mode, screened, n_dm):
+    """FP32-stored cderi (the reference's FASTDF_FP32 mode): J and K equal the
+    FP64 oracle on the FP32-rounded tensor widened to FP64 (1e-12 / 1e-11
+    relative), for one and two factors, full and screened pair sets; the
+    narrow K modes read it too (1e-5)."""
+    import torch
+    from electrolites_amd import fastdf
+    nao, naux, nocc = 150, 310, 37
+    rng = np.random.default_rng(50 + 10 * mode + n_dm)
+    pa, _ = O.pair_addresses(nao)
+    if screened:
+        rows, cols = np.divmod(pa, nao)
+        pa = pa[(rng.random(len(pa)) > .3) | (rows == cols)]
+    cd32 = (rng.standard_normal((naux, len(pa))) * .1).astype(np.float32)
+    cs = [rand_psd(nao, nocc, rng) for _ in range(n_dm)]
+    dm = np.array([2 * c @ c.T for c in cs])
+    if mode == 0:
+        dm = fastdf.tag_array(dm, mo_coeff=np.array(cs), mo_occ=np.full((n_dm, nocc), 2.))
+    df = FakeDF(cd32.astype(np.float64), pa, nao)
+    df._cderi = {0: torch.from_numpy(cd32).cuda()}
+    assert df._cderi[0].dtype == torch.float32
+    vj, vk = fastdf.get_jk(df, dm, hermi=1)
+    rj, rk = O.get_jk(cd32.astype(np.float64), pa, np.asarray(dm))
+    assert rel(vj, rj) < 1e-12
+    assert rel(vk, rk) < 1e-11
+    for prec in ('fp32', 'bf16x3'):
+        df.k_precision = prec
+        _, vkl = fastdf.get_jk(df, dm, hermi=1)
+        assert rel(vkl, rk) < 1e-5, prec
