@@ -246,3 +246,36 @@ def test_fp32_storage(mode, screened, n_dm):
         df.k_precision = prec
         _, vkl = fastdf.get_jk(df, dm, hermi=1)
         assert rel(vkl, rk) < 1e-5, prec
+
+
+@pytest.mark.gpu
+def test_uhf_shapes_and_zero_density(water2):
+    """UHF-shaped inputs, as GPU4PySCF's UHF passes them: (2, nao, nao) tagged
+    with per-spin mo_coeff / mo_occ (occupations 0/1, different electron
+    counts per spin; only occupied columns form the factor), and untagged
+    (2, 2, nao, nao) stacks; outputs keep the input shape.  A zero density
+    (no occupied orbital) returns the inputs, as the reference does
+    (fastdf.py:850-851)."""
+    from electrolites_amd import fastdf
+    nao, cderi, pa = water2
+    df = FakeDF(cderi, pa, nao)
+    rng = np.random.default_rng(21)
+    nmo = nao
+    C = np.array([np.linalg.qr(rng.standard_normal((nao, nmo)))[0] for _ in range(2)])
+    occ = np.zeros((2, nmo))
+    occ[0, :6] = 1.
+    occ[1, :4] = 1.
+    dm = np.array([(C[s] * occ[s]) @ C[s].T for s in range(2)])
+    tagged = fastdf.tag_array(dm, mo_coeff=C, mo_occ=occ)
+    vj, vk = fastdf.get_jk(df, tagged, hermi=1)
+    assert vj.shape == vk.shape == (2, nao, nao)
+    rj, rk = O.get_jk(cderi, pa, dm)
+    assert rel(vj, rj) < 1e-12 and rel(vk, rk) < 1e-11
+    d4 = np.array([dm, dm[::-1] * .5])                     # (2, 2, nao, nao)
+    vj4, vk4 = fastdf.get_jk(df, d4, hermi=1)
+    assert vj4.shape == vk4.shape == d4.shape
+    rj4, rk4 = O.get_jk(cderi, pa, d4.reshape(-1, nao, nao))
+    assert rel(vj4.reshape(-1, nao, nao), rj4) < 1e-12 and rel(vk4.reshape(-1, nao, nao), rk4) < 1e-11
+    z = fastdf.tag_array(np.zeros((nao, nao)), mo_coeff=C[0], mo_occ=np.zeros(nmo))
+    out = fastdf.get_jk(df, z, hermi=1)
+    assert out[0] is z and out[1] is z
