@@ -68,7 +68,7 @@ class Entry:
 
 
 _TUNED = {}
-BN_CANDIDATES = (64, 128, 160)
+BN_CANDIDATES = (0, 128, 320)      # 0: each item's own width (kcore::pick_bn)
 
 
 class GroupedJK:

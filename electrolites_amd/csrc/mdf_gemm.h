@@ -60,10 +60,14 @@ struct Mfma<float> {
 
 __device__ __forceinline__ double4_t mfma16(double a, double b, double4_t c) { return Mfma<double>::mma(a, b, c); }
 
-template <int BM, int BN>
+// A workgroup of WGM x WGN wavefronts (64 WGM WGN threads) computes a
+// BM x BN tile; wavefront w owns sub-tile (w % WGM, w / WGM).
+template <int BM, int BN, int WGM = 2, int WGN = 2>
 struct TileShape {
-    static constexpr int WM = BM / 2, WN = BN / 2;       // per-wave tile
+    static constexpr int THREADS = 64 * WGM * WGN;
+    static constexpr int WM = BM / WGM, WN = BN / WGN;   // per-wave tile
     static constexpr int MI = WM / 16, NI = WN / 16;     // MFMA tiles per wave
+    static_assert(MI * 16 == WM && NI * 16 == WN, "wave tile must be a multiple of 16");
     static constexpr int PA = BM + 16, PB = BN + 16;     // LDS pitches (elements)
     static constexpr int LDS_ELEMS = 2 * G_BK * (PA + PB);
     static constexpr int LDS_DOUBLES = LDS_ELEMS;
@@ -81,14 +85,15 @@ struct Acc {
     }
 };
 
-// Runs the K loop.  `lds` must hold TileShape<BM,BN>::LDS_ELEMS elements of T.
-template <int BM, int BN, class LA, class LB, typename T = double>
+// Runs the K loop.  `lds` must hold TileShape<BM,BN,WGM,WGN>::LDS_ELEMS
+// elements of T.
+template <int BM, int BN, class LA, class LB, typename T = double, int WGM = 2, int WGN = 2>
 __device__ __forceinline__ void gemm_mainloop(LA &la, LB &lb, int nk, T *lds,
-                                     Acc<TileShape<BM, BN>::MI, TileShape<BM, BN>::NI, T> &acc) {
-    using S = TileShape<BM, BN>;
+                                     Acc<TileShape<BM, BN, WGM, WGN>::MI, TileShape<BM, BN, WGM, WGN>::NI, T> &acc) {
+    using S = TileShape<BM, BN, WGM, WGN>;
     constexpr int SA = G_BK * S::PA, SB = G_BK * S::PB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WGM, wn = wave / WGM;
     T *const Bbase = lds + 2 * SA;
     const MDF_LDS T *Ar = (const MDF_LDS T *)(lds + wm * S::WM + (lane & 15));
     const MDF_LDS T *Br = (const MDF_LDS T *)(Bbase + wn * S::WN + (lane & 15));
@@ -133,11 +138,11 @@ __device__ __forceinline__ void gemm_mainloop(LA &la, LB &lb, int nk, T *lds,
 // Visits every accumulator element with its tile-local (row, col); the
 // element-to-row layout is that of the MFMA type LT (an FP64 shadow of an
 // FP32 accumulator keeps the FP32 layout).
-template <int BM, int BN, typename LT, class AccT, class F>
+template <int BM, int BN, typename LT, class AccT, class F, int WGM = 2, int WGN = 2>
 __device__ __forceinline__ void gemm_epilogue_layout(const AccT &acc, F &&f) {
-    using S = TileShape<BM, BN>;
+    using S = TileShape<BM, BN, WGM, WGN>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WGM, wn = wave / WGM;
 #pragma unroll
     for (int mi = 0; mi < S::MI; mi++)
 #pragma unroll
@@ -187,10 +192,10 @@ struct RowsLoader {
 
 // Row-major matrix M[k][col] (col contiguous) as the B operand: tile columns
 // c0 .. c0+C-1.
-template <int C, typename T = double, typename Src = double>
+template <int C, typename T = double, typename Src = double, int THREADS = G_THREADS>
 struct ColsLoader {
     using reg_t = Src;
-    static constexpr int N = C * G_BK / G_THREADS;
+    static constexpr int N = C * G_BK / THREADS;
     const Src *base;
     long ld;
     int c0, ncols, k0, kmax;

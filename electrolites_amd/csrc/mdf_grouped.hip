@@ -60,7 +60,8 @@ struct GKItem {
     // split-BF16 (prec 2): workspace offsets of [side][FT hi, FT lo, R hi, R lo]
     void *b3[2][4];
     long ldr, ldft;
-    int bn3;                // step-1 tile width (kb3::pick_bn(ldr))
+    int bn3;                // split-BF16 step-1 tile width (kb3::pick_bn(ldr))
+    int bn1;                // FP64/FP32 step-1 tile width (the plan's bn, or kcore::pick_bn(nocc))
 };
 
 struct Tile {
@@ -169,8 +170,9 @@ __global__ __launch_bounds__(jcore::J_THREADS) void gj_reduce2(const GJItem *ite
     for (long t = (n) * xr_x / N_XCD + xr_j; t < (n) * (xr_x + 1) / N_XCD; t += xr_per)
 
 template <int BN, typename T, typename Src>
-__global__ __launch_bounds__(G_THREADS) void gk_step1(const GKItem *items, const Tile *tiles, long n, char *work) {
-    __shared__ T lds[TileShape<128, BN>::LDS_ELEMS];
+__global__ __launch_bounds__(kcore::Step1Shape<BN>::THREADS) void gk_step1(const GKItem *items, const Tile *tiles,
+                                                                          long n, char *work) {
+    __shared__ T lds[kcore::Step1Shape<BN>::S::LDS_ELEMS];
     XCD_RUN_LOOP(t, n) {
         const Tile e = tiles[t];            // (item, L, i-tile, k-tile | right-factor flag << 16)
         const GKItem &it = items[e.item];
@@ -314,7 +316,8 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     MDF_CHECK_ARG(items && nitems >= 0 && out && work_bytes, "mdf_grouped_plan_create: null argument");
     MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32 || prec == MDF_PREC_BF16X3,
                   "mdf_grouped_plan_create: precision %d", prec);
-    MDF_CHECK_ARG(bn == 0 || bn == 64 || bn == 128 || bn == 160, "mdf_grouped_plan_create: bn %d not in {0,64,128,160}", bn);
+    MDF_CHECK_ARG(bn == 0 || bn == 64 || bn == 128 || bn == 160 || bn == 256 || bn == 320,
+                  "mdf_grouped_plan_create: bn %d not in {0,64,128,160,256,320}", bn);
     for (int m = 0; m < nitems; m++)
         MDF_CHECK_ARG(items[m].cderi_dtype == items[0].cderi_dtype &&
                           (items[m].cderi_dtype == MDF_DTYPE_F64 || items[m].cderi_dtype == MDF_DTYPE_F32),
@@ -385,6 +388,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                 g.vk = x.vk + (long)d * x.nao * x.nao;
                 g.nao = x.nao; g.nocc = x.nocc; g.ksplit = kp.ksplit; g.ntile = kp.ntile;
                 g.nL = kp.nL;
+                g.bn1 = bn ? bn : kcore::pick_bn(x.nocc);
                 g.vec = (x.nocc % (16 / (int)elem)) == 0;
                 // workspace offsets are assigned per wave below; keep the sizes
                 if (b3) {
@@ -450,17 +454,20 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
         const bool b3 = prec == MDF_PREC_BF16X3;
         for (long b = 0; b < maxpass; b++) {
           // split-BF16: one step-1 launch per (pair map or not, tile width)
-          for (int cls = 0; cls < (b3 ? 6 : 1); cls++) {
+          // one step-1 launch per tile width (and, for split-BF16, per pair-map flag)
+          for (int cls = 0; cls < (b3 ? 6 : 5); cls++) {
             static const int kBN3[3] = {128, 256, 320};
-            const int mp = cls / 3, cbn = b3 ? kBN3[cls % 3] : p->bn;
+            static const int kBN1[5] = {64, 128, 160, 256, 320};
+            const int mp = b3 ? cls / 3 : 0, cbn = b3 ? kBN3[cls % 3] : kBN1[cls];
             const long s1 = k1.size();
             for (size_t i = w0; i < w1; i++) {
                 const GKItem &g = p->hk[i];
                 const long L0 = b * g.nL;
                 if (L0 >= g.naux) continue;
                 if (b3 && ((g.pmap != nullptr) != (mp == 1) || g.bn3 != cbn)) continue;
+                if (!b3 && g.bn1 != cbn) continue;
                 const long L1 = std::min(L0 + g.nL, g.naux);
-                const int nkt = b3 ? (int)cdiv(g.ldr, (long)cbn) : (int)cdiv(g.nocc, p->bn);
+                const int nkt = b3 ? (int)cdiv(g.ldr, (long)cbn) : (int)cdiv(g.nocc, cbn);
                 const int nit = (int)cdiv(g.nao, 128);
                 for (int side = 0; side < (g.fr ? 2 : 1); side++)
                     for (long L = L0; L < L1; L++)
@@ -540,11 +547,12 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
 
 template <int BN, typename Src>
 static void launch_gk1_s(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
-    const int g = persistent_grid(n, 2);
+    const int g = persistent_grid(n, BN >= 256 ? 1 : 2);
+    const dim3 b(kcore::Step1Shape<BN>::THREADS);
     if (p->prec == MDF_PREC_FP32)
-        hipLaunchKernelGGL((gk_step1<BN, float, Src>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
+        hipLaunchKernelGGL((gk_step1<BN, float, Src>), dim3(g), b, 0, st, p->kitems, tiles, n, w);
     else
-        hipLaunchKernelGGL((gk_step1<BN, double, Src>), dim3(g), dim3(G_THREADS), 0, st, p->kitems, tiles, n, w);
+        hipLaunchKernelGGL((gk_step1<BN, double, Src>), dim3(g), b, 0, st, p->kitems, tiles, n, w);
 }
 
 template <int BN>
@@ -621,7 +629,9 @@ extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work
                 MDF_LAUNCHED("gk_step2_b3");
             } else if (ph.kind == 1) {
                 const Tile *t = p->k1 + ph.start;
-                switch (p->bn) {
+                switch (ph.bn) {
+                case 320: launch_gk1<320>(p, t, ph.count, w, st); break;
+                case 256: launch_gk1<256>(p, t, ph.count, w, st); break;
                 case 160: launch_gk1<160>(p, t, ph.count, w, st); break;
                 case 128: launch_gk1<128>(p, t, ph.count, w, st); break;
                 default: launch_gk1<64>(p, t, ph.count, w, st); break;

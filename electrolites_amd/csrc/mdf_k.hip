@@ -34,11 +34,11 @@ namespace {
 // (k-tile fastest, then i-tile, then aux row) so the k-tiles of one A tile
 // share an L2.
 template <int BN, bool MAPPED, typename T, typename Src>
-__global__ __launch_bounds__(G_THREADS) void k_step1(
+__global__ __launch_bounds__(Step1Shape<BN>::THREADS) void k_step1(
     const Src *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao,
     const double *__restrict__ F, long ldf, int nocc, T *__restrict__ R, long ldr,
     long strideL, int nkt, int nit, long ntiles) {
-    __shared__ T lds[TileShape<128, BN>::LDS_ELEMS];
+    __shared__ T lds[Step1Shape<BN>::S::LDS_ELEMS];
     const long x = xcd_tile(blockIdx.x, gridDim.x);
     if (x >= ntiles) return;
     const int kt = (int)(x % nkt), it = (int)((x / nkt) % nit);
@@ -134,12 +134,12 @@ void launch_step1(int nL, hipStream_t st, const Src *blk, long ld, const int *pm
                   const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
     const int nkt = (int)cdiv(nocc, BN), nit = (int)cdiv(nao, 128);
     const long ntiles = (long)nkt * nit * nL;
-    const dim3 g((unsigned)round_xcd(ntiles));
+    const dim3 g((unsigned)round_xcd(ntiles)), b(Step1Shape<BN>::THREADS);
     if (pmap)
-        hipLaunchKernelGGL((k_step1<BN, true, T, Src>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
+        hipLaunchKernelGGL((k_step1<BN, true, T, Src>), g, b, 0, st, blk, ld, pmap, nao, f, ldf,
                            nocc, r, ldr, strideL, nkt, nit, ntiles);
     else
-        hipLaunchKernelGGL((k_step1<BN, false, T, Src>), g, dim3(G_THREADS), 0, st, blk, ld, pmap, nao, f, ldf,
+        hipLaunchKernelGGL((k_step1<BN, false, T, Src>), g, b, 0, st, blk, ld, pmap, nao, f, ldf,
                            nocc, r, ldr, strideL, nkt, nit, ntiles);
 }
 
@@ -147,6 +147,8 @@ template <typename T, typename Src>
 int step1(const KPlan &p, int nL, hipStream_t st, const Src *blk, long ld, const int *pmap, int nao,
           const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
     switch (p.bn) {
+    case 320: launch_step1<320, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 256: launch_step1<256, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     case 160: launch_step1<160, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     case 128: launch_step1<128, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     default: launch_step1<64, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;

@@ -20,11 +20,12 @@ __device__ __forceinline__ long tri(long i) { return i * (i + 1) / 2; }
 // ---------------------------------------------------------------- step 1
 // A operand of step 1: rows i0..i0+127 of the unpacked A_L, K-step kt covers
 // j = 16 kt .. 16 kt + 15.
-template <bool MAPPED, typename T = double, typename Src = double>
+template <bool MAPPED, typename T = double, typename Src = double, int THREADS = G_THREADS>
 struct PackedALoader {
     using reg_t = Src;                                   // the cderi storage type
-    static constexpr int N = 128 * G_BK / G_THREADS;     // 8
-    static constexpr int TPJ = G_THREADS / G_BK;         // threads per j row (upper)
+    static constexpr int N = 128 * G_BK / THREADS;       // 8 (256 threads) or 4 (512)
+    static constexpr int TPJ = THREADS / G_BK;           // threads per j row (upper)
+    static constexpr int TPR = G_BK / N;                 // threads per i row (lower)
     const Src *row;      // cderi row L
     const int *pmap;     // MAPPED: pair index of (i, j), -1 if not stored
     int nao, i0;
@@ -41,7 +42,7 @@ struct PackedALoader {
             return;
         }
         upper = 0;
-        const int i = i0 + (t >> 1), jb = j0 + (t & 1) * N;
+        const int i = i0 + t / TPR, jb = j0 + (t % TPR) * N;
         if (!MAPPED && i0 >= j0 + G_BK - 1) {
             // every i >= every j: contiguous in j
             const Src *src = row + tri(i) + jb;
@@ -71,11 +72,22 @@ struct PackedALoader {
 #pragma unroll
             for (int v = 0; v < N; v++) lds[jj * pitch + ib + v] = (T)r[v];
         } else {
-            const int ii = t >> 1, jb = (t & 1) * N;
+            const int ii = t / TPR, jb = (t % TPR) * N;
 #pragma unroll
             for (int v = 0; v < N; v++) lds[(jb + v) * pitch + ii] = (T)r[v];
         }
     }
+};
+
+// Step-1 tile widths: 64, 128 and 160 columns on 2 x 2 wavefronts, 256 and
+// 320 on 2 x 4 (a workgroup of 512 threads then covers up to 320 occupied
+// columns, so each cderi tile is read once per 320 columns instead of once
+// per 128 or 160).
+template <int BN>
+struct Step1Shape {
+    static constexpr int WGN = BN >= 256 ? 4 : 2;
+    static constexpr int THREADS = 128 * WGN;
+    using S = TileShape<128, BN, 2, WGN>;
 };
 
 // One 128 x BN tile of step 1 for aux row `row` (the cderi row L): rows
@@ -84,16 +96,18 @@ template <int BN, bool MAPPED, typename T, typename Src = double>
 __device__ __forceinline__ void step1_tile(const Src *__restrict__ row, const int *__restrict__ pmap, int nao,
                                   const double *__restrict__ F, long ldf, int nocc, T *__restrict__ Rl, long ldr,
                                   int i0, int k0, T *lds) {
-    using S = TileShape<128, BN>;
-    PackedALoader<MAPPED, T, Src> la{row, pmap, nao, i0, 0};
-    ColsLoader<BN, T, double> lb{F, ldf, k0, nocc, 0, nao};
+    using C = Step1Shape<BN>;
+    using S = typename C::S;
+    PackedALoader<MAPPED, T, Src, C::THREADS> la{row, pmap, nao, i0, 0};
+    ColsLoader<BN, T, double, C::THREADS> lb{F, ldf, k0, nocc, 0, nao};
     Acc<S::MI, S::NI, T> acc;
     acc.zero();
-    gemm_mainloop<128, BN>(la, lb, (nao + G_BK - 1) / G_BK, lds, acc);
-    gemm_epilogue<128, BN, T>(acc, [&](int r, int c, T v) {
+    gemm_mainloop<128, BN, decltype(la), decltype(lb), T, 2, C::WGN>(la, lb, (nao + G_BK - 1) / G_BK, lds, acc);
+    auto emit = [&](int r, int c, T v) {
         const int i = i0 + r, k = k0 + c;
         if (i < nao && k < nocc) Rl[(long)i * ldr + k] = v;
-    });
+    };
+    gemm_epilogue_layout<128, BN, T, decltype(acc), decltype(emit) &, 2, C::WGN>(acc, emit);
 }
 
 // ---------------------------------------------------------------- step 2
@@ -215,13 +229,16 @@ __device__ __forceinline__ void finalize_range(const double *__restrict__ part, 
 }
 
 // step-1 N tile: the width in {64, 128, 160} that pads nocc least (ties: wider)
+// step-1 tile width: each k-tile re-reads the cderi tile (worth about 64
+// columns of MFMA work), so minimise nkt * (64 + BN) over the widths; ties
+// go to the wider tile
 inline int pick_bn(int nocc) {
-    const int cand[3] = {160, 128, 64};
+    const int cand[5] = {64, 128, 160, 256, 320};
     int best = 64;
-    long best_pad = 1L << 40;
+    long best_cost = 1L << 40;
     for (int b : cand) {
-        const long pad = (long)((nocc + b - 1) / b) * b - nocc;
-        if (pad < best_pad) { best_pad = pad; best = b; }
+        const long cost = (long)((nocc + b - 1) / b) * (64 + b);
+        if (cost <= best_cost) { best_cost = cost; best = b; }
     }
     return best;
 }
