@@ -239,12 +239,14 @@ def bench_serve(args):
     rates = [float(x) for x in args.rates.split(',')]
     per_rate = {}
     for rate in rates:
+        torch.cuda.empty_cache()
         arr = serve.poisson_arrivals(len(mols), rate, seed=args.seed + 7)
         reqs = [serve.Request(m, arr[i], rid=i, k_precision='mixed') for i, m in enumerate(mols)]
         srv = serve.Server(graphs=args.graphs)
         s = srv.serve(reqs)
         s['max_active'] = max(tr[1] for tr in srv.trace) if srv.trace else 0
         s['ticks'] = srv.ticks
+        torch.cuda.empty_cache()
         base = [serve.Request(m, arr[i], rid=i) for i, m in enumerate(mols)]
         b = serve.serve_one_at_a_time(base)
         de = max(abs(r.e_tot - q.e_tot) for r, q in zip(sorted(reqs, key=lambda r: r.rid), base))

@@ -75,9 +75,12 @@ class GroupedJK:
     """A device plan for one batch of :class:`Entry` objects."""
 
     def __init__(self, entries, with_j=True, with_k=True, precision='fp64', bn=None, tune=True,
-                 work_limit=0):
+                 work_limit=0, work_pool=None):
         self.entries = list(entries)
         self.work_limit = int(work_limit)
+        # a shared grow-only workspace (an _dev.Workspace) for callers that
+        # run many plans one after another on one stream; None: own buffer
+        self.work_pool = work_pool
         self.with_j, self.with_k = with_j, with_k
         self.precision = precision
         self.prec = fastdf.PRECISIONS[precision]
@@ -90,8 +93,8 @@ class GroupedJK:
                 bn = self._autotune(sig)
         self.bn = bn
         self._plan, self.work_bytes = self._create(bn)
-        self.work = _dev.torch().empty(max(self.work_bytes, 256), dtype=_dev.torch().uint8,
-                                       device=_dev.device())
+        self.work = None if work_pool is not None else \
+            _dev.torch().empty(max(self.work_bytes, 256), dtype=_dev.torch().uint8, device=_dev.device())
         self.graph = None
 
     def _create(self, bn):
@@ -138,11 +141,14 @@ class GroupedJK:
         if self.graph is not None:
             self.graph.replay()
             return
-        _lib.call('mdf_grouped_jk', self._plan, self.work.data_ptr(), self.work.numel(), _dev.stream_ptr())
+        work = self.work if self.work is not None else self.work_pool.get(self.work_bytes)
+        _lib.call('mdf_grouped_jk', self._plan, work.data_ptr(), work.numel(), _dev.stream_ptr())
 
     def capture_graph(self):
         """Capture one run in a HIP graph; later ``run()`` calls replay it."""
         t = _dev.torch()
+        if self.work is None:
+            raise ValueError('a captured plan needs its own workspace (work_pool=None)')
         s = t.cuda.Stream()
         s.wait_stream(t.cuda.current_stream())
         with t.cuda.stream(s):

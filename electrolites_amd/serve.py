@@ -168,6 +168,7 @@ class Server:
         self.pipeline = pipeline
         self._plan_tick = {}
         self._weight = [0., 0.]
+        self._work = _dev.Workspace()
         self.workers = workers
         self.host_diag_max = host_diag_max
         self.low_precision = low_precision
@@ -250,7 +251,11 @@ class Server:
         key = (prec,) + tuple(sorted(id(s) for s in slots))
         g = self._plans.get(key)
         if g is None:
-            g = batch.GroupedJK([s.entry for s in slots], precision=prec, tune=False)
+            # plans run one after another on one stream: one shared workspace
+            # (per-plan buffers would pile up in the caching allocator as the
+            # membership changes, starving the libraries' own allocations)
+            g = batch.GroupedJK([s.entry for s in slots], precision=prec, tune=False,
+                                work_pool=None if self.graphs else self._work)
             g.uses = 0
             self._plans[key] = g
         g.uses += 1
@@ -373,6 +378,7 @@ class Server:
         if self._builder is not None:
             self._builder.shutdown()
             self._builder = None
+        self._work.release()
 
 
 def serve_one_at_a_time(requests, conv_tol=1e-10, max_cycle=100, k_precision='fp64'):
