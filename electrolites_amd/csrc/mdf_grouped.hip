@@ -453,7 +453,6 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
         }
         const bool b3 = prec == MDF_PREC_BF16X3;
         for (long b = 0; b < maxpass; b++) {
-          // split-BF16: one step-1 launch per (pair map or not, tile width)
           // one step-1 launch per tile width (and, for split-BF16, per pair-map flag)
           for (int cls = 0; cls < (b3 ? 6 : 5); cls++) {
             static const int kBN3[3] = {128, 256, 320};
@@ -474,6 +473,12 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                         for (int it = 0; it < nit; it++)
                             for (int kt = 0; kt < nkt; kt++) k1.push_back({(int)i, (int)L, it, kt | (side << 16)});
             }
+            // longest tiles first (cost ~ the item's nao K-steps): the
+            // grid-stride loop then ends on short tiles, not on one long one;
+            // each item's tiles stay contiguous for the XCD runs
+            std::stable_sort(k1.begin() + s1, k1.end(), [&](const Tile &x, const Tile &y) {
+                return p->hk[x.item].nao > p->hk[y.item].nao;
+            });
             p->kphases.push_back({1, s1, (long)k1.size() - s1, cbn, mp});
           }
             const long s2 = k2.size();
@@ -483,6 +488,16 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                 for (int s = 0; s < g.ksplit; s++)
                     for (int tl = 0; tl < kgrid[i]; tl++) k2.push_back({(int)i, tl, s, (int)b});
             }
+            // longest first: a step-2 tile's cost is its share of the pass's
+            // (L, k) steps
+            auto s2cost = [&](const Tile &x) {
+                const GKItem &g = p->hk[x.item];
+                const long nLb = std::min(g.nL, g.naux - (long)x.c * g.nL);
+                const long kc = prec == MDF_PREC_BF16X3 ? g.ldr / b3::BK : cdiv((long)g.nocc, (long)G_BK);
+                return nLb * kc / g.ksplit;
+            };
+            std::stable_sort(k2.begin() + s2, k2.end(),
+                             [&](const Tile &x, const Tile &y) { return s2cost(x) > s2cost(y); });
             p->kphases.push_back({2, s2, (long)k2.size() - s2, 0, 0});
         }
         const long sf = kf.size();
