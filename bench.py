@@ -412,7 +412,29 @@ def main():
     fp32_store['j_hbm_frac'] = bj32 / (fp32_store['j_ms'] * 1e-3) / HBM_PEAK
     log(f'FP32 storage: J {fp32_store["j_ms"]:.2f} ms ({100 * fp32_store["j_hbm_frac"]:.1f} % HBM), '
         f'K {fp32_store["k_ms"]:.1f} ms')
-    del c32, vj32, vk32s, vj64, df32
+    del c32, vj32, vk32s, df32
+
+    # J without the cderi (three-centre integrals contracted on the fly)
+    ri_j = None
+    if not args.synthetic_cderi:
+        from electrolites_amd import fastdfj3c
+
+        class MolDF:
+            pass
+
+        mdf = MolDF()
+        mdf.mol = mol
+        fastdfj3c.get_j(mdf, dm, hermi=1)                  # tables, metric factor
+        e4 = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e4[0].record()
+        vjr = fastdfj3c.get_j(mdf, dm, hermi=1)
+        e4[1].record()
+        torch.cuda.synchronize()
+        ri_j = {'ms': e4[0].elapsed_time(e4[1]), 'chunks': len(mdf._mdf_j3c.tasks),
+                'rel_err_vs_cderi_j': float((vjr - vj64).abs().max() / vj64.abs().max())}
+        log(f'RI-J without a cderi: {ri_j["ms"]:.1f} ms ({ri_j["chunks"]} chunks per sweep)')
+        del vjr, mdf
+    del vj64
 
     S = naux * npairs * 8
     bj = 2 * S + 16 * (npairs + naux)
@@ -438,6 +460,7 @@ def main():
                       'k_bf16x3_rel_err_vs_fp64': low['bf16x3'][1]},
         'cderi_build': build,
         'fp32_storage': fp32_store,
+        'ri_j_no_cderi': ri_j,
         'roofline': {'bound': 'mfma', 'achieved': fk / tk / 1e12,
                      'peak': FP64_MFMA_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': fk / tk / FP64_MFMA_PEAK, 'traffic': traffic.get('k'),

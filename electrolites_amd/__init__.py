@@ -6,6 +6,11 @@ over the two entry points the reference replaces (electrolites/__init__.py
 
     gpu4pyscf.df.df_jk.get_jk        -> electrolites_amd.fastdf.get_jk
     gpu4pyscf.df.df._cholesky_eri    -> electrolites_amd.fastdfcderi._cholesky_eri
+    gpu4pyscf.df.df_jk.get_j         -> electrolites_amd.fastdfj3c.get_j
+
+(the reference's 'fastdfj3c' entry patches the two Int3c2eOpt contractions
+under ``df_jk.get_j``, :127-132; this build replaces the whole J-without-a-
+cderi function, whose contractions are its own kernels)
 
 Targets are checked before anything is assigned (``check_targets``, as
 electrolites/__init__.py:302-322), the originals are kept for fall-through
@@ -27,8 +32,9 @@ __version__ = '0.1.0'
 _TARGETS = {
     'fastdf': (('gpu4pyscf.df.df_jk', 'get_jk'),),
     'fastdfcderi': (('gpu4pyscf.df.df', '_cholesky_eri'),),
+    'fastdfj3c': (('gpu4pyscf.df.df_jk', 'get_j'),),
 }
-MODULES = ('fastdf', 'fastdfcderi')
+MODULES = ('fastdf', 'fastdfcderi', 'fastdfj3c')
 _patched = []
 
 
@@ -63,6 +69,12 @@ def _install(name):
         if getattr(fastdfcderi, '_orig_cholesky_eri', None) is None:
             fastdfcderi._orig_cholesky_eri = mod._cholesky_eri
         mod._cholesky_eri = fastdfcderi._cholesky_eri
+    elif name == 'fastdfj3c':
+        from . import fastdfj3c
+        mod = importlib.import_module('gpu4pyscf.df.df_jk')
+        if getattr(fastdfj3c, '_orig_get_j', None) is None:
+            fastdfj3c._orig_get_j = mod.get_j
+        mod.get_j = fastdfj3c.get_j
 
 
 def patch(*names, strict=True):

@@ -35,6 +35,8 @@ _SIGS = {
     'mdf_build_pair_map': (I, [P, L, I, P, P]),
     'mdf_fill_zero': (I, [P, L, P]),
     'mdf_cast_f32': (I, [P, L, L, L, P, L, P]),
+    'mdf_j_rho': (I, [P, I, L, L, L, P, I, L, P, L, I, P, SZ, P]),
+    'mdf_j_vj': (I, [P, I, L, L, L, P, I, L, P, L, I, P, SZ, P]),
     'mdf_int3c2e': (I, [P, P, P, P, P, P, L, P, L, P]),
     'mdf_int2c2e': (I, [P, P, P, P, L, P]),
     'mdf_int1e': (I, [P, I, P, P, I, P, P, P, P, P, P]),

@@ -32,9 +32,10 @@ __global__ __launch_bounds__(J_THREADS) void j_rho_partial(const S *__restrict__
 }
 
 __global__ __launch_bounds__(J_THREADS) void j_rho_reduce(const double *__restrict__ part, long nchunk, int ndm,
-                                                          long naux, double *__restrict__ rho, long ldrho) {
+                                                          long naux, double *__restrict__ rho, long ldrho,
+                                                          int accumulate) {
     const long e0 = (long)blockIdx.x * J_THREADS;
-    rho_reduce_range(part, nchunk, ndm, naux, rho, ldrho, e0, min(e0 + J_THREADS, (long)ndm * naux));
+    rho_reduce_range(part, nchunk, ndm, naux, rho, ldrho, e0, min(e0 + J_THREADS, (long)ndm * naux), accumulate);
 }
 
 template <int NDM, int VEC, typename S>
@@ -55,19 +56,26 @@ __global__ __launch_bounds__(J_THREADS) void j_vj_reduce(const double *__restric
     vj_reduce_range(part, ngroup, ndm, npairs, vj, ldvj, accumulate, e0, min(e0 + J_THREADS, (long)ndm * npairs));
 }
 
+// passes: 1 = rho only (rho += with accumulate_rho), 2 = vj only (from the
+// given rho), 3 = both (rho written, then vj)
 template <int NDM, int VEC, typename S>
 int run_j_group(const S *L, long ld, long naux, long npairs, const double *d, long ldd,
                 double *rho, long ldrho, double *vj, long ldvj, int accumulate,
-                double *work, const JPlan &pl, hipStream_t st) {
+                double *work, const JPlan &pl, hipStream_t st, int passes = 3, int accumulate_rho = 0) {
     double *part1 = work;
     double *part2 = work + align_up(pl.part1_words, 64);
-    dim3 g1((unsigned)pl.nchunk, (unsigned)pl.nrblock);
-    hipLaunchKernelGGL((j_rho_partial<NDM, VEC, S>), g1, dim3(J_THREADS), 0, st, L, ld, naux, npairs, d, ldd, part1);
-    MDF_LAUNCHED("j_rho_partial");
-    long nt = (long)NDM * naux;
-    hipLaunchKernelGGL(j_rho_reduce, dim3((unsigned)cdiv(nt, (long)J_THREADS)), dim3(J_THREADS), 0, st,
-                       part1, pl.nchunk, NDM, naux, rho, ldrho);
-    MDF_LAUNCHED("j_rho_reduce");
+    long nt;
+    if (passes & 1) {
+        dim3 g1((unsigned)pl.nchunk, (unsigned)pl.nrblock);
+        hipLaunchKernelGGL((j_rho_partial<NDM, VEC, S>), g1, dim3(J_THREADS), 0, st, L, ld, naux, npairs, d, ldd,
+                           part1);
+        MDF_LAUNCHED("j_rho_partial");
+        nt = (long)NDM * naux;
+        hipLaunchKernelGGL(j_rho_reduce, dim3((unsigned)cdiv(nt, (long)J_THREADS)), dim3(J_THREADS), 0, st,
+                           part1, pl.nchunk, NDM, naux, rho, ldrho, accumulate_rho);
+        MDF_LAUNCHED("j_rho_reduce");
+    }
+    if (!(passes & 2)) return MDF_OK;
     dim3 g2((unsigned)pl.ncblock, (unsigned)pl.ngroup);
     if (pl.ngroup == 1 && !accumulate) {
         hipLaunchKernelGGL((j_vj_pass<NDM, VEC, S>), g2, dim3(J_THREADS), 0, st,
@@ -88,12 +96,12 @@ int run_j_group(const S *L, long ld, long naux, long npairs, const double *d, lo
 template <int VEC, typename S>
 int dispatch_j(int nd, const S *L, long ld, long naux, long npairs, const double *d, long ldd,
                double *rho, long ldrho, double *vj, long ldvj, int accumulate, double *work,
-               const JPlan &pl, hipStream_t st) {
+               const JPlan &pl, hipStream_t st, int passes = 3, int acc_rho = 0) {
     switch (nd) {
-    case 1: return run_j_group<1, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st);
-    case 2: return run_j_group<2, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st);
-    case 3: return run_j_group<3, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st);
-    default: return run_j_group<4, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st);
+    case 1: return run_j_group<1, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho);
+    case 2: return run_j_group<2, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho);
+    case 3: return run_j_group<3, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho);
+    default: return run_j_group<4, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho);
     }
 }
 
@@ -119,7 +127,7 @@ extern "C" size_t mdf_get_j_workspace(long naux, long npairs, int n_dm) {
 template <typename S>
 static int get_j_typed(const S *cderi, int vec, long naux, long npairs, long ld, const double *dm_pairs, int n_dm,
                        long ldd, double *rho, long ldrho, double *vj_pairs, long ldvj, int accumulate, void *work,
-                       hipStream_t st) {
+                       hipStream_t st, int passes = 3, int acc_rho = 0) {
     for (int n0 = 0; n0 < n_dm; n0 += 4) {
         const int nd = n_dm - n0 < 4 ? n_dm - n0 : 4;
         JPlan pl = plan_j(naux, npairs, nd, vec);
@@ -128,13 +136,13 @@ static int get_j_typed(const S *cderi, int vec, long naux, long npairs, long ld,
         double *r = rho + n0 * ldrho, *v = vj_pairs + n0 * ldvj;
         if (vec == 1)
             rc = dispatch_j<1, S>(nd, cderi, ld, naux, npairs, d, ldd, r, ldrho, v, ldvj, accumulate, (double *)work,
-                                  pl, st);
+                                  pl, st, passes, acc_rho);
         else if constexpr (sizeof(S) == 8)
             rc = dispatch_j<2, S>(nd, cderi, ld, naux, npairs, d, ldd, r, ldrho, v, ldvj, accumulate, (double *)work,
-                                  pl, st);
+                                  pl, st, passes, acc_rho);
         else
             rc = dispatch_j<4, S>(nd, cderi, ld, naux, npairs, d, ldd, r, ldrho, v, ldvj, accumulate, (double *)work,
-                                  pl, st);
+                                  pl, st, passes, acc_rho);
         if (rc) return rc;
     }
     return MDF_OK;
@@ -170,4 +178,57 @@ extern "C" int mdf_get_j(const void *cderi, int cderi_dtype, long naux, long npa
                            ldvj, accumulate, work, st);
     return get_j_typed((const double *)cderi, vec, naux, npairs, ld, dm_pairs, n_dm, ldd, rho, ldrho, vj_pairs, ldvj,
                        accumulate, work, st);
+}
+
+// The two halves of mdf_get_j on their own, for a three-centre block that is
+// produced and consumed chunk by chunk (the J build without a stored cderi):
+//   mdf_j_rho: rho[n, P] (+)= sum_p B[P, p] d[n, p]
+//   mdf_j_vj:  vj[n, p] (+)= sum_P c[n, P] B[P, p]
+// Replace GPU4PySCF's Int3c2eOpt.contract_dm / contract_auxvec as the
+// reference's RI-J path calls them (electrolites/fastdfj3c.py:388-530,
+// :726-784).  Same workspace as mdf_get_j.
+static int j_half(const void *B, int dtype, long naux, long width, long ld, const double *x, int n_dm, long ldx,
+                  double *y, long ldy, int accumulate, void *work, size_t work_bytes, void *stream, int passes) {
+    MDF_CHECK_ARG(naux >= 0 && width >= 0 && n_dm >= 1 && ld >= width, "mdf_j_%s: bad sizes",
+                  passes == 1 ? "rho" : "vj");
+    MDF_CHECK_ARG(B && x && y, "mdf_j_%s: null pointer", passes == 1 ? "rho" : "vj");
+    MDF_CHECK_ARG(passes == 1 ? (ldx >= width && ldy >= naux) : (ldx >= naux && ldy >= width),
+                  "mdf_j_%s: leading dimensions too small", passes == 1 ? "rho" : "vj");
+    MDF_CHECK_ARG(dtype == MDF_DTYPE_F64 || dtype == MDF_DTYPE_F32, "mdf_j: dtype %d", dtype);
+    hipStream_t st = as_stream(stream);
+    if (width == 0 || naux == 0) {
+        if (!accumulate) {
+            const long n = passes == 1 ? naux : width;
+            for (int i = 0; i < n_dm && n > 0; i++) MDF_HIP(hipMemsetAsync(y + i * ldy, 0, n * sizeof(double), st));
+        }
+        return MDF_OK;
+    }
+    const size_t need = mdf_get_j_workspace(naux, width, n_dm);
+    MDF_CHECK_ARG(work != nullptr && work_bytes >= need, "mdf_j: workspace %zu bytes < %zu required", work_bytes,
+                  need);
+    const int vec = j_vec(B, ld, dtype);
+    // pass 1: d = x, rho = y; pass 2: rho = x (the fitted coefficients), vj = y
+    if (dtype == MDF_DTYPE_F32)
+        return passes == 1
+            ? get_j_typed((const float *)B, vec, naux, width, ld, x, n_dm, ldx, y, ldy, y, ldy, 0, work, st, 1,
+                          accumulate)
+            : get_j_typed((const float *)B, vec, naux, width, ld, x, n_dm, ldx, (double *)x, ldx, y, ldy, accumulate,
+                          work, st, 2, 0);
+    return passes == 1
+        ? get_j_typed((const double *)B, vec, naux, width, ld, x, n_dm, ldx, y, ldy, y, ldy, 0, work, st, 1,
+                      accumulate)
+        : get_j_typed((const double *)B, vec, naux, width, ld, x, n_dm, ldx, (double *)x, ldx, y, ldy, accumulate,
+                      work, st, 2, 0);
+}
+
+extern "C" int mdf_j_rho(const void *B, int dtype, long naux, long width, long ld, const double *d, int n_dm,
+                         long ldd, double *rho, long ldrho, int accumulate, void *work, size_t work_bytes,
+                         void *stream) {
+    return j_half(B, dtype, naux, width, ld, d, n_dm, ldd, rho, ldrho, accumulate, work, work_bytes, stream, 1);
+}
+
+extern "C" int mdf_j_vj(const void *B, int dtype, long naux, long width, long ld, const double *c, int n_dm,
+                        long ldc, double *vj, long ldvj, int accumulate, void *work, size_t work_bytes,
+                        void *stream) {
+    return j_half(B, dtype, naux, width, ld, c, n_dm, ldc, vj, ldvj, accumulate, work, work_bytes, stream, 2);
 }

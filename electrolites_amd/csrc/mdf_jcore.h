@@ -123,13 +123,14 @@ __device__ __forceinline__ void rho_partial_tile(const S *__restrict__ L, long l
     __syncthreads();
 }
 
-// rho[n, r] = sum_c part[c, n, r] for the flat element range [e0, e1).
+// rho[n, r] (+)= sum_c part[c, n, r] for the flat element range [e0, e1).
 __device__ __forceinline__ void rho_reduce_range(const double *__restrict__ part, long nchunk, int ndm, long naux,
-                                        double *__restrict__ rho, long ldrho, long e0, long e1) {
+                                        double *__restrict__ rho, long ldrho, long e0, long e1,
+                                        int accumulate = 0) {
     for (long t = e0 + threadIdx.x; t < e1; t += J_THREADS) {
         const int n = (int)(t / naux);
         const long r = t % naux;
-        double s = 0.;
+        double s = accumulate ? rho[n * ldrho + r] : 0.;
         for (long c = 0; c < nchunk; c++) s += part[(c * ndm + n) * naux + r];
         rho[n * ldrho + r] = s;
     }

@@ -40,6 +40,12 @@ class DF:
     def get_jk(self, dm, hermi=1, with_j=True, with_k=True, omega=None):
         return fastdf.get_jk(self, dm, hermi, with_j, with_k, omega)
 
+    def get_j(self, dm, hermi=1):
+        """J without the cderi: three-centre integrals contracted on the fly
+        (:func:`electrolites_amd.fastdfj3c.get_j`; ``df_jk.get_j``)."""
+        from . import fastdfj3c
+        return fastdfj3c.get_j(self, dm, hermi)
+
     def cderi(self):
         if self._cderi is None:
             self.build()

@@ -59,6 +59,19 @@ int mdf_get_j(const void *cderi, int cderi_dtype, long naux, long npairs, long l
               double *rho, long ldrho, double *vj_pairs, long ldvj, int accumulate,
               void *work, size_t work_bytes, void *stream);
 
+/* The two halves of mdf_get_j for a three-centre block B (naux, width) that
+ * is produced and consumed chunk by chunk -- the J build without a stored
+ * cderi.  Replace GPU4PySCF's Int3c2eOpt.contract_dm / contract_auxvec as
+ * the reference's RI-J path calls them (electrolites/fastdfj3c.py:388-530,
+ * :726-784):
+ *   mdf_j_rho: rho[n, P] (+)= sum_p B[P, p] d[n, p]     d (n_dm, width), ldd
+ *   mdf_j_vj:  vj[n, p]  (+)= sum_P c[n, P] B[P, p]     c (n_dm, naux), ldc
+ * Workspace: mdf_get_j_workspace(naux, width, n_dm). */
+int mdf_j_rho(const void *B, int dtype, long naux, long width, long ld, const double *d, int n_dm, long ldd,
+              double *rho, long ldrho, int accumulate, void *work, size_t work_bytes, void *stream);
+int mdf_j_vj(const void *B, int dtype, long naux, long width, long ld, const double *c, int n_dm, long ldc,
+             double *vj, long ldvj, int accumulate, void *work, size_t work_bytes, void *stream);
+
 /* ---- K: exchange over the packed cderi (FP64 MFMA) ----------------------
  * Replaces electrolites/fastdf.py:974-1005 (fill_symmetric unpack, step 1
  * contract('Lij,njk->nikL'), step 2 _syrk / contract('nikL,njkL->nij')) and

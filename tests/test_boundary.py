@@ -25,7 +25,12 @@ def fake_gpu4pyscf(monkeypatch):
     df_pkg = types.ModuleType('gpu4pyscf.df')
     df_jk = types.ModuleType('gpu4pyscf.df.df_jk')
     df_mod = types.ModuleType('gpu4pyscf.df.df')
+    def orig_get_j(dfobj, dm, hermi=1):
+        calls.append(('get_j', hermi))
+        return 'orig-vj'
+
     df_jk.get_jk = orig_get_jk
+    df_jk.get_j = orig_get_j
     df_mod._cholesky_eri = orig_cholesky_eri
     root.df = df_pkg
     df_pkg.df_jk = df_jk
@@ -34,20 +39,24 @@ def fake_gpu4pyscf(monkeypatch):
                       ('gpu4pyscf.df.df_jk', df_jk), ('gpu4pyscf.df.df', df_mod)]:
         monkeypatch.setitem(sys.modules, name, mod)
     import electrolites_amd
-    from electrolites_amd import fastdf, fastdfcderi
+    from electrolites_amd import fastdf, fastdfcderi, fastdfj3c
     monkeypatch.setattr(fastdf, '_orig_get_jk', None)
     monkeypatch.setattr(fastdfcderi, '_orig_cholesky_eri', None, raising=False)
+    monkeypatch.setattr(fastdfj3c, '_orig_get_j', None, raising=False)
     monkeypatch.setattr(electrolites_amd, '_patched', [])
-    return types.SimpleNamespace(df_jk=df_jk, df=df_mod, calls=calls,
-                                 orig_get_jk=orig_get_jk, orig_cholesky_eri=orig_cholesky_eri)
+    return types.SimpleNamespace(df_jk=df_jk, df=df_mod, calls=calls, orig_get_jk=orig_get_jk,
+                                 orig_cholesky_eri=orig_cholesky_eri, orig_get_j=orig_get_j)
 
 
 def test_patch_installs_and_keeps_originals(fake_gpu4pyscf):
     import electrolites_amd
     from electrolites_amd import fastdf, fastdfcderi
-    assert electrolites_amd.check_targets() == {'fastdf': None, 'fastdfcderi': None}
-    assert electrolites_amd.patch() == ('fastdf', 'fastdfcderi')
+    from electrolites_amd import fastdfj3c
+    assert electrolites_amd.check_targets() == {'fastdf': None, 'fastdfcderi': None, 'fastdfj3c': None}
+    assert electrolites_amd.patch() == ('fastdf', 'fastdfcderi', 'fastdfj3c')
     assert fake_gpu4pyscf.df_jk.get_jk is fastdf.get_jk
+    assert fake_gpu4pyscf.df_jk.get_j is fastdfj3c.get_j
+    assert fastdfj3c._orig_get_j is fake_gpu4pyscf.orig_get_j
     assert fake_gpu4pyscf.df._cholesky_eri is fastdfcderi._cholesky_eri
     assert fastdf._orig_get_jk is fake_gpu4pyscf.orig_get_jk
     assert fastdfcderi._orig_cholesky_eri is fake_gpu4pyscf.orig_cholesky_eri
@@ -78,7 +87,7 @@ def test_missing_target_is_reported(fake_gpu4pyscf, monkeypatch):
     assert rep['fastdf'] is None and 'missing' in rep['fastdfcderi']
     with pytest.raises(RuntimeError):
         electrolites_amd.patch()
-    assert electrolites_amd.patch(strict=False) == ('fastdf',)
+    assert electrolites_amd.patch(strict=False) == ('fastdf', 'fastdfj3c')
 
 
 def test_standalone_decline_raises(monkeypatch):
