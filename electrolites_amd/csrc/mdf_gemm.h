@@ -126,10 +126,13 @@ __device__ __forceinline__ void gemm_mainloop(LA &la, LB &lb, int nk, T *lds,
             for (int mi = 0; mi < S::MI; mi++)
 #pragma unroll
                 for (int ni = 0; ni < S::NI; ni++) acc.v[mi][ni] = Mfma<T>::mma(a[mi], b[ni], acc.v[mi][ni]);
-        }
-        if (more) {
-            la.store(ra, lds + (cur ^ 1) * SA, S::PA);
-            lb.store(rb, Bbase + (cur ^ 1) * SB, S::PB);
+            // the next K-step's tile goes into the other buffer (read by no
+            // wave since the last barrier) halfway through this step's MFMAs,
+            // so the stores overlap the matrix pipe instead of following it
+            if (kq == G_BK / 8 - 1 && more) {
+                la.store(ra, lds + (cur ^ 1) * SA, S::PA);
+                lb.store(rb, Bbase + (cur ^ 1) * SB, S::PB);
+            }
         }
         __syncthreads();
     }
