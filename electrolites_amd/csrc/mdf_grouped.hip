@@ -34,6 +34,11 @@ using namespace mdf;
 namespace {
 
 constexpr int GJ_NDM = 8;   // densities per molecule in one grouped J pass
+// J tiles are launched in classes by the item's density count (at most 1, 2,
+// 4 or 8), each with the matching register width: an item with one density
+// does not pay for eight accumulators and eight wavefront sums per row.
+constexpr int GJ_CLASSES = 4;
+inline int gj_class(int ndm) { return ndm <= 1 ? 0 : ndm <= 2 ? 1 : ndm <= 4 ? 2 : 3; }
 
 // Per-molecule J work on the device.
 struct GJItem {
@@ -85,6 +90,7 @@ struct mdf_grouped_plan {
     int dtype;                          // cderi storage type of every item
     int nj, nk;                         // items
     long t_j1, t_jr1, t_j2, t_jr2;      // tile counts
+    long jc1[GJ_CLASSES + 1], jc2[GJ_CLASSES + 1];   // pass-1/2 tile offsets per density class
     long t_k1, t_k2, t_kf;
     void *dev;                          // owned device memory: items + tables
     GJItem *jitems;
@@ -104,7 +110,7 @@ namespace {
 constexpr long RED_CHUNK = 4096;    // elements per reduction tile
 constexpr size_t MDF_GROUPED_DEFAULT_LIMIT = 32ull << 30;   // K wave workspace
 
-template <typename S>
+template <int NDM, typename S>
 __global__ __launch_bounds__(jcore::J_THREADS) void gj_pass1(const GJItem *items, const Tile *tiles, long n,
                                                              char *work) {
     constexpr int W = 16 / sizeof(S);       // values per 16-byte vector
@@ -114,11 +120,9 @@ __global__ __launch_bounds__(jcore::J_THREADS) void gj_pass1(const GJItem *items
         double *part = (double *)(work + (size_t)it.part1);
         const S *L = (const S *)it.cderi;
         if (it.vec == W)
-            jcore::rho_partial_tile<GJ_NDM, W, S>(L, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part, e.a,
-                                                  e.b);
+            jcore::rho_partial_tile<NDM, W, S>(L, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part, e.a, e.b);
         else
-            jcore::rho_partial_tile<GJ_NDM, 1, S>(L, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part, e.a,
-                                                  e.b);
+            jcore::rho_partial_tile<NDM, 1, S>(L, it.ld, it.naux, it.npairs, it.d, it.npairs, it.ndm, part, e.a, e.b);
     }
 }
 
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(jcore::J_THREADS) void gj_reduce1(const GJItem *ite
     }
 }
 
-template <typename S>
+template <int NDM, typename S>
 __global__ __launch_bounds__(jcore::J_THREADS) void gj_pass2(const GJItem *items, const Tile *tiles, long n,
                                                              char *work) {
     constexpr int W = 16 / sizeof(S);
@@ -144,11 +148,11 @@ __global__ __launch_bounds__(jcore::J_THREADS) void gj_pass2(const GJItem *items
         double *out = (double *)(work + (size_t)it.part2) + (long)e.b * it.ndm * it.npairs;
         const S *L = (const S *)it.cderi;
         if (it.vec == W)
-            jcore::vj_pass_tile<GJ_NDM, W, S>(L, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out, it.npairs,
-                                              e.a);
+            jcore::vj_pass_tile<NDM, W, S>(L, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out, it.npairs,
+                                           e.a);
         else
-            jcore::vj_pass_tile<GJ_NDM, 1, S>(L, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out, it.npairs,
-                                              e.a);
+            jcore::vj_pass_tile<NDM, 1, S>(L, it.ld, it.npairs, it.rho, it.naux, it.ndm, r0, r1, out, it.npairs,
+                                           e.a);
     }
 }
 
@@ -316,8 +320,8 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     MDF_CHECK_ARG(items && nitems >= 0 && out && work_bytes, "mdf_grouped_plan_create: null argument");
     MDF_CHECK_ARG(prec == MDF_PREC_FP64 || prec == MDF_PREC_FP32 || prec == MDF_PREC_BF16X3,
                   "mdf_grouped_plan_create: precision %d", prec);
-    MDF_CHECK_ARG(bn == 0 || bn == 64 || bn == 128 || bn == 160 || bn == 256 || bn == 320,
-                  "mdf_grouped_plan_create: bn %d not in {0,64,128,160,256,320}", bn);
+    MDF_CHECK_ARG(bn == 0 || bn == 64 || bn == 128 || bn == 160 || bn == 192 || bn == 256 || bn == 320,
+                  "mdf_grouped_plan_create: bn %d not in {0,64,128,160,192,256,320}", bn);
     for (int m = 0; m < nitems; m++)
         MDF_CHECK_ARG(items[m].cderi_dtype == items[0].cderi_dtype &&
                           (items[m].cderi_dtype == MDF_DTYPE_F64 || items[m].cderi_dtype == MDF_DTYPE_F32),
@@ -331,6 +335,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     p->dtype = nitems ? items[0].cderi_dtype : MDF_DTYPE_F64;
     const int wvec = p->dtype == MDF_DTYPE_F32 ? 4 : 2;     // values per 16-byte vector
     std::vector<Tile> j1, jr1, j2, jr2, k0, k1, k2, kf;
+    std::vector<Tile> j1c[GJ_CLASSES], j2c[GJ_CLASSES];
     std::vector<int> kgrid;             // step-2 output tiles per K item
     if (work_limit == 0) work_limit = MDF_GROUPED_DEFAULT_LIMIT;
     size_t off = 0;
@@ -361,11 +366,12 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
             g.part1 = (double *)take(pl.part1_words * sizeof(double));
             g.part2 = (double *)take(pl.part2_words * sizeof(double));
             const int ji = (int)p->hj.size();
+            const int jc = gj_class(x.n_dm);
             for (long c = 0; c < pl.nchunk; c++)
-                for (long r = 0; r < pl.nrblock; r++) j1.push_back({ji, (int)c, (int)r, 0});
+                for (long r = 0; r < pl.nrblock; r++) j1c[jc].push_back({ji, (int)c, (int)r, 0});
             for (long e = 0; e < cdiv((long)x.n_dm * x.naux, RED_CHUNK); e++) jr1.push_back({ji, (int)e, 0, 0});
             for (long g2 = 0; g2 < pl.ngroup; g2++)
-                for (long c = 0; c < pl.ncblock; c++) j2.push_back({ji, (int)c, (int)g2, 0});
+                for (long c = 0; c < pl.ncblock; c++) j2c[jc].push_back({ji, (int)c, (int)g2, 0});
             for (long e = 0; e < cdiv((long)x.n_dm * x.npairs, RED_CHUNK); e++) jr2.push_back({ji, (int)e, 0, 0});
             p->hj.push_back(g);
         }
@@ -454,10 +460,10 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
         const bool b3 = prec == MDF_PREC_BF16X3;
         for (long b = 0; b < maxpass; b++) {
           // one step-1 launch per tile width (and, for split-BF16, per pair-map flag)
-          for (int cls = 0; cls < (b3 ? 6 : 5); cls++) {
-            static const int kBN3[3] = {128, 256, 320};
-            static const int kBN1[5] = {64, 128, 160, 256, 320};
-            const int mp = b3 ? cls / 3 : 0, cbn = b3 ? kBN3[cls % 3] : kBN1[cls];
+          for (int cls = 0; cls < (b3 ? 8 : 6); cls++) {
+            static const int kBN3[4] = {128, 192, 256, 320};
+            static const int kBN1[6] = {64, 128, 160, 192, 256, 320};
+            const int mp = b3 ? cls / 4 : 0, cbn = b3 ? kBN3[cls % 4] : kBN1[cls];
             const long s1 = k1.size();
             for (size_t i = w0; i < w1; i++) {
                 const GKItem &g = p->hk[i];
@@ -512,6 +518,14 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     off = std::max(off, kmax);
     p->nj = (int)p->hj.size();
     p->nk = (int)p->hk.size();
+    for (int c = 0; c < GJ_CLASSES; c++) {
+        p->jc1[c] = (long)j1.size();
+        p->jc2[c] = (long)j2.size();
+        j1.insert(j1.end(), j1c[c].begin(), j1c[c].end());
+        j2.insert(j2.end(), j2c[c].begin(), j2c[c].end());
+    }
+    p->jc1[GJ_CLASSES] = (long)j1.size();
+    p->jc2[GJ_CLASSES] = (long)j2.size();
     p->t_j1 = j1.size(); p->t_jr1 = jr1.size(); p->t_j2 = j2.size(); p->t_jr2 = jr2.size();
     p->t_k0 = k0.size(); p->t_k1 = k1.size(); p->t_k2 = k2.size(); p->t_kf = kf.size();
     p->work_bytes = off + 256;
@@ -560,9 +574,39 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
     return MDF_OK;
 }
 
+// Pass PASS (1 or 2) of the grouped J over n tiles of density class c.
+// Persistent workgroups, eight per CU for one or two densities (few
+// registers; the loads need many waves in flight), four otherwise.
+template <int PASS, int NDM, typename S>
+static void launch_gj_t(const GJItem *items, const Tile *tiles, long n, char *w, hipStream_t st) {
+    const dim3 g(persistent_grid(n, NDM <= 2 ? 8 : 4)), b(jcore::J_THREADS);
+    if (PASS == 1)
+        hipLaunchKernelGGL((gj_pass1<NDM, S>), g, b, 0, st, items, tiles, n, w);
+    else
+        hipLaunchKernelGGL((gj_pass2<NDM, S>), g, b, 0, st, items, tiles, n, w);
+}
+
+template <int PASS, typename S>
+static void launch_gj_s(int c, const GJItem *items, const Tile *tiles, long n, char *w, hipStream_t st) {
+    switch (c) {
+    case 0: launch_gj_t<PASS, 1, S>(items, tiles, n, w, st); break;
+    case 1: launch_gj_t<PASS, 2, S>(items, tiles, n, w, st); break;
+    case 2: launch_gj_t<PASS, 4, S>(items, tiles, n, w, st); break;
+    default: launch_gj_t<PASS, GJ_NDM, S>(items, tiles, n, w, st); break;
+    }
+}
+
+template <int PASS>
+static void launch_gj(int c, int dtype, const GJItem *items, const Tile *tiles, long n, char *w, hipStream_t st) {
+    if (dtype == MDF_DTYPE_F32)
+        launch_gj_s<PASS, float>(c, items, tiles, n, w, st);
+    else
+        launch_gj_s<PASS, double>(c, items, tiles, n, w, st);
+}
+
 template <int BN, typename Src>
 static void launch_gk1_s(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
-    const int g = persistent_grid(n, BN >= 256 ? 1 : 2);
+    const int g = persistent_grid(n, BN >= 192 ? 1 : 2);
     const dim3 b(kcore::Step1Shape<BN>::THREADS);
     if (p->prec == MDF_PREC_FP32)
         hipLaunchKernelGGL((gk_step1<BN, float, Src>), dim3(g), b, 0, st, p->kitems, tiles, n, w);
@@ -580,7 +624,7 @@ static void launch_gk1(const mdf_grouped_plan *p, const Tile *tiles, long n, cha
 
 template <bool MAPPED, int BN>
 static void launch_gk1_b3(const mdf_grouped_plan *p, const Tile *tiles, long n, char *w, hipStream_t st) {
-    // one workgroup per CU (the 256- and 320-wide tiles hold 120-140 KB of LDS)
+    // one workgroup per CU (the 192-, 256- and 320-wide tiles hold 100-140 KB of LDS)
     const int per_cu = BN == 128 ? 2 : 1;
     const dim3 g(persistent_grid(n, per_cu)), b(kcore::kb3::Step1Cfg<BN>::THREADS);
     if (p->dtype == MDF_DTYPE_F32)
@@ -598,23 +642,21 @@ extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work
     hipStream_t st = as_stream(stream);
     char *w = (char *)work;
     if (p->with_j && p->nj) {
-        if (p->dtype == MDF_DTYPE_F32)
-            hipLaunchKernelGGL(gj_pass1<float>, dim3(persistent_grid(p->t_j1, 4)), dim3(jcore::J_THREADS), 0, st,
-                               p->jitems, p->j1, p->t_j1, w);
-        else
-            hipLaunchKernelGGL(gj_pass1<double>, dim3(persistent_grid(p->t_j1, 4)), dim3(jcore::J_THREADS), 0, st,
-                               p->jitems, p->j1, p->t_j1, w);
-        MDF_LAUNCHED("gj_pass1");
+        for (int c = 0; c < GJ_CLASSES; c++) {
+            const long n = p->jc1[c + 1] - p->jc1[c];
+            if (n == 0) continue;
+            launch_gj<1>(c, p->dtype, p->jitems, p->j1 + p->jc1[c], n, w, st);
+            MDF_LAUNCHED("gj_pass1");
+        }
         hipLaunchKernelGGL(gj_reduce1, dim3(persistent_grid(p->t_jr1, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
                            p->jr1, p->t_jr1, w);
         MDF_LAUNCHED("gj_reduce1");
-        if (p->dtype == MDF_DTYPE_F32)
-            hipLaunchKernelGGL(gj_pass2<float>, dim3(persistent_grid(p->t_j2, 4)), dim3(jcore::J_THREADS), 0, st,
-                               p->jitems, p->j2, p->t_j2, w);
-        else
-            hipLaunchKernelGGL(gj_pass2<double>, dim3(persistent_grid(p->t_j2, 4)), dim3(jcore::J_THREADS), 0, st,
-                               p->jitems, p->j2, p->t_j2, w);
-        MDF_LAUNCHED("gj_pass2");
+        for (int c = 0; c < GJ_CLASSES; c++) {
+            const long n = p->jc2[c + 1] - p->jc2[c];
+            if (n == 0) continue;
+            launch_gj<2>(c, p->dtype, p->jitems, p->j2 + p->jc2[c], n, w, st);
+            MDF_LAUNCHED("gj_pass2");
+        }
         hipLaunchKernelGGL(gj_reduce2, dim3(persistent_grid(p->t_jr2, 4)), dim3(jcore::J_THREADS), 0, st, p->jitems,
                            p->jr2, p->t_jr2, w);
         MDF_LAUNCHED("gj_reduce2");
@@ -634,6 +676,8 @@ extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work
                 case 641: launch_gk1_b3<true, 320>(p, t1, ph.count, w, st); break;
                 case 512: launch_gk1_b3<false, 256>(p, t1, ph.count, w, st); break;
                 case 513: launch_gk1_b3<true, 256>(p, t1, ph.count, w, st); break;
+                case 384: launch_gk1_b3<false, 192>(p, t1, ph.count, w, st); break;
+                case 385: launch_gk1_b3<true, 192>(p, t1, ph.count, w, st); break;
                 case 257: launch_gk1_b3<true, 128>(p, t1, ph.count, w, st); break;
                 default: launch_gk1_b3<false, 128>(p, t1, ph.count, w, st); break;
                 }
@@ -647,6 +691,7 @@ extern "C" int mdf_grouped_jk(const mdf_grouped_plan *p, void *work, size_t work
                 switch (ph.bn) {
                 case 320: launch_gk1<320>(p, t, ph.count, w, st); break;
                 case 256: launch_gk1<256>(p, t, ph.count, w, st); break;
+                case 192: launch_gk1<192>(p, t, ph.count, w, st); break;
                 case 160: launch_gk1<160>(p, t, ph.count, w, st); break;
                 case 128: launch_gk1<128>(p, t, ph.count, w, st); break;
                 default: launch_gk1<64>(p, t, ph.count, w, st); break;

@@ -149,6 +149,7 @@ int step1(const KPlan &p, int nL, hipStream_t st, const Src *blk, long ld, const
     switch (p.bn) {
     case 320: launch_step1<320, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     case 256: launch_step1<256, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 192: launch_step1<192, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     case 160: launch_step1<160, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     case 128: launch_step1<128, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     default: launch_step1<64, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
@@ -237,6 +238,10 @@ int run_k_b3(const KPlan &p, const Src *cderi, long naux, long ld, const int *pa
                 break;
             case 256:
                 launch_step1_b3<256>(nL, st, blk, ld, pair_map, nao, q.fthi, q.ftlo, y.ldft, nocc, q.rhi, q.rlo,
+                                     y.ldr, strideL);
+                break;
+            case 192:
+                launch_step1_b3<192>(nL, st, blk, ld, pair_map, nao, q.fthi, q.ftlo, y.ldft, nocc, q.rhi, q.rlo,
                                      y.ldr, strideL);
                 break;
             default:

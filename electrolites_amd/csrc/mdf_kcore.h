@@ -85,7 +85,7 @@ struct PackedALoader {
 // per 128 or 160).
 template <int BN>
 struct Step1Shape {
-    static constexpr int WGN = BN >= 256 ? 4 : 2;
+    static constexpr int WGN = BN >= 192 ? 4 : 2;
     static constexpr int THREADS = 128 * WGN;
     using S = TileShape<128, BN, 2, WGN>;
 };
@@ -233,7 +233,7 @@ __device__ __forceinline__ void finalize_range(const double *__restrict__ part, 
 // columns of MFMA work), so minimise nkt * (64 + BN) over the widths; ties
 // go to the wider tile
 inline int pick_bn(int nocc) {
-    const int cand[5] = {64, 128, 160, 256, 320};
+    const int cand[6] = {64, 128, 160, 192, 256, 320};
     int best = 64;
     long best_cost = 1L << 40;
     for (int b : cand) {
@@ -403,8 +403,8 @@ struct FTLoader {
     __device__ __forceinline__ void store(const uint4 (&r)[N], u16 *h, u16 *l) const { P::put(r, h, l); }
 };
 
-// Step-1 tile widths: 128 columns on 2 x 2 wavefronts, 256 or 320 on 2 x 4
-// (one workgroup then covers up to 320 occupied columns, so each cderi tile
+// Step-1 tile widths: 128 columns on 2 x 2 wavefronts, 192, 256 or 320 on
+// 2 x 4 (one workgroup then covers up to 320 occupied columns, so each cderi tile
 // is read once per factor instead of once per 128 columns).
 template <int BN>
 struct Step1Cfg {
@@ -416,7 +416,7 @@ struct Step1Cfg {
 inline int pick_bn(long ldr) {
     int best = 128;
     long pad = -1;
-    for (int bn : {128, 256, 320}) {
+    for (int bn : {128, 192, 256, 320}) {
         const long p = (ldr + bn - 1) / bn * bn - ldr;
         if (pad < 0 || p < pad || (p == pad && bn > best)) {
             best = bn;

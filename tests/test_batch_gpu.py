@@ -21,7 +21,7 @@ def make_batch(seed=0):
     dfs, dms = [], []
     # (nao, naux, nocc, n_dm, tagged, screened)
     spec = [(37, 60, 5, 1, True, False), (72, 330, 15, 2, True, False), (129, 500, 40, 3, False, False),
-            (90, 150, 12, 1, True, True), (161, 610, 45, 1, True, False)]
+            (90, 150, 12, 1, True, True), (161, 610, 45, 1, True, False), (210, 260, 170, 1, True, False)]
     for nao, naux, nocc, n_dm, tagged, screened in spec:
         pa, _ = O.pair_addresses(nao)
         if screened:

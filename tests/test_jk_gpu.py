@@ -113,7 +113,7 @@ def test_j_only_branch(water2):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('nao,naux,nocc', [(37, 19, 5), (64, 130, 17),
-                                           (129, 300, 64), (200, 611, 90)])
+                                           (129, 300, 64), (200, 611, 90), (240, 300, 170)])
 def test_synthetic_shapes(nao, naux, nocc):
     """Ragged shapes: odd npairs (scalar loads), tile tails, nocc not a
     multiple of the MFMA tile."""
@@ -188,7 +188,8 @@ def test_device_in_device_out(water2):
 @pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
 @pytest.mark.parametrize('nao,naux,nocc,mode,screened', [(64, 130, 17, 0, False), (200, 611, 90, 0, False),
                                                          (200, 300, 45, 1, False), (170, 260, 33, 0, True),
-                                                         (150, 200, 40, 1, True)])
+                                                         (150, 200, 40, 1, True), (230, 280, 170, 0, False),
+                                                         (200, 260, 161, 1, True)])
 def test_low_precision_k_error(precision, nao, naux, nocc, mode, screened):
     """FP32 and split-BF16 exchange against the FP64 oracle: one factor
     (mo-tagged) and two factors (untagged), full and screened pair sets,
