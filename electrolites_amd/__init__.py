@@ -7,7 +7,11 @@ over the two entry points the reference replaces (electrolites/__init__.py
     gpu4pyscf.df.df_jk.get_jk        -> electrolites_amd.fastdf.get_jk
     gpu4pyscf.df.df._cholesky_eri    -> electrolites_amd.fastdfcderi._cholesky_eri
     gpu4pyscf.df.df_jk.get_j         -> electrolites_amd.fastdfj3c.get_j
+    gpu4pyscf.df.df.DF.loop          -> electrolites_amd.fastdffp32.patched_loop
 
+('fastdffp32', as the reference's entry of that name (:164), gives the
+block iterator the tensors this build writes: FP32 storage widened, host
+residency streamed.)
 (the reference's 'fastdfj3c' entry patches the two Int3c2eOpt contractions
 under ``df_jk.get_j``, :127-132; this build replaces the whole J-without-a-
 cderi function, whose contractions are its own kernels)
@@ -33,8 +37,9 @@ _TARGETS = {
     'fastdf': (('gpu4pyscf.df.df_jk', 'get_jk'),),
     'fastdfcderi': (('gpu4pyscf.df.df', '_cholesky_eri'),),
     'fastdfj3c': (('gpu4pyscf.df.df_jk', 'get_j'),),
+    'fastdffp32': (('gpu4pyscf.df.df', 'DF.loop'),),
 }
-MODULES = ('fastdf', 'fastdfcderi', 'fastdfj3c')
+MODULES = ('fastdf', 'fastdfcderi', 'fastdfj3c', 'fastdffp32')
 _patched = []
 
 
@@ -75,6 +80,12 @@ def _install(name):
         if getattr(fastdfj3c, '_orig_get_j', None) is None:
             fastdfj3c._orig_get_j = mod.get_j
         mod.get_j = fastdfj3c.get_j
+    elif name == 'fastdffp32':
+        from . import fastdffp32
+        mod = importlib.import_module('gpu4pyscf.df.df')
+        if fastdffp32._orig_loop is None:
+            fastdffp32._orig_loop = mod.DF.loop
+        mod.DF.loop = fastdffp32.patched_loop
 
 
 def patch(*names, strict=True):

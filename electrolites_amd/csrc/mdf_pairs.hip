@@ -121,6 +121,40 @@ extern "C" int mdf_cast_f32(const double *src, long lds, long rows, long cols, f
     return MDF_OK;
 }
 
+namespace {
+// one row per blockIdx.y (grid-strided over rows), columns grid-strided over
+// blockIdx.x: no per-element division, coalesced on both sides
+__global__ __launch_bounds__(256) void widen_f32_kernel(const float *__restrict__ src, long lds, long rows, long cols,
+                                                        double *__restrict__ dst, long ldd) {
+    for (long r = blockIdx.y; r < rows; r += gridDim.y) {
+        const float *s = src + r * lds;
+        double *d = dst + r * ldd;
+        for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < cols; c += (long)gridDim.x * 256) d[c] = (double)s[c];
+    }
+}
+}  // namespace
+
+extern "C" int mdf_widen_f32(const float *src, long lds, long rows, long cols, double *dst, long ldd,
+                             void *stream) {
+    MDF_CHECK_ARG(rows >= 0 && cols >= 0 && lds >= cols && ldd >= cols, "mdf_widen_f32: bad shape");
+    MDF_CHECK_ARG((src && dst) || rows * cols == 0, "mdf_widen_f32: null pointer");
+    if (rows * cols == 0) return MDF_OK;
+    const dim3 g((unsigned)std::min(mdf::cdiv(cols, 256L), 64L), (unsigned)std::min(rows, 4096L));
+    hipLaunchKernelGGL(widen_f32_kernel, g, dim3(256), 0, mdf::as_stream(stream), src, lds, rows, cols, dst, ldd);
+    MDF_LAUNCHED("widen_f32_kernel");
+    return MDF_OK;
+}
+
+extern "C" int mdf_copy_2d(void *dst, long dpitch, const void *src, long spitch, long width, long rows,
+                           void *stream) {
+    MDF_CHECK_ARG(rows >= 0 && width >= 0 && dpitch >= width && spitch >= width, "mdf_copy_2d: bad shape");
+    MDF_CHECK_ARG((src && dst) || rows * width == 0, "mdf_copy_2d: null pointer");
+    if (rows * width == 0) return MDF_OK;
+    MDF_HIP(hipMemcpy2DAsync(dst, (size_t)dpitch, src, (size_t)spitch, (size_t)width, (size_t)rows, hipMemcpyDefault,
+                             mdf::as_stream(stream)));
+    return MDF_OK;
+}
+
 extern "C" int mdf_fill_zero(double *x, long n, void *stream) {
     MDF_CHECK_ARG(x || n == 0, "mdf_fill_zero: null pointer");
     if (n == 0) return MDF_OK;

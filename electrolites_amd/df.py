@@ -6,6 +6,11 @@ GPU4PySCF is absent: ``build()`` runs the MI355X cderi build
 ``_cderi = {device_id: cderi}`` and ``_cderi_idx``; ``get_jk`` is
 :func:`electrolites_amd.fastdf.get_jk` (``DF.get_jk`` -> ``df_jk.get_jk``,
 gpu4pyscf/df/df.py:105-117 as the reference describes its caller).
+
+``storage='host'`` builds the cderi into page-locked host memory (for a
+tensor larger than the device); J/K then keep a prefix of its rows on the
+device and stream the rest (:mod:`electrolites_amd.dfstore`).  ``loop`` is
+the block iterator GPU4PySCF's other consumers use (``DF.loop``).
 """
 from . import _dev
 from . import fastdf
@@ -13,8 +18,9 @@ from . import fastdfcderi
 
 
 class DF:
-    def __init__(self, mol, auxbasis=None, verbose=0):
+    def __init__(self, mol, auxbasis=None, verbose=0, storage='device'):
         self.mol = mol
+        self.storage = storage
         self.auxbasis = auxbasis or getattr(mol, 'auxbasis', None)
         self.verbose = verbose
         self._cderi = None
@@ -32,6 +38,10 @@ class DF:
         return self
 
     def reset(self):
+        from . import dfstore
+        for c in (self._cderi or {}).values():
+            dfstore.release(c)
+        self.__dict__.pop('_mdf_resident', None)
         self._cderi = None
         self._cderi_idx = None
         self.__dict__.pop('_mdf_layout', None)
@@ -39,6 +49,18 @@ class DF:
 
     def get_jk(self, dm, hermi=1, with_j=True, with_k=True, omega=None):
         return fastdf.get_jk(self, dm, hermi, with_j, with_k, omega)
+
+    def loop(self, blksize=None, unpack=True):
+        """Blocks of aux rows as FP64 device tensors: ``(unpacked, packed)``
+        with ``unpacked`` the ``(nL, nao, nao)`` symmetric matrices (or None)
+        and ``packed`` the ``(nL, npairs)`` rows (``DF.loop``;
+        :func:`electrolites_amd.fastdffp32.loop`)."""
+        from . import fastdffp32
+        return fastdffp32.loop(self, blksize, unpack)
+
+    @property
+    def nao(self):
+        return self.mol.nao
 
     def get_j(self, dm, hermi=1):
         """J without the cderi: three-centre integrals contracted on the fly

@@ -27,12 +27,14 @@ replaced, or, when there is none (standalone use), raised as
 :class:`Declined`.  A float32 cderi (the FP32 storage mode) is covered: its
 values are widened to FP64 before arithmetic.
 """
+import ctypes
 import os
 
 import numpy as np
 
 from . import _dev
 from . import _lib
+from . import dfstore
 
 STATS = {'calls': 0, 'declined': 0, 'j_only': 0, 'mode0': 0, 'mode1': 0,
          'k_calls': 0}
@@ -258,6 +260,61 @@ def _vk(cderi, lay, fl, fr, precision='fp64'):
     return vk
 
 
+def _jk_host(dfobj, cderi, lay, dms3, hermi, with_j, fl, fr, precision):
+    """J and K over a host-resident cderi (electrolites/fastdf.py:805-1026
+    with ``_blocks`` :611-671): the device prefix of its rows
+    (:func:`dfstore.resident_prefix`, kept between calls) and the rest
+    streamed in blocks, each block contracted once for both -- ``mdf_get_j``
+    adding its rows' share of vj (and writing their rho), ``mdf_get_k``
+    adding their share of vk -- so the tensor crosses PCIe once per call.
+    ``fl`` None: J only."""
+    t = _dev.torch()
+    n, nao = dms3.shape[0], dms3.shape[-1]
+    naux, npairs = cderi.shape
+    elem = cderi.element_size()
+    dt = _dtype_code(cderi)
+    blk_rows = dfstore.block_rows(npairs, elem, naux)
+    with_k = fl is not None
+    jbytes = _lib.query('mdf_get_j_workspace', blk_rows, npairs, n) if with_j else 0
+    kbytes, prec, two, nocc = 0, PRECISIONS[precision], False, 0
+    if with_k:
+        nocc, two = fl.shape[-1], fr is not None
+        per_row = nao * nocc * (8 if prec == 0 else 4) * (2 if two else 1)
+        rows = max(1, min(blk_rows, K_WORKSPACE_BYTES // max(per_row, 1)))
+        kbytes = _lib.query('mdf_get_k_workspace', blk_rows, nao, nocc, int(two), prec, rows)
+    nbytes = max(jbytes, kbytes)
+    # what this call allocates beside the prefix: two streamed blocks, the
+    # workspace, the pair vectors and the outputs
+    reserve = 2 * blk_rows * npairs * elem + nbytes + 8 * n * (2 * npairs + naux + 2 * nao * nao)
+    resident = dfstore.resident_prefix(dfobj, cderi, reserve)
+    work = _dev.WORKSPACE.get(nbytes)
+    if with_j:
+        d = _sparse_dm(dms3, lay, hermi)
+        rho = t.empty((n, naux), dtype=t.float64, device=dms3.device)
+        vjp = t.zeros((n, npairs), dtype=t.float64, device=dms3.device)
+    if with_k:
+        vk = t.zeros((n, nao, nao), dtype=t.float64, device=dms3.device)
+    STATS['host_calls'] = STATS.get('host_calls', 0) + 1
+    for blk, L0, nL in dfstore.blocks(cderi, resident, blk_rows):
+        ld = blk.stride(0)
+        if with_j:
+            _lib.call('mdf_get_j', _dev.ptr(blk), dt, nL, npairs, ld, _dev.ptr(d), n, npairs,
+                      ctypes.c_void_p(rho.data_ptr() + 8 * L0), naux, _dev.ptr(vjp), npairs, 1,
+                      _dev.ptr(work), nbytes, _dev.stream_ptr())
+        if with_k:
+            for i in range(n):
+                STATS['k_calls'] += 1
+                _lib.call('mdf_get_k', _dev.ptr(blk), dt, nL, npairs, ld, _dev.ptr(lay.map), nao,
+                          _dev.ptr(fl[i]), _dev.ptr(fr[i]) if two else None, nocc, nocc, prec,
+                          _dev.ptr(vk[i]), nao, 1, _dev.ptr(work), nbytes, _dev.stream_ptr())
+    vj = None
+    if with_j:
+        vj = t.empty((n, nao, nao), dtype=t.float64, device=dms3.device)
+        _lib.call('mdf_unpack_pairs', _dev.ptr(vjp), n, npairs, _dev.ptr(lay.addr),
+                  npairs, nao, _dev.ptr(vj), nao * nao, _dev.stream_ptr())
+    return vj, (vk if with_k else None)
+
+
 def _finish(x, dms, out_device):
     x = x.reshape(tuple(np.shape(dms)))
     if out_device:
@@ -294,6 +351,9 @@ def _get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
     if getattr(dfobj, '_cderi', None) is None:
         dfobj.build()
     cderi = _cderi_of(dfobj)
+    host = dfstore.is_host(cderi)
+    if host:
+        cderi = dfstore.as_tensor(cderi)
     if cderi.dtype not in (t.float64, t.float32):
         return _fall_through(dfobj, dms, hermi, with_j, with_k, omega,
                              'cderi neither float64 nor float32')
@@ -315,7 +375,10 @@ def _get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
 
     if not with_k:
         STATS['j_only'] += 1
-        vj = _vj(cderi, lay, dms3, hermi) if with_j else None
+        if host:
+            vj, _ = _jk_host(dfobj, cderi, lay, dms3, hermi, with_j, None, None, 'fp64')
+        else:
+            vj = _vj(cderi, lay, dms3, hermi) if with_j else None
         return (None if vj is None else _finish(vj, dms, out_device)), None
 
     fl, fr = factorize_dm(dms, hermi)
@@ -325,12 +388,15 @@ def _get_jk(dfobj, dms, hermi=0, with_j=True, with_k=True, omega=None):
     if fl.shape[0] != dms3.shape[0]:
         raise ValueError(f'{fl.shape[0]} density factors for '
                          f'{dms3.shape[0]} densities')
-    vj = _vj(cderi, lay, dms3, hermi) if with_j else None
+    vj = _vj(cderi, lay, dms3, hermi) if with_j and not host else None
     prec = getattr(dfobj, 'k_precision', 'fp64') or 'fp64'
     if prec not in PRECISIONS:
         raise ValueError(f'k_precision {prec!r}: expected one of {sorted(PRECISIONS)}')
     STATS['k_' + prec] = STATS.get('k_' + prec, 0) + 1
-    vk = _vk(cderi, lay, fl, fr, prec)
+    if host:
+        vj, vk = _jk_host(dfobj, cderi, lay, dms3, hermi, with_j, fl, fr, prec)
+    else:
+        vk = _vk(cderi, lay, fl, fr, prec)
     return ((None if vj is None else _finish(vj, dms, out_device)),
             _finish(vk, dms, out_device))
 

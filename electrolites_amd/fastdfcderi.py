@@ -131,12 +131,15 @@ def _chunks(shells, naux, budget):
     return out
 
 
-def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None):
+def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'):
     """Build the cderi of ``mol`` (orbital basis ``mol.ao``, fitting basis
     ``mol.aux``) on the current device.  Returns ``(cderi, (pair_addr,
     diag_idx))``.  ``timer`` (a :class:`_Timer`) collects per-stage device
     times.  ``dtype`` 'float64' or 'float32' (FP32 storage; default from
-    ``FASTDF_FP32``)."""
+    ``FASTDF_FP32``).  ``storage`` 'device' (the tensor in HBM) or 'host'
+    (a page-locked host tensor written chunk by chunk through
+    :class:`electrolites_amd.dfstore.HostStore`, for a cderi larger than
+    the device; electrolites/_dfstore.py)."""
     _dev.require_gpu()
     t = _dev.torch()
     tm = timer or _NoTimer()
@@ -146,7 +149,11 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None):
     npairs = nao * (nao + 1) // 2
     linv, coef = decompose_j2c(aux, timer)
     nrow = naux if coef is None else coef.shape[0]
-    cderi = t.empty((nrow, npairs), dtype=t.float32 if fp32 else t.float64, device=_dev.device())
+    if storage not in ('device', 'host'):
+        raise ValueError(f'storage {storage!r}: expected device or host')
+    on_host = storage == 'host'
+    sdt = t.float32 if fp32 else t.float64
+    cderi = None if on_host else t.empty((nrow, npairs), dtype=sdt, device=_dev.device())
     bound = getattr(mol, '_mdf_pair_bound', None)
     if bound is None:
         bound = ints.pair_bound(mol.ao)
@@ -160,6 +167,10 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None):
                  for I0, I1 in chunks)
     buf = t.empty(naux * widest, dtype=t.float64, device=_dev.device())
     out64 = t.empty(nrow * widest, dtype=t.float64, device=_dev.device()) if fp32 else None
+    store = None
+    if on_host:
+        from . import dfstore
+        store = dfstore.HostStore(nrow, npairs, sdt, widest)
     for I0, I1 in chunks:
         tasks = ints.PairTasks(mol.ao, I0, I1, bound)
         STATS['pairs_kept'] += tasks.npairs_kept
@@ -170,9 +181,12 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None):
         _lib.call('mdf_fill_zero', _dev.ptr(blk), naux * width, _dev.stream_ptr())
         ints.int3c2e_chunk(ao, aux, tasks, blk, width)
         tm.mark('int3c2e:1')
-        dst = cderi[:, tasks.p0:]
+        if on_host:
+            dst, ldd = store.slot(width), width
+        else:
+            dst, ldd = cderi[:, tasks.p0:], npairs
         # FP64 result straight into the tensor, or into a chunk then rounded
-        out, ldo = (out64, width) if fp32 else (dst, npairs)
+        out, ldo = (out64, width) if fp32 else (dst, ldd)
         tm.mark('trmm:0')
         if coef is None:
             _lib.call('mdf_trmm', _dev.ptr(linv), naux, naux, _dev.ptr(blk), width, width,
@@ -182,9 +196,14 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None):
                       _dev.ptr(blk), width, 0., _dev.ptr(out), ldo, _dev.stream_ptr())
         tm.mark('trmm:1')
         if fp32:
-            _lib.call('mdf_cast_f32', _dev.ptr(out64), width, nrow, width, _dev.ptr(dst), npairs,
+            _lib.call('mdf_cast_f32', _dev.ptr(out64), width, nrow, width, _dev.ptr(dst), ldd,
                       _dev.stream_ptr())
+        if on_host:
+            store.write(dst, tasks.p0)
         STATS['chunks'] += 1
+    if on_host:
+        cderi = store.finish()
+        STATS['host_builds'] = STATS.get('host_builds', 0) + 1
     STATS['builds'] += 1
     rows, cols = np.tril_indices(nao)
     pair_addr = (rows * nao + cols).astype(np.int32)
@@ -219,9 +238,9 @@ def _cholesky_eri(intopt, omega=None, use_gpu_memory=None):
     if omega not in (None, 0, 0.):
         raise NotImplementedError('range-separated cderi (omega != 0) is not '
                                   'covered by the MI355X build')
-    if use_gpu_memory is False:
-        raise NotImplementedError('host-resident cderi is not covered: the '
-                                  'tensor stays in HBM')
     mol = getattr(intopt, 'mol', intopt)
-    cderi, idx = cholesky_eri(mol)
+    # use_gpu_memory False: GPU4PySCF's host-resident tensor (df/df.py), here
+    # a page-locked host tensor the J/K calls stream from
+    storage = 'host' if use_gpu_memory is False else getattr(intopt, 'storage', 'device')
+    cderi, idx = cholesky_eri(mol, storage=storage)
     return [cderi], idx

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 4
+#define MDF_ABI_VERSION 5
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
@@ -167,6 +167,16 @@ int mdf_fill_zero(double *x, long n, void *stream);
  * FP32 store of a cderi chunk, as the reference's FASTDF_FP32 build casts
  * each FP64 block on its way into the tensor (electrolites/_dffp32.py:67-198). */
 int mdf_cast_f32(const double *src, long lds, long rows, long cols, float *dst, long ldd, void *stream);
+/* dst[r * ldd + c] = (double)src[r * lds + c]: the widening of one block of
+ * an FP32-stored cderi for the block iterator's consumers, which take FP64
+ * blocks (electrolites/_dffp32.py:166-196 BlockWidener, used by
+ * electrolites/fastdffp32.py:72-141 DF.loop). */
+int mdf_widen_f32(const float *src, long lds, long rows, long cols, double *dst, long ldd, void *stream);
+/* rows x width bytes from src (row pitch spitch bytes) to dst (pitch dpitch)
+ * on the stream, either side device or page-locked host memory: the host
+ * store's chunk write-out and block prefetch (electrolites/_dfstore.py, the
+ * cudaMemcpy2DAsync of its 'stage' mode; electrolites/fastdf.py:611-671). */
+int mdf_copy_2d(void *dst, long dpitch, const void *src, long spitch, long width, long rows, void *stream);
 
 /* ---- Gaussian integrals (Rys quadrature, one kernel per angular class) --
  * A basis is a set of contracted shells; all arrays are DEVICE memory:

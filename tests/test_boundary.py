@@ -29,9 +29,14 @@ def fake_gpu4pyscf(monkeypatch):
         calls.append(('get_j', hermi))
         return 'orig-vj'
 
+    def orig_loop(self, blksize=None, unpack=True):
+        calls.append(('loop', blksize))
+        yield 'orig-block'
+
     df_jk.get_jk = orig_get_jk
     df_jk.get_j = orig_get_j
     df_mod._cholesky_eri = orig_cholesky_eri
+    df_mod.DF = type('DF', (), {'loop': orig_loop})
     root.df = df_pkg
     df_pkg.df_jk = df_jk
     df_pkg.df = df_mod
@@ -39,21 +44,31 @@ def fake_gpu4pyscf(monkeypatch):
                       ('gpu4pyscf.df.df_jk', df_jk), ('gpu4pyscf.df.df', df_mod)]:
         monkeypatch.setitem(sys.modules, name, mod)
     import electrolites_amd
-    from electrolites_amd import fastdf, fastdfcderi, fastdfj3c
+    from electrolites_amd import fastdf, fastdfcderi, fastdfj3c, fastdffp32
     monkeypatch.setattr(fastdf, '_orig_get_jk', None)
+    monkeypatch.setattr(fastdffp32, '_orig_loop', None)
     monkeypatch.setattr(fastdfcderi, '_orig_cholesky_eri', None, raising=False)
     monkeypatch.setattr(fastdfj3c, '_orig_get_j', None, raising=False)
     monkeypatch.setattr(electrolites_amd, '_patched', [])
     return types.SimpleNamespace(df_jk=df_jk, df=df_mod, calls=calls, orig_get_jk=orig_get_jk,
-                                 orig_cholesky_eri=orig_cholesky_eri, orig_get_j=orig_get_j)
+                                 orig_cholesky_eri=orig_cholesky_eri, orig_get_j=orig_get_j,
+                                 orig_loop=orig_loop)
 
 
 def test_patch_installs_and_keeps_originals(fake_gpu4pyscf):
     import electrolites_amd
     from electrolites_amd import fastdf, fastdfcderi
     from electrolites_amd import fastdfj3c
-    assert electrolites_amd.check_targets() == {'fastdf': None, 'fastdfcderi': None, 'fastdfj3c': None}
-    assert electrolites_amd.patch() == ('fastdf', 'fastdfcderi', 'fastdfj3c')
+    from electrolites_amd import fastdffp32
+    assert electrolites_amd.check_targets() == {'fastdf': None, 'fastdfcderi': None, 'fastdfj3c': None,
+                                                'fastdffp32': None}
+    assert electrolites_amd.patch() == ('fastdf', 'fastdfcderi', 'fastdfj3c', 'fastdffp32')
+    assert fake_gpu4pyscf.df.DF.loop is fastdffp32.patched_loop
+    assert fastdffp32._orig_loop is fake_gpu4pyscf.orig_loop
+    # a tensor GPU4PySCF built itself (not a torch tensor) goes to its own loop
+    obj = fake_gpu4pyscf.df.DF()
+    obj._cderi = {0: np.zeros((3, 6))}
+    assert list(obj.loop(2)) == ['orig-block'] and fake_gpu4pyscf.calls[-1] == ('loop', 2)
     assert fake_gpu4pyscf.df_jk.get_jk is fastdf.get_jk
     assert fake_gpu4pyscf.df_jk.get_j is fastdfj3c.get_j
     assert fastdfj3c._orig_get_j is fake_gpu4pyscf.orig_get_j
@@ -87,7 +102,7 @@ def test_missing_target_is_reported(fake_gpu4pyscf, monkeypatch):
     assert rep['fastdf'] is None and 'missing' in rep['fastdfcderi']
     with pytest.raises(RuntimeError):
         electrolites_amd.patch()
-    assert electrolites_amd.patch(strict=False) == ('fastdf', 'fastdfj3c')
+    assert electrolites_amd.patch(strict=False) == ('fastdf', 'fastdfj3c', 'fastdffp32')
 
 
 def test_standalone_decline_raises(monkeypatch):
