@@ -48,6 +48,9 @@ def parse():
     ap.add_argument('--requests', type=int, default=24, help='serve: requests per rate')
     ap.add_argument('--rates', default='0.5,2,8', help='serve: Poisson rates (req/s)')
     ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--no-host', action='store_true', help='getjk: skip the host-resident (streamed) line')
+    ap.add_argument('--host-frac', type=float, default=.25,
+                    help='getjk: share of the aux rows moved to page-locked host memory for the streamed line')
     ap.add_argument('--graphs', action='store_true', help='serve: replay grouped runs as HIP graphs')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='getjk: first run a plain device copy of this many MiB (the reference '
@@ -436,6 +439,63 @@ def main():
         del vjr, mdf
     del vj64
 
+    # out-of-HBM residency: a leading share of the aux rows moved to
+    # page-locked host memory and streamed through get_jk with no device
+    # prefix, against the same rows resident in HBM
+    host_line = None
+    if not args.no_host:
+        from electrolites_amd import dfstore
+        rows = max(1, int(naux * args.host_frac))
+        dev_part = cderi[:rows]
+        t0 = time.perf_counter()
+        hpart = dfstore.to_host(dev_part)
+        t_pin = time.perf_counter() - t0
+
+        class HostDF:
+            _cderi = {torch.cuda.current_device(): hpart}
+            _cderi_idx = None
+
+            def build(self):
+                pass
+
+        class DevDF:
+            _cderi = {torch.cuda.current_device(): dev_part}
+            _cderi_idx = None
+
+            def build(self):
+                pass
+
+        hdf, ddf = HostDF(), DevDF()
+        old = dfstore.RESIDENT_FRAC
+        dfstore.RESIDENT_FRAC = 0.
+        try:
+            fastdf.get_jk(hdf, dm, hermi=1)
+            torch.cuda.synchronize()
+            b0 = dfstore.stats()['streamed_bytes']
+            t0 = time.perf_counter()
+            hj, hk = fastdf.get_jk(hdf, dm, hermi=1)
+            torch.cuda.synchronize()
+            t_host = time.perf_counter() - t0
+            streamed = dfstore.stats()['streamed_bytes'] - b0
+        finally:
+            dfstore.RESIDENT_FRAC = old
+        fastdf.get_jk(ddf, dm, hermi=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dj, dk = fastdf.get_jk(ddf, dm, hermi=1)
+        torch.cuda.synchronize()
+        t_dev = time.perf_counter() - t0
+        host_line = {'rows': rows, 'GB': rows * npairs * 8 / 1e9, 'streamed_GB': streamed / 1e9,
+                     'host_ms': t_host * 1e3, 'device_ms': t_dev * 1e3,
+                     'stream_GBps': streamed / t_host / 1e9, 'copy_to_host_s': t_pin,
+                     'block_MB': dfstore.STREAM_BLOCK_BYTES / 1e6,
+                     'rel_diff_vs_device': max(float((hj - dj).abs().max() / dj.abs().max()),
+                                               float((hk - dk).abs().max() / dk.abs().max()))}
+        log(f'host-resident get_jk over {rows} aux rows ({host_line["GB"]:.1f} GB streamed): '
+            f'{t_host * 1e3:.0f} ms = {host_line["stream_GBps"]:.1f} GB/s over PCIe; the same rows in HBM '
+            f'{t_dev * 1e3:.1f} ms; rel diff {host_line["rel_diff_vs_device"]:.1e}')
+        del hj, hk, dj, dk, hdf, ddf, hpart, dev_part
+
     S = naux * npairs * 8
     bj = 2 * S + 16 * (npairs + naux)
     traffic = measured_traffic(args.nwater) if args.nocc == 320 else {}
@@ -461,6 +521,7 @@ def main():
         'cderi_build': build,
         'fp32_storage': fp32_store,
         'ri_j_no_cderi': ri_j,
+        'host_resident': host_line,
         'roofline': {'bound': 'mfma', 'achieved': fk / tk / 1e12,
                      'peak': FP64_MFMA_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': fk / tk / FP64_MFMA_PEAK, 'traffic': traffic.get('k'),
