@@ -64,7 +64,7 @@ struct GKItem {
     int nao, nocc, ksplit, ntile, vec;
     // split-BF16 (prec 2): workspace offsets of [side][FT hi, FT lo, R hi, R lo]
     void *b3[2][4];
-    long ldr, ldft;
+    long ldr, ldft;         // R row stride (FP64/FP32: kcore::r_ld; split-BF16: a multiple of 32), FT row stride
     int bn3;                // split-BF16 step-1 tile width (kb3::pick_bn(ldr))
     int bn1;                // FP64/FP32 step-1 tile width (the plan's bn, or kcore::pick_bn(nocc))
 };
@@ -184,13 +184,13 @@ __global__ __launch_bounds__(kcore::Step1Shape<BN>::THREADS) void gk_step1(const
         const int ktile = e.c & 0xffff;
         const double *F = right ? it.fr : it.fl;
         const long lb = e.a % it.nL;        // row within the pass
-        T *R = (T *)(work + (size_t)(right ? it.rr : it.rl)) + lb * it.nao * it.nocc;
+        T *R = (T *)(work + (size_t)(right ? it.rr : it.rl)) + lb * it.nao * it.ldr;
         const Src *row = (const Src *)it.cderi + (long)e.a * it.ld;
         if (it.pmap)
-            kcore::step1_tile<BN, true, T, Src>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
+            kcore::step1_tile<BN, true, T, Src>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.ldr, e.b * 128,
                                                 ktile * BN, lds);
         else
-            kcore::step1_tile<BN, false, T, Src>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.nocc, e.b * 128,
+            kcore::step1_tile<BN, false, T, Src>(row, it.pmap, it.nao, F, it.nocc, it.nocc, R, it.ldr, e.b * 128,
                                                  ktile * BN, lds);
         __syncthreads();
     }
@@ -219,10 +219,10 @@ __global__ __launch_bounds__(G_THREADS, 2) void gk_step2(const GKItem *items, co
         const T *rr = it.fr ? (const T *)(work + (size_t)it.rr) : rl;
         double *o = (double *)(work + (size_t)it.part) + (long)e.b * it.nao * it.nao;
         if (it.vec)
-            kcore::step2_tile<true, T>(rl, rr, it.nao, it.nocc, it.nocc, (long)it.nao * it.nocc, ti * kcore::S2_B,
+            kcore::step2_tile<true, T>(rl, rr, it.nao, it.nocc, it.ldr, (long)it.nao * it.ldr, ti * kcore::S2_B,
                                        tj * kcore::S2_B, s0, s1, o, e.c > 0, lds);
         else
-            kcore::step2_tile<false, T>(rl, rr, it.nao, it.nocc, it.nocc, (long)it.nao * it.nocc, ti * kcore::S2_B,
+            kcore::step2_tile<false, T>(rl, rr, it.nao, it.nocc, it.ldr, (long)it.nao * it.ldr, ti * kcore::S2_B,
                                         tj * kcore::S2_B, s0, s1, o, e.c > 0, lds);
         __syncthreads();
     }
@@ -385,7 +385,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
             const bool b3 = prec == MDF_PREC_BF16X3;
             const long nL = b3 ? kcore::block_rows(x.naux, x.nao, (int)kcore::kb3::pad32(x.nocc), two, 4)
                                : kcore::block_rows(x.naux, x.nao, x.nocc, two, elem);
-            kcore::KPlan kp = kcore::plan_k(x.naux, x.nao, x.nocc, two, nL);
+            kcore::KPlan kp = kcore::plan_k(x.naux, x.nao, x.nocc, two, nL, b3 ? sizeof(double) : elem);
             for (int d = 0; d < x.n_dm; d++) {
                 GKItem g{};
                 g.cderi = x.cderi; g.ld = x.ld; g.naux = x.naux; g.pmap = x.pair_map;
@@ -395,7 +395,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                 g.nao = x.nao; g.nocc = x.nocc; g.ksplit = kp.ksplit; g.ntile = kp.ntile;
                 g.nL = kp.nL;
                 g.bn1 = bn ? bn : kcore::pick_bn(x.nocc);
-                g.vec = (x.nocc % (16 / (int)elem)) == 0;
+                g.vec = 1;             // R rows padded to whole 16-byte vectors (kcore::r_ld)
                 // workspace offsets are assigned per wave below; keep the sizes
                 if (b3) {
                     const kcore::kb3::Layout y = kcore::kb3::layout(x.nao, x.nocc, kp.nL, kp.ksplit, two);
@@ -405,6 +405,7 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
                     g.rl = (void *)0;
                     g.part = (double *)y.total;
                 } else {
+                    g.ldr = kcore::r_ld(x.nocc, elem);
                     g.rl = (void *)(align_up(kp.r_words * elem, 256));
                     g.part = (double *)(align_up(kp.part_words * sizeof(double), 256));
                 }

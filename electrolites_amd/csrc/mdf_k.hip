@@ -188,7 +188,7 @@ int run_k(const KPlan &p, const Src *cderi, long naux, long ld, const int *pair_
     double *part = (double *)work;
     T *rl = (T *)(part + align_up(p.part_words, 64));
     T *rr = two ? rl + align_up(p.r_words, 64) : rl;
-    const long ldr = nocc, strideL = (long)nao * nocc;
+    const long ldr = r_ld(nocc, sizeof(T)), strideL = (long)nao * ldr;
     const long part_stride = (long)nao * nao;
     for (long L0 = 0; L0 < naux; L0 += p.nL) {
         const int nL = (int)(naux - L0 < p.nL ? naux - L0 : p.nL);
@@ -280,7 +280,7 @@ static long k_rows_rule(long naux, int nao, int nocc, bool two, int prec) {
     return block_rows(naux, nao, nocc, two, prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double));
 }
 static size_t k_bytes(long naux, int nao, int nocc, bool two, int prec, long rows) {
-    KPlan p = plan_k(naux, nao, nocc, two, rows);
+    KPlan p = plan_k(naux, nao, nocc, two, rows, prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double));
     if (prec == MDF_PREC_BF16X3) return kb3::layout(nao, nocc, p.nL, p.ksplit, two).total;
     return k_workspace(p, two, prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double));
 }
@@ -331,7 +331,7 @@ extern "C" int mdf_get_k(const void *cderi, int cderi_dtype, long naux, long npa
         }
         rows = lo;
     }
-    KPlan p = plan_k(naux, nao, nocc, two, rows);
+    KPlan p = plan_k(naux, nao, nocc, two, rows, prec == MDF_PREC_FP32 ? sizeof(float) : sizeof(double));
     if (cderi_dtype == MDF_DTYPE_F32) {
         const float *c = (const float *)cderi;
         if (prec == MDF_PREC_BF16X3)

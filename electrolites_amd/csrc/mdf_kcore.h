@@ -302,7 +302,15 @@ inline long block_rows(long naux, int nao, int nocc, bool two, size_t elem) {
     return r < 1 ? 1 : r;
 }
 
-inline KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows) {
+// Row stride of a transformed block R[L][i][k]: nocc rounded up to whole
+// 16-byte vectors, so step 2 streams every row with vector loads whatever
+// the parity of nocc (columns nocc..ldr-1 are never written or read).
+inline long r_ld(int nocc, size_t elem) {
+    const long e = 16 / (long)elem;
+    return (nocc + e - 1) / e * e;
+}
+
+inline KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows, size_t elem = 8) {
     KPlan p;
     p.ntile = (nao + S2_B - 1) / S2_B;
     p.ntiles_grid = two ? p.ntile * p.ntile : p.ntile * (p.ntile + 1) / 2;
@@ -310,7 +318,7 @@ inline KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows) {
     p.bn = pick_bn(nocc);
     p.nL = max_rows < naux ? max_rows : naux;
     if (p.nL < 1) p.nL = 1;
-    p.r_words = (size_t)p.nL * nao * nocc;
+    p.r_words = (size_t)p.nL * nao * r_ld(nocc, elem);
     p.part_words = (size_t)p.ksplit * nao * nao;
     return p;
 }
