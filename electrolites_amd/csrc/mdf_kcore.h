@@ -446,15 +446,30 @@ __device__ __forceinline__ void step1_tile(const Src *__restrict__ row, const in
     b3::Acc32<C::S::MI, C::S::NI> acc;
     acc.zero();
     b3::mainloop<BM, BN, 2, C::WGN>(la, lb, (nao + b3::BK - 1) / b3::BK, lds, acc);
-    b3::epilogue<BM, BN, 2, C::WGN>(acc, [&](int r, int c, float v) {
-        const int i = i0 + r, k = k0 + c;
-        if (i < nao && k < ldr) {
+    // The split tile leaves through LDS one plane at a time (the mainloop's
+    // last barrier has freed it): each lane puts its MFMA results at their
+    // (row, column), then every thread writes whole 16-byte runs of 8
+    // columns.  ldr and k0 are multiples of 32, so a run lies entirely inside
+    // or outside ldr.
+    constexpr int P = BN + 8;                  // row pitch (u16), 16-byte multiple
+    constexpr int CH = BN / 8;                 // 16-byte runs per row
+    static_assert(BM * P <= C::S::LDS_ELEMS, "one output plane must fit the tile's LDS");
+    for (int plane = 0; plane < 2; plane++) {
+        u16 *R = plane ? Rlo : Rhi;
+        b3::epilogue<BM, BN, 2, C::WGN>(acc, [&](int r, int c, float v) {
             u16 h, l;
             b3::split(v, h, l);
-            Rhi[(long)i * ldr + k] = h;
-            Rlo[(long)i * ldr + k] = l;
+            lds[r * P + c] = plane ? l : h;
+        });
+        __syncthreads();
+        for (int u = threadIdx.x; u < BM * CH; u += C::THREADS) {
+            const int r = u / CH, c = (u % CH) * 8;
+            const int i = i0 + r, k = k0 + c;
+            if (i < nao && k < ldr)
+                *reinterpret_cast<uint4 *>(R + (long)i * ldr + k) = *reinterpret_cast<const uint4 *>(lds + r * P + c);
         }
-    });
+        __syncthreads();
+    }
 }
 
 // Rows r0..r0+127 of the R planes; K-step kt of this split is
