@@ -36,6 +36,14 @@ LMAX_AUX = 4
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, '..', 'csrc', 'mdf_int_gen.hip')
 THREADS = 128
+#: a three-centre class whose Cartesian bra block times the aux components
+#: holds at most this many accumulators computes every aux component m in one
+#: thread (the Rys roots and recurrences once instead of 2 lc + 1 times)
+ALLM_MAX = 60
+
+
+def allm(la, lb, lc):
+    return ncart(la) * ncart(lb) * (2 * lc + 1) <= ALLM_MAX
 
 
 def fmt(c):
@@ -48,6 +56,8 @@ def gen_class(la, lb, lc, two_c):
     nroots = (la + lb + lc) // 2 + 1
     ca, cb, cc = cart_powers(la), cart_powers(lb), cart_powers(lc)
     nca, ncb = len(ca), len(cb)
+    am = (not two_c) and allm(la, lb, lc)
+    nm = 2 * lc + 1 if am else 1
     Ca, Cb, Cc = cart2sph(la), cart2sph(lb), cart2sph(lc)
     L = []
     w = L.append
@@ -68,10 +78,14 @@ def gen_class(la, lb, lc, two_c):
           'const int *__restrict__ pairs, int np, const int *__restrict__ kl, int nk, '
           'double *__restrict__ out, long ldo, long p0) {')
         w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
-        w(f'    if (tid >= (long)np * nk * {2 * lc + 1}) return;')
-        w('    const int ip = (int)(tid % np);')
-        w('    const long rest = tid / np;')
-        w('    const int kq = (int)(rest % nk), m = (int)(rest / nk);')
+        if am:
+            w('    if (tid >= (long)np * nk) return;')
+            w('    const int ip = (int)(tid % np), kq = (int)(tid / np);')
+        else:
+            w(f'    if (tid >= (long)np * nk * {2 * lc + 1}) return;')
+            w('    const int ip = (int)(tid % np);')
+            w('    const long rest = tid / np;')
+            w('    const int kq = (int)(rest % nk), m = (int)(rest / nk);')
         w('    const int ish = pairs[2 * ip], jsh = pairs[2 * ip + 1], ksh = kl[kq];')
     w('    const double Ax = ao.xyz[3 * ish], Ay = ao.xyz[3 * ish + 1], Az = ao.xyz[3 * ish + 2];')
     if not two_c:
@@ -79,9 +93,9 @@ def gen_class(la, lb, lc, two_c):
         w('    const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;')
         w('    const double rab2 = ABx * ABx + ABy * ABy + ABz * ABz;')
     w('    const double Cx = aux.xyz[3 * ksh], Cy = aux.xyz[3 * ksh + 1], Cz = aux.xyz[3 * ksh + 2];')
-    w(f'    double acc[{nca * ncb}];')
+    w(f'    double acc[{nm * nca * ncb}];')
     w('#pragma unroll')
-    w(f'    for (int t = 0; t < {nca * ncb}; t++) acc[t] = 0.;')
+    w(f'    for (int t = 0; t < {nm * nca * ncb}; t++) acc[t] = 0.;')
     w('    const int npa = ao.nprim[ish], oa = ao.poff[ish];')
     if not two_c:
         w('    const int npb = ao.nprim[jsh], ob = ao.poff[jsh];')
@@ -150,10 +164,8 @@ def gen_class(la, lb, lc, two_c):
                 for a in range(0, lab - b + 1):
                     w(f'                    const double I{d}_{a}_{b}_{f} = I{d}_{a+1}_{b-1}_{f} + '
                       f'AB{d} * I{d}_{a}_{b-1}_{f};')
-    # products per m
-    w('                    switch (m) {')
-    for mi in range(2 * lc + 1):
-        w(f'                    case {mi}: {{')
+    # products per m (every m in this thread, or the thread's m)
+    def products(mi, base, indent):
         sup = [(k, Cc[mi, k]) for k in range(len(cc)) if abs(Cc[mi, k]) > 1e-15]
         for a_i, (ax, ay, az) in enumerate(ca):
             for b_i, (bx, by, bz) in enumerate(cb):
@@ -161,10 +173,18 @@ def gen_class(la, lb, lc, two_c):
                 for k, coef in sup:
                     kx, ky, kz = cc[k]
                     terms.append(f'{fmt(coef)} * Ix_{ax}_{bx}_{kx} * Iy_{ay}_{by}_{ky} * Iz_{az}_{bz}_{kz}')
-                w(f'                        acc[{a_i * ncb + b_i}] += ' + ' + '.join(terms) + ';')
-        w('                        break;')
+                w(f'{indent}acc[{base + a_i * ncb + b_i}] += ' + ' + '.join(terms) + ';')
+    if am:
+        for mi in range(2 * lc + 1):
+            products(mi, mi * nca * ncb, '                    ')
+    else:
+        w('                    switch (m) {')
+        for mi in range(2 * lc + 1):
+            w(f'                    case {mi}: {{')
+            products(mi, 0, '                        ')
+            w('                        break;')
+            w('                    }')
         w('                    }')
-    w('                    }')
     w('                }')          # roots
     w('            }')              # ic
     w('        }')                  # ib / block
@@ -178,21 +198,25 @@ def gen_class(la, lb, lc, two_c):
             w(f'    out[(long)(prow0 + {s}) * ldo + qcol] = ' + ' + '.join(terms) + ';')
     else:
         w('    const int i0 = ao.aooff[ish], j0 = ao.aooff[jsh];')
-        w('    double *orow = out + (long)(aux.aooff[ksh] + m) * ldo - p0;')
         w('    const bool diag = ish == jsh;')
-        for s in range(nsa):
-            for t in range(nsb):
-                terms = []
-                for a in range(nca):
-                    if abs(Ca[s, a]) < 1e-15:
-                        continue
-                    for b in range(ncb):
-                        if abs(Cb[t, b]) < 1e-15:
+        for mi in range(nm):
+            mexpr = str(mi) if am else 'm'
+            w(f'    {{ double *orow = out + (long)(aux.aooff[ksh] + {mexpr}) * ldo - p0;')
+            base = mi * nca * ncb
+            for s in range(nsa):
+                for t in range(nsb):
+                    terms = []
+                    for a in range(nca):
+                        if abs(Ca[s, a]) < 1e-15:
                             continue
-                        terms.append(f'{fmt(Ca[s, a] * Cb[t, b])} * acc[{a * ncb + b}]')
-                cond = f'if (!diag || {s} >= {t}) ' if la == lb else ''
-                w(f'    {cond}{{ const long i = i0 + {s}; orow[i * (i + 1) / 2 + j0 + {t}] = '
-                  + ' + '.join(terms) + '; }')
+                        for b in range(ncb):
+                            if abs(Cb[t, b]) < 1e-15:
+                                continue
+                            terms.append(f'{fmt(Ca[s, a] * Cb[t, b])} * acc[{base + a * ncb + b}]')
+                    cond = f'if (!diag || {s} >= {t}) ' if la == lb else ''
+                    w(f'    {cond}{{ const long i = i0 + {s}; orow[i * (i + 1) / 2 + j0 + {t}] = '
+                      + ' + '.join(terms) + '; }')
+            w('    }')
     w('}')
     return name, '\n'.join(L)
 
@@ -360,10 +384,14 @@ def main():
     parts.append('    return MDF_OK;')
     parts.append('}')
     parts.append('')
+    cases = ' || '.join(f'(la == {a} && lb == {b} && lc == {c})' for (a, b, c) in names3 if allm(a, b, c))
+    parts.append(f'static bool allm_class(int la, int lb, int lc) {{ return {cases}; }}')
+    parts.append('')
     parts.append('int launch_int3c(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, '
                  'const int *pairs, int np, const int *kl, int nk, double *out, long ldo, long p0, '
                  'hipStream_t st) {')
-    parts.append('    const long n = (long)np * nk * (2 * lc + 1);')
+    parts.append('    // one thread per (pair, aux shell) for the all-m classes, per (pair, aux shell, m) otherwise')
+    parts.append('    const long n = (long)np * nk * (allm_class(la, lb, lc) ? 1 : 2 * lc + 1);')
     parts.append('    if (n == 0) return MDF_OK;')
     parts.append(f'    const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})), b({THREADS});')
     parts.append(f'    switch ((la * {LMAX_AO + 1} + lb) * {LMAX_AUX + 1} + lc) {{')
