@@ -30,8 +30,24 @@ struct PackedALoader {
     const int *pmap;     // MAPPED: pair index of (i, j), -1 if not stored
     int nao, i0;
     int upper;           // layout used by the last load()
+    // MAPPED: the pair indices of the next K-step, loaded one call ahead so
+    // the value gathers do not wait on the map (calls come with kt = 0, 1, ...)
+    int pm[N];
+    __device__ __forceinline__ void map_step(int kt, int (&p)[N]) const {
+        const int t = threadIdx.x, i = i0 + t / TPR, jb = kt * G_BK + (t % TPR) * N;
+#pragma unroll
+        for (int v = 0; v < N; v++) p[v] = (i < nao && jb + v < nao) ? pmap[(long)i * nao + jb + v] : -1;
+    }
     __device__ __forceinline__ void load(int kt, Src (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * G_BK;
+        if (MAPPED) {
+            upper = 0;
+            if (kt == 0) map_step(0, pm);
+#pragma unroll
+            for (int v = 0; v < N; v++) r[v] = pm[v] >= 0 ? row[pm[v]] : (Src)0;
+            map_step(kt + 1, pm);
+            return;
+        }
         if (!MAPPED && j0 >= i0 + 127) {
             // every j > every i: (i, j) at tri(j) + i, contiguous in i
             upper = 1;
@@ -350,8 +366,22 @@ struct PackedA32 {
     const int *pmap;
     int nao, i0;
     int upper;
+    int pm[N];           // MAPPED: the next K-step's pair indices (one call ahead)
+    __device__ __forceinline__ void map_step(int kt, int (&p)[N]) const {
+        const int t = threadIdx.x, i = i0 + t / TPR, jb = kt * b3::BK + (t % TPR) * N;
+#pragma unroll
+        for (int v = 0; v < N; v++) p[v] = (i < nao && jb + v < nao) ? pmap[(long)i * nao + jb + v] : -1;
+    }
     __device__ __forceinline__ void load(int kt, Src (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * b3::BK;
+        if (MAPPED) {
+            upper = 0;
+            if (kt == 0) map_step(0, pm);
+#pragma unroll
+            for (int v = 0; v < N; v++) r[v] = pm[v] >= 0 ? row[pm[v]] : (Src)0;
+            map_step(kt + 1, pm);
+            return;
+        }
         if (!MAPPED && j0 >= i0 + 127) {
             // every j > every i: (i, j) at tri(j) + i, contiguous in i.  Thread
             // t takes row i = i0 + t % 128 and j = j0 + N (t / 128) + v: each
