@@ -2,12 +2,12 @@
 
 Workload (one "step" = one FP64 ``get_jk`` call, J and K, over a resident
 cderi): a 64-water cluster at 1.0 g/cm^3 (seed 0), def2-SVP orbital basis
-(nao 1536) with the 109-function-per-water fitting set (naux 6976), the full
-packed triangle (npairs 1,180,416, 65.9 GB in FP64), and a mo-tagged random
-PSD density of rank nocc = 320.  The cderi is built on the GPU first (Rys
+(nao 1536) with the 109-function-per-water fitting set (naux 6976), the
+screened-in pairs of the packed triangle (378,596 of 1,180,416; 21.1 GB in
+FP64), and a mo-tagged random PSD density of rank nocc = 320.  The cderi is built on the GPU first (Rys
 int2c2e/int3c2e, Cholesky, L^-1, triangular multiply) and its build is timed
-per stage; ``--synthetic-cderi`` replaces it with seeded N(0,1) values of the
-same shape for quick kernel timing.
+per stage; ``--synthetic-cderi`` replaces it with seeded N(0,1) values over
+the full triangle (65.9 GB) for quick kernel timing.
 
 Prints ONE JSON line.  ``roofline`` is the K build (the dominant kernel
 group) against FP64 MFMA peak; ``roofline_j`` is the J build against HBM
@@ -80,13 +80,14 @@ def measured_traffic(nwater):
     return out
 
 
-def cpu_baseline(mol_nao, naux, npairs, nocc, args):
-    """CPU restatement (oracle leg) on a sample of aux rows, scaled."""
+def cpu_baseline(mol_nao, naux, npairs, nocc, args, pair_addr=None):
+    """CPU restatement (oracle leg) on a sample of aux rows, scaled; over
+    the same stored pairs as the GPU tensor."""
     from oracle import oracle as O
     nthreads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
     rng = np.random.default_rng(1)
     nao = mol_nao
-    pa, _ = O.pair_addresses(nao)
+    pa = O.pair_addresses(nao)[0] if pair_addr is None else np.asarray(pair_addr, dtype=np.int64)
     c = np.linalg.qr(rng.standard_normal((nao, nocc)))[0]
     dm = 2 * c @ c.T
     f = c * np.sqrt(2.)
@@ -111,7 +112,7 @@ def cpu_baseline(mol_nao, naux, npairs, nocc, args):
     return {'value': 3600. / per_call, 'unit': 'get_jk/hr', 'cores': nthreads,
             'kind': 'port',
             'sample': f'J over {sj} and K over {sk} of {naux} aux rows at the '
-                      f'full (nao {nao}, npairs {npairs}, nocc {nocc}) shape, '
+                      f'full (nao {nao}, npairs {npairs} stored, nocc {nocc}) shape, '
                       f'numpy/OpenBLAS FP64, scaled by naux/sample',
             'j_ms': tj * 1e3, 'k_ms': tk * 1e3}
 
@@ -296,6 +297,7 @@ def main():
     log(f'(H2O){args.nwater}: nao {nao} naux {naux} npairs {npairs} nocc {nocc}')
 
     build = None
+    idx = None                                    # pair addresses: None = the full triangle
     if args.synthetic_cderi:
         g = torch.Generator(device='cuda')
         g.manual_seed(0)
@@ -308,7 +310,8 @@ def main():
         timer = fastdfcderi._Timer()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        cderi, _ = fastdfcderi.cholesky_eri(mol, timer=timer)
+        cderi, idx = fastdfcderi.cholesky_eri(mol, timer=timer)
+        npairs = cderi.shape[1]                   # the screened-in pairs the tensor stores
         torch.cuda.synchronize()
         wall_b = time.perf_counter() - t0
         st = timer.totals()
@@ -325,7 +328,7 @@ def main():
 
     class DF:
         _cderi = {torch.cuda.current_device(): cderi}
-        _cderi_idx = None
+        _cderi_idx = idx
 
         def build(self):
             pass
@@ -393,7 +396,7 @@ def main():
 
     class DF32:
         _cderi = {torch.cuda.current_device(): c32}
-        _cderi_idx = None
+        _cderi_idx = idx
 
         def build(self):
             pass
@@ -453,14 +456,14 @@ def main():
 
         class HostDF:
             _cderi = {torch.cuda.current_device(): hpart}
-            _cderi_idx = None
+            _cderi_idx = idx
 
             def build(self):
                 pass
 
         class DevDF:
             _cderi = {torch.cuda.current_device(): dev_part}
-            _cderi_idx = None
+            _cderi_idx = idx
 
             def build(self):
                 pass
@@ -538,7 +541,7 @@ def main():
         DF._cderi = None
         torch.cuda.empty_cache()
         log('cpu baseline')
-        out['cpu_baseline'] = cpu_baseline(nao, naux, npairs, nocc, args)
+        out['cpu_baseline'] = cpu_baseline(nao, naux, npairs, nocc, args, None if idx is None else idx[0])
     print(json.dumps(out), flush=True)
 
 

@@ -76,7 +76,7 @@ def gen_class(la, lb, lc, two_c):
     else:
         w(f'__global__ __launch_bounds__({THREADS}) void {name}(const BasisDev ao, const BasisDev aux, '
           'const int *__restrict__ pairs, int np, const int *__restrict__ kl, int nk, '
-          'double *__restrict__ out, long ldo, long p0) {')
+          'double *__restrict__ out, long ldo, long p0, const int *__restrict__ cmap) {')
         w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
         if am:
             w('    if (tid >= (long)np * nk) return;')
@@ -201,7 +201,7 @@ def gen_class(la, lb, lc, two_c):
         w('    const bool diag = ish == jsh;')
         for mi in range(nm):
             mexpr = str(mi) if am else 'm'
-            w(f'    {{ double *orow = out + (long)(aux.aooff[ksh] + {mexpr}) * ldo - p0;')
+            w(f'    {{ double *orow = out + (long)(aux.aooff[ksh] + {mexpr}) * ldo;')
             base = mi * nca * ncb
             for s in range(nsa):
                 for t in range(nsb):
@@ -214,8 +214,8 @@ def gen_class(la, lb, lc, two_c):
                                 continue
                             terms.append(f'{fmt(Ca[s, a] * Cb[t, b])} * acc[{base + a * ncb + b}]')
                     cond = f'if (!diag || {s} >= {t}) ' if la == lb else ''
-                    w(f'    {cond}{{ const long i = i0 + {s}; orow[i * (i + 1) / 2 + j0 + {t}] = '
-                      + ' + '.join(terms) + '; }')
+                    w(f'    {cond}{{ const long i = i0 + {s}; const long f = i * (i + 1) / 2 + j0 + {t} - p0; '
+                      f'orow[cmap ? cmap[f] : f] = ' + ' + '.join(terms) + '; }')
             w('    }')
     w('}')
     return name, '\n'.join(L)
@@ -389,7 +389,7 @@ def main():
     parts.append('')
     parts.append('int launch_int3c(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, '
                  'const int *pairs, int np, const int *kl, int nk, double *out, long ldo, long p0, '
-                 'hipStream_t st) {')
+                 'const int *cmap, hipStream_t st) {')
     parts.append('    // one thread per (pair, aux shell) for the all-m classes, per (pair, aux shell, m) otherwise')
     parts.append('    const long n = (long)np * nk * (allm_class(la, lb, lc) ? 1 : 2 * lc + 1);')
     parts.append('    if (n == 0) return MDF_OK;')
@@ -397,7 +397,7 @@ def main():
     parts.append(f'    switch ((la * {LMAX_AO + 1} + lb) * {LMAX_AUX + 1} + lc) {{')
     for (la, lb, lc), n in names3.items():
         key = (la * (LMAX_AO + 1) + lb) * (LMAX_AUX + 1) + lc
-        parts.append(f'    case {key}: hipLaunchKernelGGL({n}, g, b, 0, st, ao, aux, pairs, np, kl, nk, out, ldo, p0); break;')
+        parts.append(f'    case {key}: hipLaunchKernelGGL({n}, g, b, 0, st, ao, aux, pairs, np, kl, nk, out, ldo, p0, cmap); break;')
     parts.append('    default: set_error("int3c2e: class (%d %d|%d) not generated", la, lb, lc); return MDF_EINVAL;')
     parts.append('    }')
     parts.append('    MDF_LAUNCHED("int3c");')

@@ -20,7 +20,7 @@ struct BasisDev {
 };
 
 int launch_int3c(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, const int *pairs, int np,
-                 const int *kl, int nk, double *out, long ldo, long p0, hipStream_t st);
+                 const int *kl, int nk, double *out, long ldo, long p0, const int *cmap, hipStream_t st);
 int launch_int1e(int la, int lb, const BasisDev &ao, const int *pairs, int np, int natm, const double *charges,
                  const double *atom_xyz, double *S, double *T, double *V, int nao, hipStream_t st);
 int launch_int2c(int la, int lc, const BasisDev &aux, const int *pl, int np, const int *ql, int nq,

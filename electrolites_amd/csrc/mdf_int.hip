@@ -26,7 +26,7 @@ static BasisDev to_dev(const mdf_basis *b) {
 
 extern "C" int mdf_int3c2e(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
                            const long *pair_class_off, const int *aux_sorted, const long *aux_l_off,
-                           long p0, double *out, long ldo, void *stream) {
+                           long p0, const int *col_map, double *out, long ldo, void *stream) {
     MDF_CHECK_ARG(ao && aux && pair_class_off && aux_l_off && out, "mdf_int3c2e: null argument");
     MDF_CHECK_ARG(ao->lmax <= LMAX_AO && aux->lmax <= LMAX_AUX,
                   "mdf_int3c2e: angular momentum beyond the generated classes (ao %d > %d or aux %d > %d)",
@@ -43,7 +43,7 @@ extern "C" int mdf_int3c2e(const mdf_basis *ao, const mdf_basis *aux, const int 
                 const long k0 = aux_l_off[lc], k1 = aux_l_off[lc + 1];
                 if (k1 <= k0) continue;
                 int rc = launch_int3c(la, lb, lc, a, x, pairs + 2 * q0, (int)(q1 - q0), aux_sorted + k0,
-                                      (int)(k1 - k0), out, ldo, p0, st);
+                                      (int)(k1 - k0), out, ldo, p0, col_map, st);
                 if (rc) return rc;
             }
         }

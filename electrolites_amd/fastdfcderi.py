@@ -131,6 +131,24 @@ def _chunks(shells, naux, budget):
     return out
 
 
+def kept_pairs(mol, bound=None):
+    """The function pairs the cderi stores: every (i, j), i >= j, whose shell
+    pair passes the screening (``ints.PAIR_SCREEN`` on the shell-pair bound,
+    the test :class:`ints.PairTasks` applies), as a mask over the packed lower
+    triangle p = i(i+1)/2 + j.  GPU4PySCF's cderi is stored the same way, over
+    its screened pair list with the addresses in ``_cderi_idx``."""
+    if bound is None:
+        bound = getattr(mol, '_mdf_pair_bound', None)
+        if bound is None:
+            bound = ints.pair_bound(mol.ao)
+            mol._mdf_pair_bound = bound
+    ao_loc = np.asarray(mol.ao.ao_loc)
+    shell_of = np.repeat(np.arange(len(ao_loc) - 1), np.diff(ao_loc))
+    rows, cols = np.tril_indices(int(ao_loc[-1]))
+    keep = np.asarray(bound)[shell_of[rows], shell_of[cols]] >= ints.PAIR_SCREEN
+    return keep, rows, cols
+
+
 def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'):
     """Build the cderi of ``mol`` (orbital basis ``mol.ao``, fitting basis
     ``mol.aux``) on the current device.  Returns ``(cderi, (pair_addr,
@@ -139,14 +157,26 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
     ``FASTDF_FP32``).  ``storage`` 'device' (the tensor in HBM) or 'host'
     (a page-locked host tensor written chunk by chunk through
     :class:`electrolites_amd.dfstore.HostStore`, for a cderi larger than
-    the device; electrolites/_dfstore.py)."""
+    the device; electrolites/_dfstore.py).
+
+    Only the screened-in pairs are stored (:func:`kept_pairs`): column c of
+    the tensor is the c-th kept pair in packed order, and the returned
+    addresses ``i * nao + j`` name them.  Each chunk's integrals are written
+    straight into those columns (``mdf_int3c2e`` with a column map), so the
+    metric solve and every later pass touch no screened column."""
     _dev.require_gpu()
     t = _dev.torch()
     tm = timer or _NoTimer()
     fp32 = FP32 if dtype is None else str(dtype).endswith('float32')
     ao, aux = _basis(mol, 'ao'), _basis(mol, 'aux')
     nao, naux = ao.nao, aux.nao
-    npairs = nao * (nao + 1) // 2
+    bound = getattr(mol, '_mdf_pair_bound', None)
+    if bound is None:
+        bound = ints.pair_bound(mol.ao)
+        mol._mdf_pair_bound = bound
+    keep, rows, cols = kept_pairs(mol, bound)
+    cidx = np.cumsum(keep) - 1                       # compact column of each kept pair
+    npairs = int(keep.sum())
     linv, coef = decompose_j2c(aux, timer)
     nrow = naux if coef is None else coef.shape[0]
     if storage not in ('device', 'host'):
@@ -154,17 +184,19 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
     on_host = storage == 'host'
     sdt = t.float32 if fp32 else t.float64
     cderi = None if on_host else t.empty((nrow, npairs), dtype=sdt, device=_dev.device())
-    bound = getattr(mol, '_mdf_pair_bound', None)
-    if bound is None:
-        bound = ints.pair_bound(mol.ao)
-        mol._mdf_pair_bound = bound
     budget = chunk_bytes or CHUNK_BYTES
     if fp32:
         budget //= 2                       # an FP64 output chunk beside the integrals
     chunks = _chunks(mol.ao, naux, budget)
-    widest = max(int(mol.ao.ao_loc[I1]) * (int(mol.ao.ao_loc[I1]) + 1) // 2 -
-                 int(mol.ao.ao_loc[I0]) * (int(mol.ao.ao_loc[I0]) + 1) // 2
-                 for I0, I1 in chunks)
+
+    def span(I0, I1):                 # full-triangle and compact column ranges of a chunk
+        i0, i1 = int(mol.ao.ao_loc[I0]), int(mol.ao.ao_loc[I1])
+        p0, p1 = i0 * (i0 + 1) // 2, i1 * (i1 + 1) // 2
+        c0 = int(cidx[p0 - 1]) + 1 if p0 > 0 else 0
+        c1 = int(cidx[p1 - 1]) + 1 if p1 > 0 else 0
+        return p0, p1, c0, c1
+
+    widest = max(max(span(I0, I1)[3] - span(I0, I1)[2] for I0, I1 in chunks), 1)
     buf = t.empty(naux * widest, dtype=t.float64, device=_dev.device())
     out64 = t.empty(nrow * widest, dtype=t.float64, device=_dev.device()) if fp32 else None
     store = None
@@ -175,16 +207,21 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
         tasks = ints.PairTasks(mol.ao, I0, I1, bound)
         STATS['pairs_kept'] += tasks.npairs_kept
         STATS['pairs_screened'] += (I1 - I0) * (I0 + I1 + 1) // 2 - tasks.npairs_kept
-        width = tasks.p1 - tasks.p0
+        p0, p1, c0, c1 = span(I0, I1)
+        width = c1 - c0
+        if width == 0:
+            continue
         blk = buf[:naux * width]
+        # the chunk's kept pairs land in its compact columns (every one is
+        # written: a kept shell pair writes all its function pairs)
+        cmap = _dev.to_device((cidx[p0:p1] - c0).astype(np.int32), dtype=t.int32)
         tm.mark('int3c2e:0')
-        _lib.call('mdf_fill_zero', _dev.ptr(blk), naux * width, _dev.stream_ptr())
-        ints.int3c2e_chunk(ao, aux, tasks, blk, width)
+        ints.int3c2e_chunk(ao, aux, tasks, blk, width, col_map=cmap)
         tm.mark('int3c2e:1')
         if on_host:
             dst, ldd = store.slot(width), width
         else:
-            dst, ldd = cderi[:, tasks.p0:], npairs
+            dst, ldd = cderi[:, c0:], npairs
         # FP64 result straight into the tensor, or into a chunk then rounded
         out, ldo = (out64, width) if fp32 else (dst, ldd)
         tm.mark('trmm:0')
@@ -199,15 +236,15 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
             _lib.call('mdf_cast_f32', _dev.ptr(out64), width, nrow, width, _dev.ptr(dst), ldd,
                       _dev.stream_ptr())
         if on_host:
-            store.write(dst, tasks.p0)
+            store.write(dst, c0)
         STATS['chunks'] += 1
     if on_host:
         cderi = store.finish()
         STATS['host_builds'] = STATS.get('host_builds', 0) + 1
     STATS['builds'] += 1
-    rows, cols = np.tril_indices(nao)
-    pair_addr = (rows * nao + cols).astype(np.int32)
-    diag = np.where(rows == cols)[0]
+    rk, ck = rows[keep], cols[keep]
+    pair_addr = (rk * nao + ck).astype(np.int32)
+    diag = np.where(rk == ck)[0]
     return cderi, (pair_addr, diag)
 
 

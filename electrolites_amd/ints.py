@@ -120,15 +120,17 @@ def _lp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
 
 
-def int3c2e_chunk(ao, aux, tasks, out, ldo):
-    """Fill ``out[P, p - p0]`` for the chunk's pairs (out pre-zeroed)."""
+def int3c2e_chunk(ao, aux, tasks, out, ldo, col_map=None):
+    """Fill ``out[P, p - p0]`` for the chunk's pairs (out pre-zeroed), or
+    ``out[P, col_map[p - p0]]`` with a device int32 column map (a cderi that
+    stores only the screened-in pairs)."""
     if ao.desc.lmax > LMAX_AO or aux.desc.lmax > LMAX_AUX:
         raise ValueError(f'angular momentum beyond the generated classes '
                          f'(orbital {ao.desc.lmax} > {LMAX_AO} or auxiliary '
                          f'{aux.desc.lmax} > {LMAX_AUX})')
     _lib.call('mdf_int3c2e', ctypes.byref(ao.desc), ctypes.byref(aux.desc),
               _dev.ptr(tasks.dev), _lp(tasks.class_off), _dev.ptr(aux.sorted),
-              _lp(aux.l_off), tasks.p0, _dev.ptr(out), ldo, _dev.stream_ptr())
+              _lp(aux.l_off), tasks.p0, _dev.ptr(col_map), _dev.ptr(out), ldo, _dev.stream_ptr())
 
 
 def int2c2e(aux, out=None):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 5
+#define MDF_ABI_VERSION 6
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
@@ -196,8 +196,9 @@ typedef struct {
     const double *coefs;
 } mdf_basis;
 
-/* Three-centre (ij|P) into one pair chunk: out[P * ldo + (p - p0)] with
- * p = i(i+1)/2 + j, i >= j.  Replaces GPU4PySCF's int3c2e evaluator in the
+/* Three-centre (ij|P) into one pair chunk: out[P * ldo + c] with
+ * c = p - p0 (col_map NULL) or c = col_map[p - p0] (the chunk's columns in a
+ * cderi that stores only the screened-in pairs), p = i(i+1)/2 + j, i >= j.  Replaces GPU4PySCF's int3c2e evaluator in the
  * reference's cderi build (electrolites/fastdfcderi.py:404-409, :604, :667).
  *   pairs          device int32 (ish, jsh) with ish >= jsh, sorted by class
  *                  (l_i, l_j); pair_class_off (HOST, 10 entries) bounds the
@@ -209,7 +210,7 @@ typedef struct {
  * Supported: orbital l <= 2, auxiliary l <= 4. */
 int mdf_int3c2e(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
                 const long *pair_class_off, const int *aux_sorted, const long *aux_l_off,
-                long p0, double *out, long ldo, void *stream);
+                long p0, const int *col_map, double *out, long ldo, void *stream);
 
 /* The Coulomb metric (P|Q), all naux x naux entries, out[P * ldo + Q].
  * Replaces electrolites/_dfj2c.py:234-262 (int2c2e). */

@@ -120,9 +120,16 @@ def test_cderi_matches_oracle(built, chunk_bytes):
     mol, df, ref, pa = built
     cderi, idx = fastdfcderi.cholesky_eri(mol, chunk_bytes=chunk_bytes)
     got = cderi.cpu().numpy()
-    assert got.shape == ref.shape
-    assert np.array_equal(idx[0], pa.astype(np.int32))
-    assert rel(got, ref) < 1e-9
+    # the tensor stores the screened-in pairs, in packed order, named by idx
+    pos = np.searchsorted(pa, idx[0])
+    assert np.array_equal(pa[pos], idx[0]) and np.all(np.diff(idx[0]) > 0)
+    assert got.shape == (ref.shape[0], len(pos))
+    assert rel(got, ref[:, pos]) < 1e-9
+    # the dropped pairs carry nothing (the oracle computes them unscreened)
+    dropped = np.setdiff1d(np.arange(ref.shape[1]), pos)
+    assert np.abs(ref[:, dropped]).max(initial=0.) < 1e-12 * np.abs(ref).max()
+    rows, cols = np.divmod(idx[0], mol.nao)
+    assert np.array_equal(idx[1], np.nonzero(rows == cols)[0])
 
 
 @pytest.mark.gpu
@@ -151,4 +158,4 @@ def test_fp32_storage_build(built, chunk_bytes):
     c32, idx = fastdfcderi.cholesky_eri(mol, chunk_bytes=chunk_bytes, dtype='float32')
     assert c32.dtype == torch.float32 and c32.shape == c64.shape
     assert torch.equal(c32, c64.float())
-    assert np.array_equal(idx[0], pa.astype(np.int32))
+    assert np.array_equal(idx[0], fastdfcderi.cholesky_eri(mol, chunk_bytes=chunk_bytes)[1][0])

@@ -89,13 +89,13 @@ def test_host_resident_jk(mol, monkeypatch, dtype, frac, max_rows, tagged, n_dm)
     from electrolites_amd.df import DF
     monkeypatch.setattr(dfstore, 'RESIDENT_FRAC', frac)
     monkeypatch.setattr(dfstore, 'RESIDENT_MAX_ROWS', max_rows)
-    npairs = mol.nao * (mol.nao + 1) // 2
-    monkeypatch.setattr(dfstore, 'STREAM_BLOCK_BYTES', 29 * npairs * (8 if dtype == 'float64' else 4))
     import electrolites_amd.fastdfcderi as FC
     monkeypatch.setattr(FC, 'FP32', dtype == 'float32')
     dev = DF(mol).build()
     host = DF(mol, storage='host').build()
     assert dfstore.is_host(host.cderi())
+    npairs = host.cderi().shape[1]             # the stored (screened-in) pairs
+    monkeypatch.setattr(dfstore, 'STREAM_BLOCK_BYTES', 29 * npairs * (8 if dtype == 'float64' else 4))
     rng = np.random.default_rng(5 + n_dm)
     dm = _dms(mol.nao, 9, n_dm, rng, tagged)
     before = dfstore.stats()['blocks']
@@ -120,11 +120,10 @@ def test_host_resident_low_precision_k(mol, monkeypatch):
     from electrolites_amd import dfstore
     from electrolites_amd.df import DF
     monkeypatch.setattr(dfstore, 'RESIDENT_FRAC', 0.)
-    npairs = mol.nao * (mol.nao + 1) // 2
-    monkeypatch.setattr(dfstore, 'STREAM_BLOCK_BYTES', 50 * npairs * 8)
     host = DF(mol, storage='host').build()
+    monkeypatch.setattr(dfstore, 'STREAM_BLOCK_BYTES', 50 * host.cderi().shape[1] * 8)
     cderi = host.cderi().numpy()
-    pa, _ = O.pair_addresses(mol.nao)
+    pa = host._cderi_idx[0]
     dm = _dms(mol.nao, 9, 1, np.random.default_rng(3), True)
     host.k_precision = 'bf16x3'
     _, vk = host.get_jk(dm, hermi=1)
@@ -154,7 +153,7 @@ def test_df_loop_blocks(mol, monkeypatch, storage, dtype):
     df = DF(mol, storage=storage).build()
     cderi = df.cderi()
     ref = cderi.cpu().double().numpy()
-    pa, _ = O.pair_addresses(mol.nao)
+    pa = df._cderi_idx[0]
     naux = ref.shape[0]
     got, L0 = [], 0
     for dense, packed in df.loop(blksize=41):

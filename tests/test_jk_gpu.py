@@ -304,8 +304,7 @@ def test_ri_j_without_cderi(chunk_bytes):
     vj = dfj.get_j(dms, hermi=1)
     rj, _ = fastdf.get_jk(df, dms, hermi=1, with_k=False)
     assert rel(vj, rj) < 1e-12
-    pa, _ = O.pair_addresses(nao)
-    oj, _ = O.get_jk(df.cderi().cpu().numpy(), pa, dms)
+    oj, _ = O.get_jk(df.cderi().cpu().numpy(), df._cderi_idx[0], dms)
     assert rel(vj, oj) < 1e-11
     ns = rng.standard_normal((nao, nao)) * .1                # hermi 0 (non-symmetric)
     vj0 = dfj.get_j(ns, hermi=0)
