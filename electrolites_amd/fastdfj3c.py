@@ -10,7 +10,9 @@ functional (and the Coulomb half of RIJONX) runs on, where GPU4PySCF's
 
 Here the three-centre integrals are evaluated chunk by chunk of orbital
 shell rows (``mdf_int3c2e``, the cderi build's Rys kernels) into one device
-buffer and contracted at once by the two halves of the J kernel
+buffer -- only the screened-in pairs, in the compact columns the cderi build
+uses, so nothing is zero-filled or streamed for a screened pair -- and
+contracted at once by the two halves of the J kernel
 (``mdf_j_rho``, ``mdf_j_vj``), twice per call: nothing of size
 ``naux x npairs`` is stored, so a molecule whose cderi would not fit (or is
 not worth building for a J-only run) still gets a GPU J.  The metric solve
@@ -47,16 +49,27 @@ class _State:
         self.ao = fastdfcderi._basis(mol, 'ao')
         self.aux = fastdfcderi._basis(mol, 'aux')
         self.nao, self.naux = self.ao.nao, self.aux.nao
-        self.npairs = self.nao * (self.nao + 1) // 2
         linv, coef = fastdfcderi.decompose_j2c(self.aux)
         self.M = linv if coef is None else coef          # (nrow, naux)
         bound = getattr(mol, '_mdf_pair_bound', None)
         if bound is None:
             bound = ints.pair_bound(mol.ao)
             mol._mdf_pair_bound = bound
-        self.tasks = [ints.PairTasks(mol.ao, I0, I1, bound)
-                      for I0, I1 in fastdfcderi._chunks(mol.ao, self.naux, chunk_bytes)]
-        widest = max(tk.p1 - tk.p0 for tk in self.tasks)
+        keep, rows, cols = fastdfcderi.kept_pairs(mol, bound)
+        cidx = np.cumsum(keep) - 1
+        self.npairs = int(keep.sum())
+        self.addr = _dev.to_device(((rows * self.nao + cols)[keep]).astype(np.int32), dtype=t.int32)
+        self.tasks = []
+        for I0, I1 in fastdfcderi._chunks(mol.ao, self.naux, chunk_bytes):
+            tk = ints.PairTasks(mol.ao, I0, I1, bound)
+            c0 = int(cidx[tk.p0 - 1]) + 1 if tk.p0 > 0 else 0
+            c1 = int(cidx[tk.p1 - 1]) + 1 if tk.p1 > 0 else 0
+            if c1 == c0:
+                continue
+            tk.c0, tk.c1 = c0, c1
+            tk.cmap = _dev.to_device((cidx[tk.p0:tk.p1] - c0).astype(np.int32), dtype=t.int32)
+            self.tasks.append(tk)
+        widest = max([tk.c1 - tk.c0 for tk in self.tasks] + [1])
         self.buf = t.empty(self.naux * widest, dtype=t.float64, device=_dev.device())
 
 
@@ -72,12 +85,12 @@ def _state(dfobj, chunk_bytes=None):
 
 
 def _sweep(st, fn):
-    """Evaluate each chunk of (P|ij) into the buffer and hand it to fn."""
+    """Evaluate each chunk of (P|ij) over its kept pairs into the buffer
+    (every column written) and hand it to fn."""
     for tk in st.tasks:
-        width = tk.p1 - tk.p0
+        width = tk.c1 - tk.c0
         blk = st.buf[:st.naux * width]
-        _lib.call('mdf_fill_zero', _dev.ptr(blk), st.naux * width, _dev.stream_ptr())
-        ints.int3c2e_chunk(st.ao, st.aux, tk, blk, width)
+        ints.int3c2e_chunk(st.ao, st.aux, tk, blk, width, col_map=tk.cmap)
         fn(tk, blk, width)
         STATS['chunks'] += 1
 
@@ -95,15 +108,15 @@ def get_j(dfobj, dm, hermi=1):
     dms3 = _dev.to_device(dm).reshape(-1, nao, nao)
     n = dms3.shape[0]
     d = t.empty((n, npairs), dtype=t.float64, device=_dev.device())
-    _lib.call('mdf_pack_dm', _dev.ptr(dms3), n, nao, nao * nao, None, npairs,
+    _lib.call('mdf_pack_dm', _dev.ptr(dms3), n, nao, nao * nao, _dev.ptr(st.addr), npairs,
               1 if hermi == 1 else 0, _dev.ptr(d), npairs, _dev.stream_ptr())
-    widest = max(tk.p1 - tk.p0 for tk in st.tasks)
+    widest = max(tk.c1 - tk.c0 for tk in st.tasks)
     work = _dev.WORKSPACE.get(_lib.query('mdf_get_j_workspace', naux, widest, n))
     rho = t.zeros((n, naux), dtype=t.float64, device=_dev.device())
 
     def contract_dm(tk, blk, width):
         _lib.call('mdf_j_rho', _dev.ptr(blk), _lib.DTYPE_F64, naux, width, width,
-                  _dev.ptr(d[:, tk.p0:]), n, npairs, _dev.ptr(rho), naux, 1,
+                  _dev.ptr(d[:, tk.c0:]), n, npairs, _dev.ptr(rho), naux, 1,
                   _dev.ptr(work), work.numel(), _dev.stream_ptr())
 
     _sweep(st, contract_dm)
@@ -119,10 +132,10 @@ def get_j(dfobj, dm, hermi=1):
 
     def contract_auxvec(tk, blk, width):
         _lib.call('mdf_j_vj', _dev.ptr(blk), _lib.DTYPE_F64, naux, width, width, _dev.ptr(c), n, naux,
-                  _dev.ptr(vjp[:, tk.p0:]), npairs, 1, _dev.ptr(work), work.numel(), _dev.stream_ptr())
+                  _dev.ptr(vjp[:, tk.c0:]), npairs, 1, _dev.ptr(work), work.numel(), _dev.stream_ptr())
 
     _sweep(st, contract_auxvec)
     vj = t.empty((n, nao, nao), dtype=t.float64, device=_dev.device())
-    _lib.call('mdf_unpack_pairs', _dev.ptr(vjp), n, npairs, None, npairs, nao, _dev.ptr(vj),
+    _lib.call('mdf_unpack_pairs', _dev.ptr(vjp), n, npairs, _dev.ptr(st.addr), npairs, nao, _dev.ptr(vj),
               nao * nao, _dev.stream_ptr())
     return fastdf._finish(vj, dm, out_device)
