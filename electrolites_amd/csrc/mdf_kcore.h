@@ -41,6 +41,8 @@ struct PackedALoader {
     __device__ __forceinline__ void load(int kt, Src (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * G_BK;
         if (MAPPED) {
+            // lanes run along j of one row i in every K-step (the transposed
+            // layout above the diagonal measured no faster here)
             upper = 0;
             if (kt == 0) map_step(0, pm);
 #pragma unroll
@@ -367,19 +369,32 @@ struct PackedA32 {
     int nao, i0;
     int upper;
     int pm[N];           // MAPPED: the next K-step's pair indices (one call ahead)
-    __device__ __forceinline__ void map_step(int kt, int (&p)[N]) const {
-        const int t = threadIdx.x, i = i0 + t / TPR, jb = kt * b3::BK + (t % TPR) * N;
+    int pm_upper;        // and their layout
+    // Above the diagonal (every j > every i) the transposed layout of the
+    // packed loader: thread t takes i = i0 + t % 128 and j = j0 + N (t / 128)
+    // + v, so a load instruction reads consecutive i of the map row j and of
+    // the stored pairs (j, i).
+    __device__ __forceinline__ void map_step(int kt, int (&p)[N], int &up) const {
+        const int t = threadIdx.x, j0 = kt * b3::BK;
+        up = j0 >= i0 + 127;
+        if (up) {
+            const int i = i0 + (t & 127), jb = j0 + (t >> 7) * N;
 #pragma unroll
-        for (int v = 0; v < N; v++) p[v] = (i < nao && jb + v < nao) ? pmap[(long)i * nao + jb + v] : -1;
+            for (int v = 0; v < N; v++) p[v] = (i < nao && jb + v < nao) ? pmap[(long)(jb + v) * nao + i] : -1;
+        } else {
+            const int i = i0 + t / TPR, jb = j0 + (t % TPR) * N;
+#pragma unroll
+            for (int v = 0; v < N; v++) p[v] = (i < nao && jb + v < nao) ? pmap[(long)i * nao + jb + v] : -1;
+        }
     }
     __device__ __forceinline__ void load(int kt, Src (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * b3::BK;
         if (MAPPED) {
-            upper = 0;
-            if (kt == 0) map_step(0, pm);
+            if (kt == 0) map_step(0, pm, pm_upper);
+            upper = pm_upper;
 #pragma unroll
             for (int v = 0; v < N; v++) r[v] = pm[v] >= 0 ? row[pm[v]] : (Src)0;
-            map_step(kt + 1, pm);
+            map_step(kt + 1, pm, pm_upper);
             return;
         }
         if (!MAPPED && j0 >= i0 + 127) {
