@@ -146,3 +146,29 @@ def test_tile_fill_dense_configs():
     far = Mole([('O', (0., 0., 0.)), ('H', (0.76, 0.59, 0.)), ('H', (-0.76, 0.59, 0.)),
                 ('O', (60., 0., 0.)), ('H', (60.76, 0.59, 0.)), ('H', (59.24, 0.59, 0.))])
     assert fastdfcderi.tile_fill(far, tile=8) < 0.75
+
+
+def test_kept_pairs_cover_the_chunk_tasks():
+    """The cderi's stored columns (fastdfcderi.kept_pairs) are exactly the
+    function pairs of the shell pairs the integral chunks compute, chunk by
+    chunk, so every stored column is written and no computed pair is lost
+    (host-only: the screening bound and the task lists)."""
+    from electrolites_amd import fastdfcderi, ints
+    from electrolites_amd.gto import water_cluster
+    mol = water_cluster(6, seed=2, side=7.0, li=True)
+    bound = ints.pair_bound(mol.ao)
+    keep, rows, cols = fastdfcderi.kept_pairs(mol, bound)
+    ao_loc = np.asarray(mol.ao.ao_loc)
+    covered = np.zeros_like(keep)
+    tri = lambda i, j: i * (i + 1) // 2 + j
+    for I0, I1 in fastdfcderi._chunks(mol.ao, mol.naux, 400000):
+        tk = ints.PairTasks(mol.ao, I0, I1, bound)
+        for ish, jsh in tk.pairs:
+            for i in range(ao_loc[ish], ao_loc[ish + 1]):
+                for j in range(ao_loc[jsh], ao_loc[jsh + 1]):
+                    if j <= i:
+                        p = tri(i, j)
+                        assert tk.p0 <= p < tk.p1
+                        covered[p] = True
+    assert np.array_equal(covered, keep)
+    assert 0 < keep.sum() < keep.size                 # something is screened at this size
