@@ -56,9 +56,13 @@ def to_device(x, dtype=None):
 
 
 class Workspace:
-    """A grow-only scratch buffer per device, reused across calls."""
+    """A grow-only scratch buffer per device, reused across calls.  With
+    ``grow`` > 1 a buffer that must grow takes at least ``grow`` times its
+    old size, so a caller whose needs creep upward (a serving loop whose
+    batch gains members) reallocates a few times instead of at every step."""
 
-    def __init__(self):
+    def __init__(self, grow=1.0):
+        self.grow = float(grow)
         self._buf = {}
 
     def get(self, nbytes):
@@ -66,8 +70,10 @@ class Workspace:
         dev = t.cuda.current_device()
         buf = self._buf.get(dev)
         if buf is None or buf.numel() < nbytes:
+            old = 0 if buf is None else buf.numel()
             self._buf.pop(dev, None)
-            buf = t.empty(max(int(nbytes), 256), dtype=t.uint8, device=device())
+            buf = None
+            buf = t.empty(max(int(nbytes), int(old * self.grow), 256), dtype=t.uint8, device=device())
             self._buf[dev] = buf
         return buf
 

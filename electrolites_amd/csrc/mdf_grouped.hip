@@ -557,7 +557,14 @@ extern "C" int mdf_grouped_plan_create(const mdf_jk_item *items, int nitems, int
         delete p;
         return MDF_EHIP;
     }
-    e = hipMemcpy(p->dev, host.data(), host.size(), hipMemcpyHostToDevice);
+    // the upload goes through a private non-blocking stream: a plain
+    // hipMemcpy would also wait for the work queued on the null stream (a
+    // serving loop creates plans while its previous runs are in flight)
+    hipStream_t up = nullptr;
+    e = hipStreamCreateWithFlags(&up, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->dev, host.data(), host.size(), hipMemcpyHostToDevice, up);
+    if (e == hipSuccess) e = hipStreamSynchronize(up);
+    if (up) (void)hipStreamDestroy(up);
     if (e != hipSuccess) {
         set_error("mdf_grouped_plan_create: copy failed: %s", hipGetErrorString(e));
         (void)hipFree(p->dev);

@@ -111,6 +111,10 @@ class Request:
         return None if self.t_done is None else self.t_done - self.arrival
 
 
+#: ticks a grouped plan stays cached after its last use
+PLAN_KEEP_TICKS = 32
+
+
 class _Slot:
     """An admitted request: its SCF state and its grouped J/K item."""
 
@@ -168,7 +172,7 @@ class Server:
         self.pipeline = pipeline
         self._plan_tick = {}
         self._weight = [0., 0.]
-        self._work = _dev.Workspace()
+        self._work = _dev.Workspace(grow=2.0)
         self.workers = workers
         self.host_diag_max = host_diag_max
         self.low_precision = low_precision
@@ -282,9 +286,13 @@ class Server:
                 main.wait_event(s.ready)        # its absorb wrote the density
             s.load_density()
             groups[s.scf.prec].append(s)
+        # every group's plan first (a new plan uploads its tables), then the runs
+        plans = []
         for prec, members in sorted(groups.items()):
             key, g = self._plan(members, prec)
             self._plan_tick[key] = self.ticks
+            plans.append(g)
+        for g in plans:
             g.run()
         ev = t.cuda.Event()
         ev.record(main)
@@ -338,7 +346,9 @@ class Server:
             for s in launch:
                 s.jk_done = ev
         finished = self._absorb_all(absorb) if absorb else []
-        for key in [k for k, tk in self._plan_tick.items() if tk < self.ticks - 2]:
+        # plans unused for a while are freed (a free synchronises the device,
+        # so not every tick)
+        for key in [k for k, tk in self._plan_tick.items() if tk < self.ticks - PLAN_KEEP_TICKS]:
             self._plans.pop(key).close()
             del self._plan_tick[key]
         self.timing['tick_s'] += time.perf_counter() - t1
