@@ -92,7 +92,8 @@ def pinned_empty(shape, dtype, pin=True):
 
 def release(host):
     """Give a host tensor from :func:`pinned_empty` back to the pool (kept
-    while the pool stays under ``POOL_BYTES``, else freed)."""
+    while the pool stays under ``POOL_BYTES``, else freed).  The caller gives
+    the memory up: a later :func:`pinned_empty` of the same size reuses it."""
     t = _dev.torch()
     if not isinstance(host, t.Tensor) or host.is_cuda or not host.is_pinned():
         return
@@ -258,17 +259,21 @@ def blocks(host, resident, blk_rows):
         ready[i] = ev
         STATS['streamed_bytes'] += nL * npairs * elem
 
-    fetch(0, starts[0])
-    for k, L0 in enumerate(starts):
-        i = k % 2
-        if k + 1 < len(starts):
-            fetch(1 - i, starts[k + 1])
-        comp.wait_event(ready[i])
-        nL = min(nb, naux - L0)
-        STATS['blocks'] += 1
-        yield bufs[i][:nL], L0, nL
-        ev = t.cuda.Event()
-        ev.record(comp)
-        free[i] = ev
-    # the buffers are freed through the caching allocator in stream order
-    io.synchronize()
+    try:
+        fetch(0, starts[0])
+        for k, L0 in enumerate(starts):
+            i = k % 2
+            if k + 1 < len(starts):
+                fetch(1 - i, starts[k + 1])
+            comp.wait_event(ready[i])
+            nL = min(nb, naux - L0)
+            STATS['blocks'] += 1
+            yield bufs[i][:nL], L0, nL
+            ev = t.cuda.Event()
+            ev.record(comp)
+            free[i] = ev
+    finally:
+        # a prefetch may still be writing a buffer (also when the consumer
+        # stops early): let it land before the buffers go back to the
+        # allocator, which hands them out again in the compute stream's order
+        io.synchronize()
