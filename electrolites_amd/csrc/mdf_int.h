@@ -25,5 +25,10 @@ int launch_int1e(int la, int lb, const BasisDev &ao, const int *pairs, int np, i
                  const double *atom_xyz, double *S, double *T, double *V, int nao, hipStream_t st);
 int launch_int2c(int la, int lc, const BasisDev &aux, const int *pl, int np, const int *ql, int nq,
                  double *out, long ldo, hipStream_t st);
+int launch_int3c_ip(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, const int *pairs, int np,
+                    const int *kl, int nk, const double *W, long ldw, const int *cmap, const int *ao_atom,
+                    const int *aux_atom, double *grad, double *grad_aux, hipStream_t st);
+int launch_int2c_ip(int la, int lc, const BasisDev &aux, const int *pl, int np, const int *ql, int nq,
+                    const double *Z, long ldz, const int *aux_atom, double *grad, hipStream_t st);
 
 }  // namespace mdf

@@ -70,6 +70,64 @@ extern "C" int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const lo
     return MDF_OK;
 }
 
+// Nuclear derivative of the three-centre term of the DF gradient:
+// grad[3 * atom + x] += sum_{P, i, j} W[P, (i, j)] d(ij|P)/dR_atom,x over the
+// orbital centres of the listed shell pairs (both orders of an off-diagonal
+// pair), and the aux centre's share (translational invariance) into
+// grad_aux.  Plays the part of GPU4PySCF's sum_ejk_int3c2e_ip1 in the
+// reference's gradient (electrolites/fastdfgrad.py:1014-1040).
+extern "C" int mdf_int3c2e_ip(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
+                              const long *pair_class_off, const int *aux_sorted, const long *aux_l_off,
+                              const double *W, long ldw, const int *col_map, const int *ao_atom,
+                              const int *aux_atom, double *grad, double *grad_aux, void *stream) {
+    MDF_CHECK_ARG(ao && aux && pair_class_off && aux_l_off && W && ao_atom && aux_atom && grad && grad_aux,
+                  "mdf_int3c2e_ip: null argument");
+    MDF_CHECK_ARG(ao->lmax <= LMAX_AO && aux->lmax <= LMAX_AUX,
+                  "mdf_int3c2e_ip: angular momentum beyond the generated classes (ao %d > %d or aux %d > %d)",
+                  ao->lmax, LMAX_AO, aux->lmax, LMAX_AUX);
+    MDF_CHECK_ARG(ldw > 0, "mdf_int3c2e_ip: ldw must be positive");
+    hipStream_t st = as_stream(stream);
+    const BasisDev a = to_dev(ao), x = to_dev(aux);
+    for (int la = 0; la <= LMAX_AO; la++)
+        for (int lb = 0; lb <= LMAX_AO; lb++) {
+            const int cls = la * (LMAX_AO + 1) + lb;
+            const long q0 = pair_class_off[cls], q1 = pair_class_off[cls + 1];
+            if (q1 <= q0) continue;
+            MDF_CHECK_ARG(q1 - q0 < (1L << 31), "mdf_int3c2e_ip: too many pairs in one class");
+            for (int lc = 0; lc <= LMAX_AUX; lc++) {
+                const long k0 = aux_l_off[lc], k1 = aux_l_off[lc + 1];
+                if (k1 <= k0) continue;
+                int rc = launch_int3c_ip(la, lb, lc, a, x, pairs + 2 * q0, (int)(q1 - q0), aux_sorted + k0,
+                                         (int)(k1 - k0), W, ldw, col_map, ao_atom, aux_atom, grad, grad_aux, st);
+                if (rc) return rc;
+            }
+        }
+    return MDF_OK;
+}
+
+// Nuclear derivative of the metric term: grad[3 * atom + x] +=
+// sum_{P, Q} Z[P, Q] d(P|Q)/dR_atom,x, every ordered pair of aux shells
+// (GPU4PySCF's int2c2e_ip1_per_atom, electrolites/fastdfgrad.py:1172-1178).
+extern "C" int mdf_int2c2e_ip(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_off,
+                              const double *Z, long ldz, const int *aux_atom, double *grad, void *stream) {
+    MDF_CHECK_ARG(aux && aux_l_off && Z && aux_atom && grad, "mdf_int2c2e_ip: null argument");
+    MDF_CHECK_ARG(aux->lmax <= LMAX_AUX, "mdf_int2c2e_ip: aux angular momentum %d > %d", aux->lmax, LMAX_AUX);
+    hipStream_t st = as_stream(stream);
+    const BasisDev x = to_dev(aux);
+    for (int la = 0; la <= LMAX_AUX; la++) {
+        const long p0 = aux_l_off[la], p1 = aux_l_off[la + 1];
+        if (p1 <= p0) continue;
+        for (int lc = 0; lc <= LMAX_AUX; lc++) {
+            const long k0 = aux_l_off[lc], k1 = aux_l_off[lc + 1];
+            if (k1 <= k0) continue;
+            int rc = launch_int2c_ip(la, lc, x, aux_sorted + p0, (int)(p1 - p0), aux_sorted + k0,
+                                     (int)(k1 - k0), Z, ldz, aux_atom, grad, st);
+            if (rc) return rc;
+        }
+    }
+    return MDF_OK;
+}
+
 // Overlap, kinetic and nuclear-attraction matrices over the listed shell
 // pairs (both triangles written; screened pairs untouched).  The one-electron
 // part of the Fock matrix for the SCF driver (PySCF int1e_ovlp / int1e_kin /

@@ -1,0 +1,172 @@
+"""The density-fitted gradient's two-electron term from the cderi the SCF
+already built.
+
+Mirrors electrolites/fastdfgrad.py ``jk_energy_per_atom`` (:1229-1325) and
+its contraction ``_ejk`` (:870-1182), the drop-in for GPU4PySCF's
+``df/grad/rhf.py:_jk_energy_per_atom``: the nuclear derivative, per atom, of
+the restricted two-electron energy
+
+    E = 1/2 j_factor tr(D J[D]) - 1/4 k_factor tr(D K[D])
+
+with J and K density fitted over the metric M = (P|Q).  With the cderi
+``B_L = sum_P A[L, P] T_P`` (T_P the three-centre integrals (ij|P); A the
+inverse Cholesky factor of M or the eigen filter's coefficients, so that
+A^T A = M^-1) and D = F_l F_r^T,
+
+    rho_B[L]  = sum_ij B_L,ij D_ij          c = A^T rho_B          (= M^-1 rho)
+    rhok[L]   = F_r^T B_L F_l               dmoo = A^T rhok        (nocc x nocc each)
+    W[P, ij]  = j c_P D_ij - 1/2 k (F_l dmoo_P F_r^T)_ij
+    Z[P, Q]   = -1/2 j c_P c_Q + 1/4 k tr(dmoo_P dmoo_Q)
+
+    dE/dR = sum_{P,ij} W[P, ij] d(ij|P)/dR + sum_{P,Q} Z[P, Q] d(P|Q)/dR
+
+(the reference's identity A A^T-free form: its ``dm_oo = aux_coef @ rhok``,
+:16-45; ``k_half = -0.5 k_factor`` for a restricted density, :900-903).  The
+three-centre integrals are not evaluated again: rho_B and rhok come from the
+cderi's blocks (:meth:`DF.loop`), and the derivative integrals are
+contracted with W and Z inside the Rys kernels (``mdf_int3c2e_ip``,
+``mdf_int2c2e_ip``), which add straight into the per-atom gradient.  W is
+held in the cderi's own compact column layout (one value per stored pair).
+
+The small dense contractions (rhok, the triangular products, W's blocks and
+Z) run as torch FP64 matrix products on the device.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _dev
+from . import _lib
+from . import fastdf
+from . import fastdfcderi
+from . import ints
+
+STATS = {'calls': 0}
+
+
+def stats():
+    return dict(STATS)
+
+
+def _class_sorted_pairs(mol, addr):
+    """The shell pairs (I >= J) whose function pairs the cderi stores, sorted
+    by class, and the packed-pair -> column map of the cderi (int32, -1 for a
+    pair it does not store).  A shell pair stored in part is an error."""
+    sh = mol.ao
+    nao, nbas = sh.nao, sh.nbas
+    ao_loc = np.asarray(sh.ao_loc)
+    addr = np.asarray(addr, dtype=np.int64)
+    i, j = np.divmod(addr, nao)
+    hi, lo = np.maximum(i, j), np.minimum(i, j)
+    cmap = np.full(nao * (nao + 1) // 2, -1, dtype=np.int32)
+    cmap[hi * (hi + 1) // 2 + lo] = np.arange(len(addr), dtype=np.int32)
+    shell_of = np.repeat(np.arange(nbas), np.diff(ao_loc))
+    rows, cols = np.tril_indices(nao)
+    sp = shell_of[rows] * nbas + shell_of[cols]
+    total = np.bincount(sp, minlength=nbas * nbas)
+    have = np.bincount(sp, weights=(cmap >= 0), minlength=nbas * nbas).astype(np.int64)
+    if np.any((have > 0) & (have < total)):
+        raise ValueError('jk_energy_per_atom: the cderi stores part of a shell pair')
+    full = np.nonzero((have == total) & (total > 0))[0]
+    I, J = np.divmod(full, nbas)
+    l = np.asarray(sh.shl_l)
+    cls = l[I] * (ints.LMAX_AO + 1) + l[J]
+    order = np.argsort(cls, kind='stable')
+    pairs = np.ascontiguousarray(np.stack([I, J], axis=1)[order].astype(np.int32))
+    off = np.zeros((ints.LMAX_AO + 1) ** 2 + 1, dtype=np.int64)
+    off[1:] = np.cumsum(np.bincount(cls, minlength=(ints.LMAX_AO + 1) ** 2))
+    return pairs, off, cmap
+
+
+def _factors(dm, hermi):
+    if hermi != 1:
+        raise ValueError('jk_energy_per_atom: a symmetric density (hermi=1) is required')
+    dms = dm
+    if np.ndim(dm) == 2 or (hasattr(dm, 'dim') and dm.dim() == 2):
+        dms = dm[None] if not hasattr(dm, 'mo_coeff') else dm
+    fl, fr = fastdf.factorize_dm(dms, 1)
+    if fl.shape[0] != 1:
+        raise ValueError('jk_energy_per_atom: one restricted density expected')
+    fl = fl[0]
+    fr = fl if fr is None else fr[0]
+    return fl.contiguous(), fr.contiguous()
+
+
+def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_response=True):
+    """``(natm, 3)`` nuclear derivative of ``1/2 j tr(D J) - 1/4 k tr(D K)``
+    for the restricted density ``dm`` (nao x nao; tagged with
+    ``mo_coeff``/``mo_occ`` or not), from the built cderi of ``dfobj``
+    (electrolites/fastdfgrad.py:1229).  ``auxbasis_response`` adds the
+    fitting functions' centres (the aux-centre share of the three-centre
+    term and the metric term), as GPU4PySCF's flag of that name."""
+    t = _dev.torch()
+    _dev.require_gpu()
+    mol = dfobj.mol
+    if getattr(dfobj, '_cderi', None) is None:
+        dfobj.build()
+    nao, natm = mol.nao, mol.natm
+    fl, fr = _factors(dm, hermi)
+    nocc = fl.shape[1]
+    D = fl @ fr.T
+    D = .5 * (D + D.T)
+    aux = fastdfcderi._basis(mol, 'aux')
+    ao = fastdfcderi._basis(mol, 'ao')
+    linv, coef = fastdfcderi.decompose_j2c(aux)
+    A = t.tril(linv) if coef is None else coef            # cderi = A T
+    nrow, naux = A.shape
+    # rho_B and rhok from the cderi's blocks
+    rho = t.empty(nrow, dtype=t.float64, device=_dev.device())
+    rhok = t.empty((nrow, nocc, nocc), dtype=t.float64, device=_dev.device())
+    frT = fr.T.contiguous()
+    L0 = 0
+    for dense, _ in dfobj.loop():
+        nL = dense.shape[0]
+        rho[L0:L0 + nL] = (dense * D).sum(dim=(1, 2))
+        rhok[L0:L0 + nL] = t.matmul(frT, t.matmul(dense, fl))
+        L0 += nL
+    if L0 != nrow:
+        raise ValueError(f'jk_energy_per_atom: the cderi has {L0} rows, the metric factor {nrow}')
+    c = A.T @ rho                                               # (naux,)
+    dmoo = (A.T @ rhok.reshape(nrow, nocc * nocc)).reshape(naux, nocc, nocc)
+    del rhok
+    Z = (-.5 * j_factor) * t.outer(c, c)
+    if k_factor != 0:
+        dm_flat = dmoo.reshape(naux, -1)
+        dmT_flat = dmoo.transpose(1, 2).contiguous().reshape(naux, -1)
+        Z += (.25 * k_factor) * (dm_flat @ dmT_flat.T)
+    # W in the cderi's compact columns
+    addr = np.asarray(dfobj._cderi_idx[0])
+    pairs, class_off, cmap = _class_sorted_pairs(mol, addr)
+    ncol = len(addr)
+    W = t.empty((naux, ncol), dtype=t.float64, device=_dev.device())
+    addr_d = _dev.to_device(addr.astype(np.int64), dtype=t.int64)
+    blk = max(1, min(naux, (1 << 28) // max(8 * nao * nao, 1)))
+    Dflat = D.reshape(-1)[addr_d]
+    for P0 in range(0, naux, blk):
+        P1 = min(naux, P0 + blk)
+        Wp = (j_factor * c[P0:P1, None]) * Dflat[None, :]
+        if k_factor != 0:
+            dense = t.matmul(t.matmul(fl, dmoo[P0:P1]), frT)  # (nP, nao, nao)
+            Wp -= (.5 * k_factor) * dense.reshape(P1 - P0, -1)[:, addr_d]
+        W[P0:P1] = Wp
+    del dmoo
+    grad = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())
+    grad_aux = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())
+    i32 = lambda a: _dev.to_device(np.ascontiguousarray(a, dtype=np.int32), dtype=t.int32)
+    pairs_d = i32(pairs.reshape(-1)) if len(pairs) else t.zeros(2, dtype=t.int32, device=_dev.device())
+    cmap_d = i32(cmap)
+    ao_atom, aux_atom = i32(mol.ao.shl_atom), i32(mol.aux.shl_atom)
+    lp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+    if ao.desc.lmax > ints.LMAX_AO or aux.desc.lmax > ints.LMAX_AUX:
+        raise ValueError('jk_energy_per_atom: angular momentum beyond the generated classes')
+    _lib.call('mdf_int3c2e_ip', ctypes.byref(ao.desc), ctypes.byref(aux.desc), _dev.ptr(pairs_d),
+              lp(class_off), _dev.ptr(aux.sorted), lp(aux.l_off), _dev.ptr(W), W.stride(0),
+              _dev.ptr(cmap_d), _dev.ptr(ao_atom), _dev.ptr(aux_atom), _dev.ptr(grad), _dev.ptr(grad_aux),
+              _dev.stream_ptr())
+    if auxbasis_response:
+        Zc = Z.contiguous()
+        _lib.call('mdf_int2c2e_ip', ctypes.byref(aux.desc), _dev.ptr(aux.sorted), lp(aux.l_off),
+                  _dev.ptr(Zc), Zc.stride(0), _dev.ptr(aux_atom), _dev.ptr(grad_aux), _dev.stream_ptr())
+        grad += grad_aux
+    STATS['calls'] += 1
+    return grad.cpu().numpy()

@@ -1,0 +1,145 @@
+"""The DF gradient's two-electron term (electrolites/fastdfgrad.py
+``jk_energy_per_atom`` :1229; the reference's own test compares it with
+GPU4PySCF's function and checks the whole gradient by finite differences,
+tests/test_dfgrad.py:1-40).
+
+GPU4PySCF is not here, so the check is the finite-difference one: the
+derivative of the oracle's DF energy
+
+    E_J = 1/2 rho^T M^-1 rho,   E_K = -1/4 sum_PQ M^-1_PQ tr(T_P D T_Q D)
+
+(rho_P = sum_ij (ij|P) D_ij, T_P = (ij|P), M = (P|Q), all from the oracle's
+integrals) under a displacement of one atom's shells in both bases, by a
+five-point stencil.  The energy restatement itself is pinned on the CPU
+against the oracle's J/K from its cderi (1/2 tr(D J) - 1/4 tr(D K))."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def _copy(d):
+    return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in d.items()}
+
+
+def energy_parts(ao, aux, D):
+    """(E_J, E_K) of the oracle's density-fitted energy for density D."""
+    T = O.int3c2e(ao, aux)
+    M = O.int2c2e(aux)
+    rho = np.einsum('ijP,ij->P', T, D)
+    ej = .5 * rho @ np.linalg.solve(M, rho)
+    naux = T.shape[2]
+    G = np.einsum('ijP,jk->Pik', T, D)                 # T_P D
+    X = G.reshape(naux, -1) @ G.transpose(0, 2, 1).reshape(naux, -1).T
+    ek = -.25 * np.trace(np.linalg.solve(M, X))
+    return ej, ek
+
+
+def fd_gradient(mol, D, h=2e-3, atoms=None):
+    """(natm, 3, 2) five-point derivative of (E_J, E_K) per atom coordinate."""
+    ao0, aux0 = mol.ao.as_dict(), mol.aux.as_dict()
+    atoms = range(mol.natm) if atoms is None else atoms
+    out = np.zeros((mol.natm, 3, 2))
+    for a in atoms:
+        for x in range(3):
+            vals = []
+            for s in (-2, -1, 1, 2):
+                ao, aux = _copy(ao0), _copy(aux0)
+                ao['xyz'][ao['atom'] == a, x] += s * h
+                aux['xyz'][aux['atom'] == a, x] += s * h
+                vals.append(np.array(energy_parts(ao, aux, D)))
+            out[a, x] = (vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h)
+    return out
+
+
+def _density(nao, nocc, seed):
+    rng = np.random.default_rng(seed)
+    c = np.linalg.qr(rng.standard_normal((nao, nocc)))[0]
+    return 2 * c @ c.T, c
+
+
+@pytest.fixture(scope='module')
+def mol():
+    from electrolites_amd.gto import water_cluster
+    return water_cluster(2, seed=3, side=4.0, li=True)
+
+
+def test_energy_restatement_matches_oracle_jk(mol):
+    """1/2 rho M^-1 rho - 1/4 tr(M^-1 X) equals 1/2 tr(D J) - 1/4 tr(D K)
+    of the oracle's cderi path (the energy the finite differences use)."""
+    ao, aux = mol.ao.as_dict(), mol.aux.as_dict()
+    D, _ = _density(mol.nao, 6, 1)
+    ej, ek = energy_parts(ao, aux, D)
+    cderi = O.cholesky_eri(O.int3c2e(ao, aux), O.int2c2e(aux))
+    pa, _ = O.pair_addresses(mol.nao)
+    vj, vk = O.get_jk(cderi, pa, D[None], hermi=1)
+    assert abs(ej - .5 * np.sum(D * vj[0])) < 1e-10 * abs(ej)
+    assert abs(ek + .25 * np.sum(D * vk[0])) < 1e-10 * abs(ek)
+
+
+def test_fd_harness_translation_invariance():
+    """The finite-difference gradient of a free molecule sums to zero over
+    atoms (a displaced-shell bookkeeping mistake breaks it)."""
+    from electrolites_amd.gto import water_cluster
+    m = water_cluster(1, seed=0)
+    D, _ = _density(m.nao, 5, 2)
+    g = fd_gradient(m, D)
+    assert np.abs(g.sum(axis=0)).max() < 1e-8
+    assert np.abs(g).max() > 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('tagged', [True, False])
+def test_gradient_matches_finite_differences(mol, tagged):
+    """J-only, K-only and J - 1/2 K gradients against the five-point
+    derivative of the oracle's energy, to 1e-7 Eh/bohr."""
+    from electrolites_amd import fastdf, fastdfgrad
+    from electrolites_amd.df import DF
+    D, c = _density(mol.nao, 8, 4)
+    fd = fd_gradient(mol, D)
+    dm = fastdf.tag_array(D, mo_coeff=c, mo_occ=np.full(8, 2.)) if tagged else D
+    df = DF(mol).build()
+    for jf, kf in [(1., 0.), (0., 1.), (1., .5)]:
+        g = fastdfgrad.jk_energy_per_atom(df, dm, j_factor=jf, k_factor=kf)
+        ref = jf * fd[..., 0] + kf * fd[..., 1]
+        err = np.abs(g - ref).max()
+        print(f'j={jf} k={kf} max|g - fd| = {err:.2e}, max|g| = {np.abs(g).max():.3f}', flush=True)
+        assert err < 1e-7
+        assert np.abs(g.sum(axis=0)).max() < 1e-9          # translation invariance
+
+
+@pytest.mark.gpu
+def test_gradient_without_aux_response(mol):
+    """Without the fitting functions' response only the orbital centres'
+    share of the three-centre term remains; it equals the full gradient
+    less the aux centres' shares (the metric term and the aux centre of the
+    three-centre term), which a free molecule's sum rule fixes: the orbital
+    share alone no longer sums to zero."""
+    from electrolites_amd import fastdfgrad
+    from electrolites_amd.df import DF
+    D, _ = _density(mol.nao, 8, 5)
+    df = DF(mol).build()
+    full = fastdfgrad.jk_energy_per_atom(df, D, 1., .5)
+    orb = fastdfgrad.jk_energy_per_atom(df, D, 1., .5, auxbasis_response=False)
+    assert np.abs(full.sum(axis=0)).max() < 1e-9
+    assert np.abs(orb.sum(axis=0)).max() > 1e-4
+    assert np.abs(full - orb).max() > 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('storage,dtype,tol', [('host', 'float64', 1e-11), ('device', 'float32', 1e-5)])
+def test_gradient_storage_modes(mol, monkeypatch, storage, dtype, tol):
+    """The gradient from a host-resident cderi equals the device one (the
+    same rows, streamed); from FP32 storage it stays within the narrow
+    tensor's rounding."""
+    import electrolites_amd.fastdfcderi as FC
+    from electrolites_amd import fastdfgrad
+    from electrolites_amd.df import DF
+    D, _ = _density(mol.nao, 8, 6)
+    ref = fastdfgrad.jk_energy_per_atom(DF(mol).build(), D, 1., .5)
+    monkeypatch.setattr(FC, 'FP32', dtype == 'float32')
+    df = DF(mol, storage=storage).build()
+    g = fastdfgrad.jk_energy_per_atom(df, D, 1., .5)
+    assert np.abs(g - ref).max() < tol * max(np.abs(ref).max(), 1.)
+    if storage == 'host':
+        df.reset()
