@@ -82,6 +82,29 @@ def _deriv(d, a, b, k, wrt):
     return e
 
 
+KCH = 16      # aux shells one thread of a three-centre kernel loops over
+QCH = 16      # aux shells one thread of a two-centre kernel loops over
+LDS_MAX_ATOMS = 1024
+
+
+def _head(w, indent='    '):
+    w(indent + 'extern __shared__ double sg[];')
+    w(indent + 'if (use_lds) {')
+    w(indent + f'    for (int i = threadIdx.x; i < nslot * natm; i += {THREADS}) sg[i] = 0.;')
+    w(indent + '    __syncthreads();')
+    w(indent + '}')
+
+
+def _tail(w, indent='    '):
+    w(indent + 'if (use_lds) {')
+    w(indent + '    __syncthreads();')
+    w(indent + f'    for (int i = threadIdx.x; i < nslot * natm; i += {THREADS}) {{')
+    w(indent + '        const double v = sg[i];')
+    w(indent + '        if (v != 0.) atomicAdd(i < 3 * natm ? grad + i : grad_aux + (i - 3 * natm), v);')
+    w(indent + '    }')
+    w(indent + '}')
+
+
 def gen3(la, lb, lc):
     name = f'int3c_ip_{la}{lb}{lc}'
     lab = la + lb
@@ -95,27 +118,39 @@ def gen3(la, lb, lc):
     w(f'__global__ __launch_bounds__({THREADS}) void {name}(const BasisDev ao, const BasisDev aux, '
       'const int *__restrict__ pairs, int np, const int *__restrict__ kl, int nk, '
       'const double *__restrict__ W, long ldw, const int *__restrict__ cmap, '
-      'const int *__restrict__ ao_atom, const int *__restrict__ aux_atom, double *grad, double *grad_aux) {')
+      'const int *__restrict__ ao_atom, const int *__restrict__ aux_atom, int natm, int use_lds, '
+      'double *grad, double *grad_aux) {')
+    w('    const int nslot = 6;')
+    _head(w)
+    w(f'    const int nkc = (nk + {KCH - 1}) / {KCH};')
     w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
-    w(f'    if (tid >= (long)np * nk * {2 * lc + 1}) return;')
+    w(f'    if (tid < (long)np * nkc * {2 * lc + 1}) {{')
     w('    const int ip = (int)(tid % np);')
     w('    const long rest = tid / np;')
-    w('    const int kq = (int)(rest % nk), m = (int)(rest / nk);')
-    w('    const int ish = pairs[2 * ip], jsh = pairs[2 * ip + 1], ksh = kl[kq];')
+    w('    const int kc = (int)(rest % nkc), m = (int)(rest / nkc);')
+    w('    const int ish = pairs[2 * ip], jsh = pairs[2 * ip + 1];')
     w('    const double Ax = ao.xyz[3 * ish], Ay = ao.xyz[3 * ish + 1], Az = ao.xyz[3 * ish + 2];')
     w('    const double Bx = ao.xyz[3 * jsh], By = ao.xyz[3 * jsh + 1], Bz = ao.xyz[3 * jsh + 2];')
     w('    const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;')
     w('    const double rab2 = ABx * ABx + ABy * ABy + ABz * ABz;')
-    w('    const double Cx = aux.xyz[3 * ksh], Cy = aux.xyz[3 * ksh + 1], Cz = aux.xyz[3 * ksh + 2];')
-    # the weight block and its Cartesian form
-    w('    const double *wrow = W + (long)(aux.aooff[ksh] + m) * ldw;')
     w('    const long i0 = ao.aooff[ish], j0 = ao.aooff[jsh];')
     w('    const double fac = ish == jsh ? 1.0 : 2.0;')
+    # the weight columns of the pair's function pairs (the same for every aux row)
     for s in range(nsa):
         for t in range(nsb):
-            w(f'    double ws_{s}_{t};')
+            w(f'    long col_{s}_{t};')
             w(f'    {{ const long i = i0 + {s}, j = j0 + {t}; const long hi = i > j ? i : j, lo = i > j ? j : i; '
-              f'const long pp = hi * (hi + 1) / 2 + lo; ws_{s}_{t} = wrow[cmap ? cmap[pp] : pp]; }}')
+              f'const long pp = hi * (hi + 1) / 2 + lo; col_{s}_{t} = cmap ? cmap[pp] : pp; }}')
+    w('    double gAx = 0., gAy = 0., gAz = 0., gBx = 0., gBy = 0., gBz = 0.;')
+    w('    double cX = 0., cY = 0., cZ = 0.;')
+    w('    int curC = -1;')
+    w('    const int npa = ao.nprim[ish], oa = ao.poff[ish];')
+    w('    const int npb = ao.nprim[jsh], ob = ao.poff[jsh];')
+    w(f'    const int k1 = min(nk, (kc + 1) * {KCH});')
+    w(f'    for (int kq = kc * {KCH}; kq < k1; kq++) {{')
+    w('    const int ksh = kl[kq];')
+    w('    const double Cx = aux.xyz[3 * ksh], Cy = aux.xyz[3 * ksh + 1], Cz = aux.xyz[3 * ksh + 2];')
+    w('    const double *wrow = W + (long)(aux.aooff[ksh] + m) * ldw;')
     for a in range(nca):
         for b in range(ncb):
             terms = []
@@ -125,11 +160,9 @@ def gen3(la, lb, lc):
                 for t in range(nsb):
                     if abs(Cb[t, b]) < 1e-15:
                         continue
-                    terms.append(f'{fmt(Ca[s, a] * Cb[t, b])} * ws_{s}_{t}')
+                    terms.append(f'{fmt(Ca[s, a] * Cb[t, b])} * wrow[col_{s}_{t}]')
             w(f'    const double wc_{a}_{b} = fac * (' + (' + '.join(terms) if terms else '0.0') + ');')
-    w('    double gAx = 0., gAy = 0., gAz = 0., gBx = 0., gBy = 0., gBz = 0.;')
-    w('    const int npa = ao.nprim[ish], oa = ao.poff[ish];')
-    w('    const int npb = ao.nprim[jsh], ob = ao.poff[jsh];')
+    w('    double hAx = 0., hAy = 0., hAz = 0., hBx = 0., hBy = 0., hBz = 0.;')
     w('    const int npc = aux.nprim[ksh], oc = aux.poff[ksh];')
     w('    for (int ia = 0; ia < npa; ia++) {')
     w('        const double ea = ao.exps[oa + ia], da = ao.coefs[oa + ia];')
@@ -173,12 +206,12 @@ def gen3(la, lb, lc):
                     tw = f'wc_{a_i}_{b_i} * {fmt(coef)}'
                     Ix, Iy, Iz = f'Ix_{ax}_{bx}_{kx}', f'Iy_{ay}_{by}_{ky}', f'Iz_{az}_{bz}_{kz}'
                     w(ind + f'    {{ const double tw = {tw};')
-                    w(ind + f'      gAx += tw * {_deriv("x", ax, bx, kx, "A")} * {Iy} * {Iz};')
-                    w(ind + f'      gAy += tw * {Ix} * {_deriv("y", ay, by, ky, "A")} * {Iz};')
-                    w(ind + f'      gAz += tw * {Ix} * {Iy} * {_deriv("z", az, bz, kz, "A")};')
-                    w(ind + f'      gBx += tw * {_deriv("x", ax, bx, kx, "B")} * {Iy} * {Iz};')
-                    w(ind + f'      gBy += tw * {Ix} * {_deriv("y", ay, by, ky, "B")} * {Iz};')
-                    w(ind + f'      gBz += tw * {Ix} * {Iy} * {_deriv("z", az, bz, kz, "B")}; }}')
+                    w(ind + f'      hAx += tw * {_deriv("x", ax, bx, kx, "A")} * {Iy} * {Iz};')
+                    w(ind + f'      hAy += tw * {Ix} * {_deriv("y", ay, by, ky, "A")} * {Iz};')
+                    w(ind + f'      hAz += tw * {Ix} * {Iy} * {_deriv("z", az, bz, kz, "A")};')
+                    w(ind + f'      hBx += tw * {_deriv("x", ax, bx, kx, "B")} * {Iy} * {Iz};')
+                    w(ind + f'      hBy += tw * {Ix} * {_deriv("y", ay, by, ky, "B")} * {Iz};')
+                    w(ind + f'      hBz += tw * {Ix} * {Iy} * {_deriv("z", az, bz, kz, "B")}; }}')
         w(ind + '    break;')
         w(ind + '}')
     w(ind + '}')
@@ -186,11 +219,19 @@ def gen3(la, lb, lc):
     w('            }')
     w('        }')
     w('    }')
-    w('    const int aA = ao_atom[ish], aB = ao_atom[jsh], aC = aux_atom[ksh];')
-    w('    atomicAdd(grad + 3 * aA, gAx); atomicAdd(grad + 3 * aA + 1, gAy); atomicAdd(grad + 3 * aA + 2, gAz);')
-    w('    atomicAdd(grad + 3 * aB, gBx); atomicAdd(grad + 3 * aB + 1, gBy); atomicAdd(grad + 3 * aB + 2, gBz);')
-    w('    atomicAdd(grad_aux + 3 * aC, -(gAx + gBx)); atomicAdd(grad_aux + 3 * aC + 1, -(gAy + gBy)); '
-      'atomicAdd(grad_aux + 3 * aC + 2, -(gAz + gBz));')
+    w('    gAx += hAx; gAy += hAy; gAz += hAz; gBx += hBx; gBy += hBy; gBz += hBz;')
+    w('    const int aC = aux_atom[ksh];')
+    w('    if (aC != curC) {')
+    w('        if (curC >= 0) ip_add(sg, use_lds, natm, grad_aux, curC, 1, cX, cY, cZ);')
+    w('        curC = aC; cX = 0.; cY = 0.; cZ = 0.;')
+    w('    }')
+    w('    cX -= hAx + hBx; cY -= hAy + hBy; cZ -= hAz + hBz;')
+    w('    }')          # kq
+    w('    ip_add(sg, use_lds, natm, grad, ao_atom[ish], 0, gAx, gAy, gAz);')
+    w('    ip_add(sg, use_lds, natm, grad, ao_atom[jsh], 0, gBx, gBy, gBz);')
+    w('    if (curC >= 0) ip_add(sg, use_lds, natm, grad_aux, curC, 1, cX, cY, cZ);')
+    w('    }')          # tid
+    _tail(w)
     w('}')
     return name, '\n'.join(L)
 
@@ -205,22 +246,32 @@ def gen2(la, lc):
     w = L.append
     w(f'__global__ __launch_bounds__({THREADS}) void {name}(const BasisDev aux, '
       'const int *__restrict__ pl, int np, const int *__restrict__ ql, int nq, '
-      'const double *__restrict__ Z, long ldz, const int *__restrict__ aux_atom, double *__restrict__ grad) {')
+      'const double *__restrict__ Z, long ldz, const int *__restrict__ aux_atom, int natm, int use_lds, '
+      'double *grad) {')
+    w('    const int nslot = 3;')
+    w('    double *grad_aux = grad;')
+    _head(w)
+    w(f'    const int nqc = (nq + {QCH - 1}) / {QCH};')
     w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
-    w(f'    if (tid >= (long)np * nq * {2 * lc + 1}) return;')
+    w(f'    if (tid < (long)np * nqc * {2 * lc + 1}) {{')
     w('    const int ip = (int)(tid % np);')
     w('    const long rest = tid / np;')
-    w('    const int kq = (int)(rest % nq), m = (int)(rest / nq);')
-    w('    const int ish = pl[ip], ksh = ql[kq];')
+    w('    const int qc = (int)(rest % nqc), m = (int)(rest / nqc);')
+    w('    const int ish = pl[ip];')
     w('    const double Ax = aux.xyz[3 * ish], Ay = aux.xyz[3 * ish + 1], Az = aux.xyz[3 * ish + 2];')
+    w('    const long r0 = aux.aooff[ish];')
+    w('    const int npa = aux.nprim[ish], oa = aux.poff[ish];')
+    w('    double gx = 0., gy = 0., gz = 0., cX = 0., cY = 0., cZ = 0.;')
+    w('    int curQ = -1;')
+    w(f'    const int q1 = min(nq, (qc + 1) * {QCH});')
+    w(f'    for (int kq = qc * {QCH}; kq < q1; kq++) {{')
+    w('    const int ksh = ql[kq];')
     w('    const double Cx = aux.xyz[3 * ksh], Cy = aux.xyz[3 * ksh + 1], Cz = aux.xyz[3 * ksh + 2];')
-    w('    const double ABx = 0., ABy = 0., ABz = 0.;')
-    w('    const long col = aux.aooff[ksh] + m, r0 = aux.aooff[ish];')
+    w('    const long col = aux.aooff[ksh] + m;')
     for a in range(len(ca)):
         terms = [f'{fmt(Ca[s, a])} * Z[(r0 + {s}) * ldz + col]' for s in range(nsa) if abs(Ca[s, a]) > 1e-15]
         w(f'    const double wc_{a} = ' + (' + '.join(terms) if terms else '0.0') + ';')
-    w('    double gx = 0., gy = 0., gz = 0.;')
-    w('    const int npa = aux.nprim[ish], oa = aux.poff[ish];')
+    w('    double hx = 0., hy = 0., hz = 0.;')
     w('    const int npc = aux.nprim[ksh], oc = aux.poff[ksh];')
     w('    for (int ia = 0; ia < npa; ia++) {')
     w('        const double ea = aux.exps[oa + ia], fab = aux.coefs[oa + ia];')
@@ -256,18 +307,27 @@ def gen2(la, lc):
                 kx, ky, kz = cc[k]
                 Ix, Iy, Iz = f'Ix_{ax}_0_{kx}', f'Iy_{ay}_0_{ky}', f'Iz_{az}_0_{kz}'
                 w(ind + f'    {{ const double tw = wc_{a_i} * {fmt(coef)};')
-                w(ind + f'      gx += tw * {_deriv("x", ax, 0, kx, "A")} * {Iy} * {Iz};')
-                w(ind + f'      gy += tw * {Ix} * {_deriv("y", ay, 0, ky, "A")} * {Iz};')
-                w(ind + f'      gz += tw * {Ix} * {Iy} * {_deriv("z", az, 0, kz, "A")}; }}')
+                w(ind + f'      hx += tw * {_deriv("x", ax, 0, kx, "A")} * {Iy} * {Iz};')
+                w(ind + f'      hy += tw * {Ix} * {_deriv("y", ay, 0, ky, "A")} * {Iz};')
+                w(ind + f'      hz += tw * {Ix} * {Iy} * {_deriv("z", az, 0, kz, "A")}; }}')
         w(ind + '    break;')
         w(ind + '}')
     w(ind + '}')
     w('            }')
     w('        }')
     w('    }')
-    w('    const int aP = aux_atom[ish], aQ = aux_atom[ksh];')
-    w('    atomicAdd(grad + 3 * aP, gx); atomicAdd(grad + 3 * aP + 1, gy); atomicAdd(grad + 3 * aP + 2, gz);')
-    w('    atomicAdd(grad + 3 * aQ, -gx); atomicAdd(grad + 3 * aQ + 1, -gy); atomicAdd(grad + 3 * aQ + 2, -gz);')
+    w('    gx += hx; gy += hy; gz += hz;')
+    w('    const int aQ = aux_atom[ksh];')
+    w('    if (aQ != curQ) {')
+    w('        if (curQ >= 0) ip_add(sg, use_lds, natm, grad, curQ, 0, cX, cY, cZ);')
+    w('        curQ = aQ; cX = 0.; cY = 0.; cZ = 0.;')
+    w('    }')
+    w('    cX -= hx; cY -= hy; cZ -= hz;')
+    w('    }')          # kq
+    w('    ip_add(sg, use_lds, natm, grad, aux_atom[ish], 0, gx, gy, gz);')
+    w('    if (curQ >= 0) ip_add(sg, use_lds, natm, grad, curQ, 0, cX, cY, cZ);')
+    w('    }')          # tid
+    _tail(w)
     w('}')
     return name, '\n'.join(L)
 
@@ -275,7 +335,16 @@ def gen2(la, lc):
 def main():
     parts = ['// Generated by electrolites_amd/codegen/gen_int_ip.py; do not edit.',
              '// Nuclear-derivative Rys kernels of the DF gradient, one per angular class.',
-             '#include "mdf_int.h"', '', 'namespace mdf {', '']
+             '#include "mdf_int.h"', '', 'namespace mdf {', '',
+             '// one atom\'s (x, y, z) into slot part (0 orbital centres, 1 aux centres) of the',
+             '// block\'s LDS sums, or straight into the gradient g without them',
+             '__device__ __forceinline__ void ip_add(double *sg, int use_lds, int natm, double *g, int atom, '
+             'int part, double x, double y, double z) {',
+             '    double *d = use_lds ? sg + 3 * (part * natm + atom) : g + 3 * atom;',
+             '    atomicAdd(d, x);',
+             '    atomicAdd(d + 1, y);',
+             '    atomicAdd(d + 2, z);',
+             '}', '']
     n3, n2 = {}, {}
     for la in range(LMAX_AO + 1):
         for lb in range(LMAX_AO + 1):
@@ -290,15 +359,17 @@ def main():
             parts += [src, '']
     parts.append('int launch_int3c_ip(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, '
                  'const int *pairs, int np, const int *kl, int nk, const double *W, long ldw, const int *cmap, '
-                 'const int *ao_atom, const int *aux_atom, double *grad, double *grad_aux, hipStream_t st) {')
-    parts.append('    const long n = (long)np * nk * (2 * lc + 1);')
+                 'const int *ao_atom, const int *aux_atom, int natm, double *grad, double *grad_aux, hipStream_t st) {')
+    parts.append(f'    const long n = (long)np * ((nk + {KCH - 1}) / {KCH}) * (2 * lc + 1);')
+    parts.append(f'    const int use_lds = natm <= {LDS_MAX_ATOMS};')
+    parts.append('    const size_t shm = use_lds ? 6 * (size_t)natm * sizeof(double) : 0;')
     parts.append('    if (n == 0) return MDF_OK;')
     parts.append(f'    const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})), b({THREADS});')
     parts.append(f'    switch ((la * {LMAX_AO + 1} + lb) * {LMAX_AUX + 1} + lc) {{')
     for (la, lb, lc), n in n3.items():
         key = (la * (LMAX_AO + 1) + lb) * (LMAX_AUX + 1) + lc
-        parts.append(f'    case {key}: hipLaunchKernelGGL({n}, g, b, 0, st, ao, aux, pairs, np, kl, nk, W, ldw, '
-                     'cmap, ao_atom, aux_atom, grad, grad_aux); break;')
+        parts.append(f'    case {key}: hipLaunchKernelGGL({n}, g, b, shm, st, ao, aux, pairs, np, kl, nk, W, ldw, '
+                     'cmap, ao_atom, aux_atom, natm, use_lds, grad, grad_aux); break;')
     parts.append('    default: set_error("int3c2e_ip: class (%d %d|%d) not generated", la, lb, lc); return MDF_EINVAL;')
     parts.append('    }')
     parts.append('    MDF_LAUNCHED("int3c_ip");')
@@ -306,14 +377,16 @@ def main():
     parts.append('}')
     parts.append('')
     parts.append('int launch_int2c_ip(int la, int lc, const BasisDev &aux, const int *pl, int np, const int *ql, '
-                 'int nq, const double *Z, long ldz, const int *aux_atom, double *grad, hipStream_t st) {')
-    parts.append('    const long n = (long)np * nq * (2 * lc + 1);')
+                 'int nq, const double *Z, long ldz, const int *aux_atom, int natm, double *grad, hipStream_t st) {')
+    parts.append(f'    const long n = (long)np * ((nq + {QCH - 1}) / {QCH}) * (2 * lc + 1);')
+    parts.append(f'    const int use_lds = natm <= {LDS_MAX_ATOMS};')
+    parts.append('    const size_t shm = use_lds ? 3 * (size_t)natm * sizeof(double) : 0;')
     parts.append('    if (n == 0) return MDF_OK;')
     parts.append(f'    const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})), b({THREADS});')
     parts.append(f'    switch (la * {LMAX_AUX + 1} + lc) {{')
     for (la, lc), n in n2.items():
-        parts.append(f'    case {la * (LMAX_AUX + 1) + lc}: hipLaunchKernelGGL({n}, g, b, 0, st, aux, pl, np, ql, nq, '
-                     'Z, ldz, aux_atom, grad); break;')
+        parts.append(f'    case {la * (LMAX_AUX + 1) + lc}: hipLaunchKernelGGL({n}, g, b, shm, st, aux, pl, np, ql, nq, '
+                     'Z, ldz, aux_atom, natm, use_lds, grad); break;')
     parts.append('    default: set_error("int2c2e_ip: class (%d|%d) not generated", la, lc); return MDF_EINVAL;')
     parts.append('    }')
     parts.append('    MDF_LAUNCHED("int2c_ip");')

@@ -27,8 +27,8 @@ int launch_int2c(int la, int lc, const BasisDev &aux, const int *pl, int np, con
                  double *out, long ldo, hipStream_t st);
 int launch_int3c_ip(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, const int *pairs, int np,
                     const int *kl, int nk, const double *W, long ldw, const int *cmap, const int *ao_atom,
-                    const int *aux_atom, double *grad, double *grad_aux, hipStream_t st);
+                    const int *aux_atom, int natm, double *grad, double *grad_aux, hipStream_t st);
 int launch_int2c_ip(int la, int lc, const BasisDev &aux, const int *pl, int np, const int *ql, int nq,
-                    const double *Z, long ldz, const int *aux_atom, double *grad, hipStream_t st);
+                    const double *Z, long ldz, const int *aux_atom, int natm, double *grad, hipStream_t st);
 
 }  // namespace mdf

@@ -79,9 +79,10 @@ extern "C" int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const lo
 extern "C" int mdf_int3c2e_ip(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
                               const long *pair_class_off, const int *aux_sorted, const long *aux_l_off,
                               const double *W, long ldw, const int *col_map, const int *ao_atom,
-                              const int *aux_atom, double *grad, double *grad_aux, void *stream) {
+                              const int *aux_atom, int natm, double *grad, double *grad_aux, void *stream) {
     MDF_CHECK_ARG(ao && aux && pair_class_off && aux_l_off && W && ao_atom && aux_atom && grad && grad_aux,
                   "mdf_int3c2e_ip: null argument");
+    MDF_CHECK_ARG(natm > 0, "mdf_int3c2e_ip: natm must be positive");
     MDF_CHECK_ARG(ao->lmax <= LMAX_AO && aux->lmax <= LMAX_AUX,
                   "mdf_int3c2e_ip: angular momentum beyond the generated classes (ao %d > %d or aux %d > %d)",
                   ao->lmax, LMAX_AO, aux->lmax, LMAX_AUX);
@@ -98,7 +99,7 @@ extern "C" int mdf_int3c2e_ip(const mdf_basis *ao, const mdf_basis *aux, const i
                 const long k0 = aux_l_off[lc], k1 = aux_l_off[lc + 1];
                 if (k1 <= k0) continue;
                 int rc = launch_int3c_ip(la, lb, lc, a, x, pairs + 2 * q0, (int)(q1 - q0), aux_sorted + k0,
-                                         (int)(k1 - k0), W, ldw, col_map, ao_atom, aux_atom, grad, grad_aux, st);
+                                         (int)(k1 - k0), W, ldw, col_map, ao_atom, aux_atom, natm, grad, grad_aux, st);
                 if (rc) return rc;
             }
         }
@@ -109,8 +110,10 @@ extern "C" int mdf_int3c2e_ip(const mdf_basis *ao, const mdf_basis *aux, const i
 // sum_{P, Q} Z[P, Q] d(P|Q)/dR_atom,x, every ordered pair of aux shells
 // (GPU4PySCF's int2c2e_ip1_per_atom, electrolites/fastdfgrad.py:1172-1178).
 extern "C" int mdf_int2c2e_ip(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_off,
-                              const double *Z, long ldz, const int *aux_atom, double *grad, void *stream) {
+                              const double *Z, long ldz, const int *aux_atom, int natm, double *grad,
+                              void *stream) {
     MDF_CHECK_ARG(aux && aux_l_off && Z && aux_atom && grad, "mdf_int2c2e_ip: null argument");
+    MDF_CHECK_ARG(natm > 0, "mdf_int2c2e_ip: natm must be positive");
     MDF_CHECK_ARG(aux->lmax <= LMAX_AUX, "mdf_int2c2e_ip: aux angular momentum %d > %d", aux->lmax, LMAX_AUX);
     hipStream_t st = as_stream(stream);
     const BasisDev x = to_dev(aux);
@@ -121,7 +124,7 @@ extern "C" int mdf_int2c2e_ip(const mdf_basis *aux, const int *aux_sorted, const
             const long k0 = aux_l_off[lc], k1 = aux_l_off[lc + 1];
             if (k1 <= k0) continue;
             int rc = launch_int2c_ip(la, lc, x, aux_sorted + p0, (int)(p1 - p0), aux_sorted + k0,
-                                     (int)(k1 - k0), Z, ldz, aux_atom, grad, st);
+                                     (int)(k1 - k0), Z, ldz, aux_atom, natm, grad, st);
             if (rc) return rc;
         }
     }

@@ -92,14 +92,40 @@ def _factors(dm, hermi):
     return fl.contiguous(), fr.contiguous()
 
 
-def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_response=True):
+class _Stages:
+    """Device time per stage from HIP events on the current stream, into
+    the caller's dict (ms); a no-op without one."""
+
+    def __init__(self, out):
+        self.out, self.ev = out, []
+
+    def mark(self, name):
+        if self.out is None:
+            return
+        t = _dev.torch()
+        e = t.cuda.Event(enable_timing=True)
+        e.record()
+        self.ev.append((name, e))
+
+    def close(self):
+        if self.out is None:
+            return
+        self.ev[-1][1].synchronize()
+        for (n, a), (_, b) in zip(self.ev, self.ev[1:]):
+            self.out[n] = self.out.get(n, 0.) + a.elapsed_time(b)
+
+
+def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_response=True, timings=None):
     """``(natm, 3)`` nuclear derivative of ``1/2 j tr(D J) - 1/4 k tr(D K)``
     for the restricted density ``dm`` (nao x nao; tagged with
     ``mo_coeff``/``mo_occ`` or not), from the built cderi of ``dfobj``
     (electrolites/fastdfgrad.py:1229).  ``auxbasis_response`` adds the
     fitting functions' centres (the aux-centre share of the three-centre
-    term and the metric term), as GPU4PySCF's flag of that name."""
+    term and the metric term), as GPU4PySCF's flag of that name.
+    ``timings``: a dict that receives the device milliseconds of each stage
+    ('metric', 'rhok', 'weights', 'ip3c', 'ip2c')."""
     t = _dev.torch()
+    tm = _Stages(timings)
     _dev.require_gpu()
     mol = dfobj.mol
     if getattr(dfobj, '_cderi', None) is None:
@@ -111,6 +137,7 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     D = .5 * (D + D.T)
     aux = fastdfcderi._basis(mol, 'aux')
     ao = fastdfcderi._basis(mol, 'ao')
+    tm.mark('metric')
     linv, coef = fastdfcderi.decompose_j2c(aux)
     A = t.tril(linv) if coef is None else coef            # cderi = A T
     nrow, naux = A.shape
@@ -118,6 +145,7 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     rho = t.empty(nrow, dtype=t.float64, device=_dev.device())
     rhok = t.empty((nrow, nocc, nocc), dtype=t.float64, device=_dev.device())
     frT = fr.T.contiguous()
+    tm.mark('rhok')
     L0 = 0
     for dense, _ in dfobj.loop():
         nL = dense.shape[0]
@@ -126,6 +154,7 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
         L0 += nL
     if L0 != nrow:
         raise ValueError(f'jk_energy_per_atom: the cderi has {L0} rows, the metric factor {nrow}')
+    tm.mark('weights')
     c = A.T @ rho                                               # (naux,)
     dmoo = (A.T @ rhok.reshape(nrow, nocc * nocc)).reshape(naux, nocc, nocc)
     del rhok
@@ -159,14 +188,18 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     lp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
     if ao.desc.lmax > ints.LMAX_AO or aux.desc.lmax > ints.LMAX_AUX:
         raise ValueError('jk_energy_per_atom: angular momentum beyond the generated classes')
+    tm.mark('ip3c')
     _lib.call('mdf_int3c2e_ip', ctypes.byref(ao.desc), ctypes.byref(aux.desc), _dev.ptr(pairs_d),
               lp(class_off), _dev.ptr(aux.sorted), lp(aux.l_off), _dev.ptr(W), W.stride(0),
-              _dev.ptr(cmap_d), _dev.ptr(ao_atom), _dev.ptr(aux_atom), _dev.ptr(grad), _dev.ptr(grad_aux),
+              _dev.ptr(cmap_d), _dev.ptr(ao_atom), _dev.ptr(aux_atom), natm, _dev.ptr(grad), _dev.ptr(grad_aux),
               _dev.stream_ptr())
+    tm.mark('ip2c')
     if auxbasis_response:
         Zc = Z.contiguous()
         _lib.call('mdf_int2c2e_ip', ctypes.byref(aux.desc), _dev.ptr(aux.sorted), lp(aux.l_off),
-                  _dev.ptr(Zc), Zc.stride(0), _dev.ptr(aux_atom), _dev.ptr(grad_aux), _dev.stream_ptr())
+                  _dev.ptr(Zc), Zc.stride(0), _dev.ptr(aux_atom), natm, _dev.ptr(grad_aux), _dev.stream_ptr())
         grad += grad_aux
+    tm.mark('end')
+    tm.close()
     STATS['calls'] += 1
     return grad.cpu().numpy()

@@ -226,7 +226,7 @@ int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_o
  * W[P * ldw + c]: one value per packed pair p = i(i+1)/2 + j (i >= j) at
  * c = col_map[p] (the compact columns of a cderi that stores only the
  * screened-in pairs) or c = p (col_map NULL).  ao_atom / aux_atom: device
- * int32 atom of each shell; grad, grad_aux: device (natm, 3) float64,
+ * int32 atom (< natm) of each shell; grad, grad_aux: device (natm, 3) float64,
  * accumulated (zero them first; they may be the same array).  pairs and
  * aux_sorted as for mdf_int3c2e.  Replaces GPU4PySCF's
  * sum_ejk_int3c2e_ip1 in the reference's DF gradient
@@ -234,13 +234,13 @@ int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_o
 int mdf_int3c2e_ip(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
                    const long *pair_class_off, const int *aux_sorted, const long *aux_l_off,
                    const double *W, long ldw, const int *col_map, const int *ao_atom,
-                   const int *aux_atom, double *grad, double *grad_aux, void *stream);
+                   const int *aux_atom, int natm, double *grad, double *grad_aux, void *stream);
 
 /* Nuclear derivative of the metric term: grad[3a + x] += sum_{P,Q} Z[P * ldz
  * + Q] d(P|Q)/dR_{a,x} over all ordered aux function pairs.  Replaces
  * GPU4PySCF's int2c2e_ip1_per_atom (electrolites/fastdfgrad.py:1172-1178). */
 int mdf_int2c2e_ip(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_off,
-                   const double *Z, long ldz, const int *aux_atom, double *grad, void *stream);
+                   const double *Z, long ldz, const int *aux_atom, int natm, double *grad, void *stream);
 
 /* Overlap, kinetic and nuclear attraction (natm point charges, charges[] and
  * atom_xyz[3*natm] device arrays) over the listed shell pairs, written to
