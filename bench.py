@@ -9,6 +9,10 @@ int2c2e/int3c2e, Cholesky, L^-1, triangular multiply) and its build is timed
 per stage; ``--synthetic-cderi`` replaces it with seeded N(0,1) values over
 the full triangle (65.9 GB) for quick kernel timing.
 
+``df_gradient_2e`` times the DF gradient's two-electron term (J - K/2 of
+the same density) from the same cderi, per stage (metric factor, rhok,
+weights, derivative integrals).
+
 Prints ONE JSON line.  ``roofline`` is the K build (the dominant kernel
 group) against FP64 MFMA peak; ``roofline_j`` is the J build against HBM
 bandwidth.  Per-call durations come from HIP events recorded on the stream
@@ -49,6 +53,8 @@ def parse():
     ap.add_argument('--rates', default='0.5,2,8', help='serve: Poisson rates (req/s)')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-host', action='store_true', help='getjk: skip the host-resident (streamed) line')
+    ap.add_argument('--no-grad', action='store_true',
+                    help='getjk: skip the DF gradient two-electron term line')
     ap.add_argument('--host-frac', type=float, default=.25,
                     help='getjk: share of the aux rows moved to page-locked host memory for the streamed line')
     ap.add_argument('--graphs', action='store_true', help='serve: replay grouped runs as HIP graphs')
@@ -499,6 +505,29 @@ def main():
             f'{t_dev * 1e3:.1f} ms; rel diff {host_line["rel_diff_vs_device"]:.1e}')
         del hj, hk, dj, dk, hdf, ddf, hpart, dev_part
 
+    # the DF gradient's two-electron term (J - K/2 of the same density) from
+    # the same cderi: one warm call, then one timed call with its stages
+    grad_line = None
+    if not args.no_grad and not args.synthetic_cderi:
+        from electrolites_amd import fastdfgrad
+        from electrolites_amd.df import DF as _DF
+        gdf = _DF(mol)
+        gdf._cderi = {torch.cuda.current_device(): cderi}
+        gdf._cderi_idx = idx
+        fastdfgrad.jk_energy_per_atom(gdf, dm, 1., .5)
+        torch.cuda.synchronize()
+        stages = {}
+        t0 = time.perf_counter()
+        g = fastdfgrad.jk_energy_per_atom(gdf, dm, 1., .5, timings=stages)
+        t_grad = time.perf_counter() - t0
+        grad_line = {'wall_ms': t_grad * 1e3, 'stages_ms': stages, 'natm': mol.natm,
+                     'max_abs_grad': float(np.abs(g).max()),
+                     'translation_residual': float(np.abs(g.sum(axis=0)).max())}
+        log(f'DF gradient 2e term: {t_grad * 1e3:.0f} ms ({stages}); sum over atoms '
+            f'{grad_line["translation_residual"]:.1e}')
+        del gdf, g
+        torch.cuda.empty_cache()
+
     S = naux * npairs * 8
     bj = 2 * S + 16 * (npairs + naux)
     traffic = measured_traffic(args.nwater) if args.nocc == 320 else {}
@@ -525,6 +554,7 @@ def main():
         'fp32_storage': fp32_store,
         'ri_j_no_cderi': ri_j,
         'host_resident': host_line,
+        'df_gradient_2e': grad_line,
         'roofline': {'bound': 'mfma', 'achieved': fk / tk / 1e12,
                      'peak': FP64_MFMA_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': fk / tk / FP64_MFMA_PEAK, 'traffic': traffic.get('k'),

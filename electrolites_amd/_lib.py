@@ -41,6 +41,7 @@ _SIGS = {
     'mdf_j_vj': (I, [P, I, L, L, L, P, I, L, P, L, I, P, SZ, P]),
     'mdf_int3c2e': (I, [P, P, P, P, P, P, L, P, P, L, P]),
     'mdf_int2c2e': (I, [P, P, P, P, L, P]),
+    'mdf_k_step1': (I, [P, I, L, L, L, P, I, P, L, I, P, L, L, P]),
     'mdf_int3c2e_ip': (I, [P, P, P, P, P, P, P, L, P, P, P, I, P, P, P]),
     'mdf_int2c2e_ip': (I, [P, P, P, P, L, P, I, P, P]),
     'mdf_int1e': (I, [P, I, P, P, I, P, P, P, P, P, P]),

@@ -347,3 +347,30 @@ extern "C" int mdf_get_k(const void *cderi, int cderi_dtype, long naux, long npa
         return run_k<float>(p, c, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
     return run_k<double>(p, c, naux, ld, pair_map, nao, fl, fr, nocc, ldf, vk, ldvk, accumulate, work, st);
 }
+
+// The first step of mdf_get_k alone, FP64: R[L] = A_L F for the nL aux rows
+// of one block (A_L the unpacked symmetric row L), R[L * strideL + i * ldr +
+// k] for i < nao, k < nocc.  The DF gradient's rhok = F_r^T A_L F_l takes its
+// inner product from here.
+extern "C" int mdf_k_step1(const void *cderi, int cderi_dtype, long nL, long npairs, long ld, const int *pair_map,
+                           int nao, const double *f, long ldf, int nocc, double *R, long ldr, long strideL,
+                           void *stream) {
+    MDF_CHECK_ARG(nL >= 0 && nao >= 0 && nocc >= 0 && nL < (1L << 31), "mdf_k_step1: bad sizes");
+    MDF_CHECK_ARG(cderi && f && R, "mdf_k_step1: null pointer");
+    MDF_CHECK_ARG(ld >= npairs && ldf >= nocc && ldr >= nocc && strideL >= (long)nao * ldr,
+                  "mdf_k_step1: leading dimension too small");
+    MDF_CHECK_ARG(cderi_dtype == MDF_DTYPE_F64 || cderi_dtype == MDF_DTYPE_F32, "mdf_k_step1: cderi dtype %d",
+                  cderi_dtype);
+    MDF_CHECK_ARG(pair_map || npairs == (long)nao * (nao + 1) / 2,
+                  "mdf_k_step1: without a pair map the cderi must hold the full packed triangle "
+                  "(npairs=%ld, nao=%d)", npairs, nao);
+    if (nL == 0 || nao == 0 || nocc == 0) return MDF_OK;
+    KPlan p{};
+    p.bn = pick_bn(nocc);
+    hipStream_t st = as_stream(stream);
+    if (cderi_dtype == MDF_DTYPE_F32)
+        return step1<double, float>(p, (int)nL, st, (const float *)cderi, ld, pair_map, nao, f, ldf, nocc, R, ldr,
+                                    strideL);
+    return step1<double, double>(p, (int)nL, st, (const double *)cderi, ld, pair_map, nao, f, ldf, nocc, R, ldr,
+                                 strideL);
+}

@@ -28,8 +28,11 @@ contracted with W and Z inside the Rys kernels (``mdf_int3c2e_ip``,
 ``mdf_int2c2e_ip``), which add straight into the per-atom gradient.  W is
 held in the cderi's own compact column layout (one value per stored pair).
 
-The small dense contractions (rhok, the triangular products, W's blocks and
-Z) run as torch FP64 matrix products on the device.
+R_L = A_L F_l comes from the K engine's first step (``mdf_k_step1``, the
+packed rows read through the pair map) for a device-resident cderi, and from
+the unpacked blocks of :meth:`DF.loop` otherwise; the remaining small dense
+contractions (F_r^T R_L, the triangular products, W's blocks and Z) run as
+torch FP64 matrix products on the device.
 """
 import ctypes
 
@@ -42,6 +45,12 @@ from . import fastdfcderi
 from . import ints
 
 STATS = {'calls': 0}
+#: bytes of the R_L = A_L F_l block buffer
+RBLOCK_BYTES = 1 << 31
+#: bytes of one block of (nao, nocc) rows F_l dmoo_P of the weight build
+WBLOCK_BYTES = 1 << 30
+#: orbital rows per lower-triangle block of the weight build
+IBLOCK = 128
 
 
 def stats():
@@ -141,19 +150,36 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     linv, coef = fastdfcderi.decompose_j2c(aux)
     A = t.tril(linv) if coef is None else coef            # cderi = A T
     nrow, naux = A.shape
-    # rho_B and rhok from the cderi's blocks
-    rho = t.empty(nrow, dtype=t.float64, device=_dev.device())
+    # rhok from the cderi's blocks; rho_B[L] = tr(A_L D) = tr(rhok[L])
     rhok = t.empty((nrow, nocc, nocc), dtype=t.float64, device=_dev.device())
     frT = fr.T.contiguous()
     tm.mark('rhok')
-    L0 = 0
-    for dense, _ in dfobj.loop():
-        nL = dense.shape[0]
-        rho[L0:L0 + nL] = (dense * D).sum(dim=(1, 2))
-        rhok[L0:L0 + nL] = t.matmul(frT, t.matmul(dense, fl))
-        L0 += nL
-    if L0 != nrow:
-        raise ValueError(f'jk_energy_per_atom: the cderi has {L0} rows, the metric factor {nrow}')
+    cderi = fastdf._cderi_of(dfobj)
+    if isinstance(cderi, t.Tensor) and cderi.is_cuda and cderi.dtype in (t.float64, t.float32) \
+            and cderi.dim() == 2 and cderi.stride(1) == 1:
+        # R_L = A_L F_l by the K engine's first step, straight from the packed rows
+        if cderi.shape[0] != nrow:
+            raise ValueError(f'jk_energy_per_atom: the cderi has {cderi.shape[0]} rows, the metric factor {nrow}')
+        lay = fastdf._layout(dfobj, cderi, nao)
+        nb = max(1, min(nrow, RBLOCK_BYTES // max(8 * nao * nocc, 1)))
+        R = t.empty((nb, nao, nocc), dtype=t.float64, device=_dev.device())
+        code = fastdf._dtype_code(cderi)
+        for L0 in range(0, nrow, nb):
+            nL = min(nb, nrow - L0)
+            _lib.call('mdf_k_step1', _dev.ptr(cderi[L0]), code, nL, cderi.shape[1], cderi.stride(0),
+                      _dev.ptr(lay.map), nao, _dev.ptr(fl), fl.stride(0), nocc, _dev.ptr(R), nocc, nao * nocc,
+                      _dev.stream_ptr())
+            rhok[L0:L0 + nL] = t.matmul(frT, R[:nL])
+        del R
+    else:
+        L0 = 0
+        for dense, _ in dfobj.loop():
+            nL = dense.shape[0]
+            rhok[L0:L0 + nL] = t.matmul(frT, t.matmul(dense, fl))
+            L0 += nL
+        if L0 != nrow:
+            raise ValueError(f'jk_energy_per_atom: the cderi has {L0} rows, the metric factor {nrow}')
+    rho = t.diagonal(rhok, dim1=1, dim2=2).sum(dim=1)
     tm.mark('weights')
     c = A.T @ rho                                               # (naux,)
     dmoo = (A.T @ rhok.reshape(nrow, nocc * nocc)).reshape(naux, nocc, nocc)
@@ -169,14 +195,28 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     ncol = len(addr)
     W = t.empty((naux, ncol), dtype=t.float64, device=_dev.device())
     addr_d = _dev.to_device(addr.astype(np.int64), dtype=t.int64)
-    blk = max(1, min(naux, (1 << 28) // max(8 * nao * nao, 1)))
     Dflat = D.reshape(-1)[addr_d]
+    # the exchange part only on the lower triangle, in row blocks of IBLOCK:
+    # block b holds the stored pairs (i, j), i in [i0, i1), j <= i, at local
+    # index (i - i0) * i1 + j of the (i1 - i0, i1) product
+    pi, pj = np.divmod(addr.astype(np.int64), nao)
+    hi, lo = np.maximum(pi, pj), np.minimum(pi, pj)
+    tri = []
+    for i0 in range(0, nao, IBLOCK):
+        i1 = min(nao, i0 + IBLOCK)
+        sel = np.nonzero((hi >= i0) & (hi < i1))[0]
+        if len(sel):
+            tri.append((i0, i1, _dev.to_device(sel, dtype=t.int64),
+                        _dev.to_device((hi[sel] - i0) * i1 + lo[sel], dtype=t.int64)))
+    blk = max(1, min(naux, WBLOCK_BYTES // max(8 * nao * max(nocc, IBLOCK), 1)))
     for P0 in range(0, naux, blk):
         P1 = min(naux, P0 + blk)
         Wp = (j_factor * c[P0:P1, None]) * Dflat[None, :]
         if k_factor != 0:
-            dense = t.matmul(t.matmul(fl, dmoo[P0:P1]), frT)  # (nP, nao, nao)
-            Wp -= (.5 * k_factor) * dense.reshape(P1 - P0, -1)[:, addr_d]
+            Y = t.matmul(fl, dmoo[P0:P1])                       # (nP, nao, nocc)
+            for i0, i1, sel, loc in tri:
+                blkK = t.matmul(Y[:, i0:i1], frT[:, :i1])       # (nP, i1 - i0, i1)
+                Wp[:, sel] -= (.5 * k_factor) * blkK.reshape(P1 - P0, -1)[:, loc]
         W[P0:P1] = Wp
     del dmoo
     grad = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())

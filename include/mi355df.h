@@ -217,6 +217,14 @@ int mdf_int3c2e(const mdf_basis *ao, const mdf_basis *aux, const int *pairs,
 int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_off,
                 double *out, long ldo, void *stream);
 
+/* The first step of mdf_get_k alone, FP64: R[L * strideL + i * ldr + k] =
+ * (A_L F)[i, k] for the nL rows of one block (A_L the unpacked symmetric
+ * row L of the cderi; pair_map and dtypes as for mdf_get_k).  The DF
+ * gradient's rhok[L] = F_r^T A_L F_l (electrolites/fastdfgrad.py:16-45,
+ * ``rhok`` from the cderi, :665-714) takes its inner product from R. */
+int mdf_k_step1(const void *cderi, int cderi_dtype, long nL, long npairs, long ld, const int *pair_map,
+                int nao, const double *f, long ldf, int nocc, double *R, long ldr, long strideL, void *stream);
+
 /* Nuclear derivative of the DF gradient's three-centre term:
  *   grad[3a + x]     += sum_{P,i,j} W[P, (i,j)] d(ij|P)/dR_{a,x}  over the
  *                       orbital centres of the listed shell pairs (an
