@@ -45,9 +45,10 @@ def parse():
     ap.add_argument('--synthetic-cderi', action='store_true')
     ap.add_argument('--cpu-sample-j', type=int, default=96)
     ap.add_argument('--cpu-sample-k', type=int, default=24)
-    ap.add_argument('--workload', choices=('getjk', 'batch', 'serve'), default='getjk',
+    ap.add_argument('--workload', choices=('getjk', 'batch', 'serve', 'grad'), default='getjk',
                     help='getjk: one (H2O)64 get_jk (default); batch: one grouped J/K launch '
-                         'over a molecule mix; serve: continuous-batching DF-SCF serving')
+                         'over a molecule mix; serve: continuous-batching DF-SCF serving; '
+                         'grad: the DF gradient two-electron term from the built cderi')
     ap.add_argument('--nmol', type=int, default=48, help='batch: molecules')
     ap.add_argument('--requests', type=int, default=24, help='serve: requests per rate')
     ap.add_argument('--rates', default='0.5,2,8', help='serve: Poisson rates (req/s)')
@@ -133,6 +134,49 @@ class _ResidentDF:
 
     def build(self):
         pass
+
+
+def bench_grad(args):
+    """The DF gradient's two-electron term (``fastdfgrad.jk_energy_per_atom``,
+    J - K/2 of the getjk workload's density) from the (H2O)64 cderi built on
+    the GPU: ``--warmup`` untimed calls, then ``--steps`` timed calls."""
+    import torch
+    from electrolites_amd import _dev, fastdf, fastdfgrad
+    from electrolites_amd.df import DF
+    from electrolites_amd.gto import water_cluster
+    _dev.require_gpu()
+    mol = water_cluster(args.nwater, seed=0)
+    nao, nocc = mol.nao, args.nocc
+    log(f'(H2O){args.nwater}: nao {nao} naux {mol.naux} natm {mol.natm} nocc {nocc}; building the cderi')
+    df = DF(mol).build()
+    rng = np.random.default_rng(0)
+    c = np.linalg.qr(rng.standard_normal((nao, nocc)))[0]
+    dm = torch.from_numpy(2 * c @ c.T).cuda()
+    fastdf.tag_array(dm, mo_coeff=torch.from_numpy(c).cuda(),
+                     mo_occ=torch.full((nocc,), 2., dtype=torch.float64, device='cuda'))
+    for _ in range(args.warmup):
+        fastdfgrad.jk_energy_per_atom(df, dm, 1., .5)
+    torch.cuda.synchronize()
+    stages = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = fastdfgrad.jk_energy_per_atom(df, dm, 1., .5, timings=stages)
+        log(f'gradient call done: sum over atoms {np.abs(g.sum(axis=0)).max():.1e}')
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {
+        'metric': 'DF gradient two-electron term calls/hr, (H2O)64 def2-SVP FP64',
+        'value': 3600. / (wall / args.steps), 'unit': 'gradient_2e/hr',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': wall / args.steps * 1e3, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic geometry: seeded (H2O)64 cluster, cderi built on the GPU; random rank-320 PSD density',
+        'config': {'workload': f'(H2O){args.nwater} DF gradient 2e term FP64', 'nao': nao, 'naux': mol.naux,
+                   'natm': mol.natm, 'nocc': nocc},
+        'stages_ms_per_call': {k: v / args.steps for k, v in stages.items()},
+        'translation_residual': float(np.abs(g.sum(axis=0)).max()),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(args):
@@ -292,6 +336,8 @@ def main():
         return bench_batch(args)
     if args.workload == 'serve':
         return bench_serve(args)
+    if args.workload == 'grad':
+        return bench_grad(args)
     import torch
     from electrolites_amd import _dev, _lib, fastdf
     from electrolites_amd.gto import water_cluster

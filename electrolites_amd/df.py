@@ -45,6 +45,7 @@ class DF:
         self._cderi = None
         self._cderi_idx = None
         self.__dict__.pop('_mdf_layout', None)
+        self.__dict__.pop('_mdf_grad_metric', None)
         return self
 
     def get_jk(self, dm, hermi=1, with_j=True, with_k=True, omega=None):

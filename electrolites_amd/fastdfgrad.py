@@ -51,6 +51,8 @@ RBLOCK_BYTES = 1 << 31
 WBLOCK_BYTES = 1 << 30
 #: orbital rows per lower-triangle block of the weight build
 IBLOCK = 128
+#: aux rows per block of the triangular metric products
+TBLOCK = 1024
 
 
 def stats():
@@ -85,6 +87,27 @@ def _class_sorted_pairs(mol, addr):
     off = np.zeros((ints.LMAX_AO + 1) ** 2 + 1, dtype=np.int64)
     off[1:] = np.cumsum(np.bincount(cls, minlength=(ints.LMAX_AO + 1) ** 2))
     return pairs, off, cmap
+
+
+def _metric_factor(dfobj, mol, aux):
+    """``(A, lower)``: the matrix the cderi is defined by (cderi = A T; the
+    inverse Cholesky factor of the metric, lower triangular, or the eigen
+    filter's coefficients), cached on the DF object for its current cderi
+    and geometry (the reference reuses the build's ``aux_coef`` the same
+    way, electrolites/fastdfgrad.py:665-714)."""
+    t = _dev.torch()
+    cderi = fastdf._cderi_of(dfobj)
+    key = (id(cderi), tuple(cderi.shape), np.asarray(mol.aux.shl_xyz).tobytes())
+    cached = getattr(dfobj, '_mdf_grad_metric', None)
+    if cached is not None and cached[0] == key:
+        return cached[1], cached[2]
+    linv, coef = fastdfcderi.decompose_j2c(aux)
+    A, lower = (t.tril(linv), True) if coef is None else (coef, False)
+    try:
+        dfobj._mdf_grad_metric = (key, A, lower)
+    except AttributeError:
+        pass
+    return A, lower
 
 
 def _factors(dm, hermi):
@@ -147,8 +170,7 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     aux = fastdfcderi._basis(mol, 'aux')
     ao = fastdfcderi._basis(mol, 'ao')
     tm.mark('metric')
-    linv, coef = fastdfcderi.decompose_j2c(aux)
-    A = t.tril(linv) if coef is None else coef            # cderi = A T
+    A, lower = _metric_factor(dfobj, mol, aux)            # cderi = A T
     nrow, naux = A.shape
     # rhok from the cderi's blocks; rho_B[L] = tr(A_L D) = tr(rhok[L])
     rhok = t.empty((nrow, nocc, nocc), dtype=t.float64, device=_dev.device())
@@ -182,13 +204,30 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     rho = t.diagonal(rhok, dim1=1, dim2=2).sum(dim=1)
     tm.mark('weights')
     c = A.T @ rho                                               # (naux,)
-    dmoo = (A.T @ rhok.reshape(nrow, nocc * nocc)).reshape(naux, nocc, nocc)
-    del rhok
+    rk = rhok.reshape(nrow, nocc * nocc)
+    if lower:
+        # A lower triangular: dmoo[P] = sum_{L >= P} A[L, P] rhok[L]
+        dmoo = t.empty((naux, nocc * nocc), dtype=t.float64, device=_dev.device())
+        for P0 in range(0, naux, TBLOCK):
+            P1 = min(naux, P0 + TBLOCK)
+            dmoo[P0:P1] = A[P0:, P0:P1].T @ rk[P0:]
+        dmoo = dmoo.reshape(naux, nocc, nocc)
+    else:
+        dmoo = (A.T @ rk).reshape(naux, nocc, nocc)
+    del rhok, rk
     Z = (-.5 * j_factor) * t.outer(c, c)
     if k_factor != 0:
+        # Z_K = 1/4 k tr(dmoo_P dmoo_Q), symmetric: the lower block rows, mirrored
         dm_flat = dmoo.reshape(naux, -1)
         dmT_flat = dmoo.transpose(1, 2).contiguous().reshape(naux, -1)
-        Z += (.25 * k_factor) * (dm_flat @ dmT_flat.T)
+        ZK = t.zeros((naux, naux), dtype=t.float64, device=_dev.device())
+        for P0 in range(0, naux, TBLOCK):
+            P1 = min(naux, P0 + TBLOCK)
+            ZK[P0:P1, :P1] = dm_flat[P0:P1] @ dmT_flat[:P1].T
+        del dmT_flat
+        ZK = t.tril(ZK)
+        Z += (.25 * k_factor) * (ZK + t.tril(ZK, -1).T)
+        del ZK
     # W in the cderi's compact columns
     addr = np.asarray(dfobj._cderi_idx[0])
     pairs, class_off, cmap = _class_sorted_pairs(mol, addr)
