@@ -44,7 +44,9 @@ from . import fastdf
 from . import fastdfcderi
 from . import ints
 
-STATS = {'calls': 0}
+STATS = {'calls': 0, 'declined': 0, 'declined_reasons': {}}
+#: the replaced GPU4PySCF methods, by class (kept by patch() for declines)
+_orig_methods = {}
 #: bytes of the R_L = A_L F_l block buffer
 RBLOCK_BYTES = 1 << 31
 #: bytes of one block of (nao, nocc) rows F_l dmoo_P of the weight build
@@ -282,3 +284,52 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     tm.close()
     STATS['calls'] += 1
     return grad.cpu().numpy()
+
+
+def _decline_reason(mf, dm, omega):
+    t = _dev.torch()
+    if omega:
+        return 'omega != 0'
+    with_df = getattr(mf, 'with_df', None)
+    store = getattr(with_df, '_cderi', None)
+    if with_df is None or store is None:
+        return 'no built cderi'
+    cderi = fastdf._cderi_of(with_df, missing_ok=True)
+    if not isinstance(cderi, t.Tensor):
+        return 'cderi not built by this package'
+    if np.ndim(dm) != 2 and not (hasattr(dm, 'dim') and dm.dim() == 2):
+        return 'not one restricted density'
+    d = dm.detach().cpu().numpy() if hasattr(dm, 'detach') else np.asarray(dm)
+    if np.iscomplexobj(d):
+        return 'complex density'
+    if not np.allclose(d, d.T, rtol=0, atol=1e-12 * max(np.abs(d).max(), 1.)):
+        return 'non-symmetric density'
+    return None
+
+
+def _orig_for(obj):
+    for cls in type(obj).__mro__:
+        if cls in _orig_methods:
+            return _orig_methods[cls]
+    return None
+
+
+def patched_jk_energy_per_atom(self, dm=None, j_factor=1, k_factor=1, omega=0, hermi=0, verbose=None):
+    """GPU4PySCF's ``Gradients.jk_energy_per_atom`` for a restricted density
+    over a cderi this package built (electrolites/fastdfgrad.py:1229-1325
+    replaces the same method).  Anything else -- a long-range call, a spin
+    density pair, a cderi GPU4PySCF built itself, a non-symmetric density --
+    is counted and handed to the method this replaced."""
+    mf = self.base
+    if dm is None:
+        dm = mf.make_rdm1()
+    reason = _decline_reason(mf, dm, omega)
+    if reason is None:
+        return jk_energy_per_atom(mf.with_df, dm, j_factor, k_factor, hermi=1,
+                                  auxbasis_response=getattr(self, 'auxbasis_response', True))
+    STATS['declined'] += 1
+    STATS['declined_reasons'][reason] = STATS['declined_reasons'].get(reason, 0) + 1
+    orig = _orig_for(self)
+    if orig is None:
+        raise fastdf.Declined(f'jk_energy_per_atom: {reason}')
+    return orig(self, dm, j_factor, k_factor, omega=omega, hermi=hermi, verbose=verbose)

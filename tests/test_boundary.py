@@ -33,6 +33,18 @@ def fake_gpu4pyscf(monkeypatch):
         calls.append(('loop', blksize))
         yield 'orig-block'
 
+    def orig_jk_energy_per_atom(self, dm=None, j_factor=1, k_factor=1, omega=0, hermi=0, verbose=None):
+        calls.append(('jk_energy_per_atom', omega))
+        return 'orig-ejk'
+
+    grad_pkg = types.ModuleType('gpu4pyscf.df.grad')
+    grad_mods = {}
+    for kind in ('rhf', 'rks'):
+        m = types.ModuleType(f'gpu4pyscf.df.grad.{kind}')
+        m.Gradients = type('Gradients', (), {'jk_energy_per_atom': orig_jk_energy_per_atom})
+        setattr(grad_pkg, kind, m)
+        grad_mods[kind] = m
+    df_pkg.grad = grad_pkg
     df_jk.get_jk = orig_get_jk
     df_jk.get_j = orig_get_j
     df_mod._cholesky_eri = orig_cholesky_eri
@@ -41,10 +53,13 @@ def fake_gpu4pyscf(monkeypatch):
     df_pkg.df_jk = df_jk
     df_pkg.df = df_mod
     for name, mod in [('gpu4pyscf', root), ('gpu4pyscf.df', df_pkg),
-                      ('gpu4pyscf.df.df_jk', df_jk), ('gpu4pyscf.df.df', df_mod)]:
+                      ('gpu4pyscf.df.df_jk', df_jk), ('gpu4pyscf.df.df', df_mod),
+                      ('gpu4pyscf.df.grad', grad_pkg), ('gpu4pyscf.df.grad.rhf', grad_mods['rhf']),
+                      ('gpu4pyscf.df.grad.rks', grad_mods['rks'])]:
         monkeypatch.setitem(sys.modules, name, mod)
     import electrolites_amd
-    from electrolites_amd import fastdf, fastdfcderi, fastdfj3c, fastdffp32
+    from electrolites_amd import fastdf, fastdfcderi, fastdfj3c, fastdffp32, fastdfgrad
+    monkeypatch.setattr(fastdfgrad, '_orig_methods', {})
     monkeypatch.setattr(fastdf, '_orig_get_jk', None)
     monkeypatch.setattr(fastdffp32, '_orig_loop', None)
     monkeypatch.setattr(fastdfcderi, '_orig_cholesky_eri', None, raising=False)
@@ -52,7 +67,8 @@ def fake_gpu4pyscf(monkeypatch):
     monkeypatch.setattr(electrolites_amd, '_patched', [])
     return types.SimpleNamespace(df_jk=df_jk, df=df_mod, calls=calls, orig_get_jk=orig_get_jk,
                                  orig_cholesky_eri=orig_cholesky_eri, orig_get_j=orig_get_j,
-                                 orig_loop=orig_loop)
+                                 orig_loop=orig_loop, grad=grad_mods,
+                                 orig_jk_energy_per_atom=orig_jk_energy_per_atom)
 
 
 def test_patch_installs_and_keeps_originals(fake_gpu4pyscf):
@@ -61,8 +77,8 @@ def test_patch_installs_and_keeps_originals(fake_gpu4pyscf):
     from electrolites_amd import fastdfj3c
     from electrolites_amd import fastdffp32
     assert electrolites_amd.check_targets() == {'fastdf': None, 'fastdfcderi': None, 'fastdfj3c': None,
-                                                'fastdffp32': None}
-    assert electrolites_amd.patch() == ('fastdf', 'fastdfcderi', 'fastdfj3c', 'fastdffp32')
+                                                'fastdffp32': None, 'fastdfgrad': None}
+    assert electrolites_amd.patch() == ('fastdf', 'fastdfcderi', 'fastdfj3c', 'fastdffp32', 'fastdfgrad')
     assert fake_gpu4pyscf.df.DF.loop is fastdffp32.patched_loop
     assert fastdffp32._orig_loop is fake_gpu4pyscf.orig_loop
     # a tensor GPU4PySCF built itself (not a torch tensor) goes to its own loop
@@ -102,7 +118,7 @@ def test_missing_target_is_reported(fake_gpu4pyscf, monkeypatch):
     assert rep['fastdf'] is None and 'missing' in rep['fastdfcderi']
     with pytest.raises(RuntimeError):
         electrolites_amd.patch()
-    assert electrolites_amd.patch(strict=False) == ('fastdf', 'fastdfj3c', 'fastdffp32')
+    assert electrolites_amd.patch(strict=False) == ('fastdf', 'fastdfj3c', 'fastdffp32', 'fastdfgrad')
 
 
 def test_standalone_decline_raises(monkeypatch):
@@ -172,3 +188,32 @@ def test_kept_pairs_cover_the_chunk_tasks():
                         covered[p] = True
     assert np.array_equal(covered, keep)
     assert 0 < keep.sum() < keep.size                 # something is screened at this size
+
+
+def test_gradient_method_patched_and_declines(fake_gpu4pyscf):
+    """Both restricted gradient classes get the method (each binds its own
+    copy, electrolites/__init__.py:133-143); a long-range call, a spin pair
+    and a cderi GPU4PySCF built itself go to the replaced method, counted."""
+    import electrolites_amd
+    from electrolites_amd import fastdfgrad
+    electrolites_amd.patch('fastdfgrad')
+    for kind in ('rhf', 'rks'):
+        cls = fake_gpu4pyscf.grad[kind].Gradients
+        assert cls.jk_energy_per_atom is fastdfgrad.patched_jk_energy_per_atom
+        assert fastdfgrad._orig_methods[cls] is fake_gpu4pyscf.orig_jk_energy_per_atom
+    electrolites_amd.patch('fastdfgrad')               # twice: originals kept
+    cls = fake_gpu4pyscf.grad['rhf'].Gradients
+    assert fastdfgrad._orig_methods[cls] is fake_gpu4pyscf.orig_jk_energy_per_atom
+
+    class WithDF:
+        _cderi = {0: np.zeros((3, 6))}                # a GPU4PySCF-built (non-torch) cderi
+
+    g = cls()
+    g.base = types.SimpleNamespace(with_df=WithDF(), make_rdm1=lambda: np.eye(3))
+    before = fastdfgrad.stats()['declined']
+    assert g.jk_energy_per_atom(np.eye(3)) == 'orig-ejk'
+    assert g.jk_energy_per_atom(np.eye(3), omega=.2) == 'orig-ejk'
+    assert fake_gpu4pyscf.calls[-1] == ('jk_energy_per_atom', .2)
+    st = fastdfgrad.stats()
+    assert st['declined'] == before + 2
+    assert st['declined_reasons'].get('omega != 0') and st['declined_reasons'].get('cderi not built by this package')

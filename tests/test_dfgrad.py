@@ -143,3 +143,21 @@ def test_gradient_storage_modes(mol, monkeypatch, storage, dtype, tol):
     assert np.abs(g - ref).max() < tol * max(np.abs(ref).max(), 1.)
     if storage == 'host':
         df.reset()
+
+
+@pytest.mark.gpu
+def test_patched_method_runs_the_hip_path(mol):
+    """The method patch() installs on GPU4PySCF's gradient classes runs this
+    package's gradient for a restricted density over its own cderi."""
+    import types
+    from electrolites_amd import fastdfgrad
+    from electrolites_amd.df import DF
+    D, _ = _density(mol.nao, 8, 7)
+    df = DF(mol).build()
+    g = types.SimpleNamespace(base=types.SimpleNamespace(with_df=df, make_rdm1=lambda: D),
+                              auxbasis_response=True)
+    before = fastdfgrad.stats()['calls']
+    out = fastdfgrad.patched_jk_energy_per_atom(g, None, 1, .5)
+    ref = fastdfgrad.jk_energy_per_atom(df, D, 1., .5)
+    assert fastdfgrad.stats()['calls'] == before + 2
+    assert np.abs(out - ref).max() < 1e-12
