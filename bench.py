@@ -9,8 +9,8 @@ int2c2e/int3c2e, Cholesky, L^-1, triangular multiply) and its build is timed
 per stage; ``--synthetic-cderi`` replaces it with seeded N(0,1) values over
 the full triangle (65.9 GB) for quick kernel timing.
 
-``df_gradient_2e`` times the DF gradient's two-electron term (J - K/2 of
-the same density) from the same cderi, per stage (metric factor, rhok,
+``df_gradient_2e`` times the DF gradient's two-electron term (j_factor 1,
+k_factor 0.5 on the same density) from the same cderi, per stage (metric factor, rhok,
 weights, derivative integrals).
 
 Prints ONE JSON line.  ``roofline`` is the K build (the dominant kernel
@@ -138,7 +138,7 @@ class _ResidentDF:
 
 def bench_grad(args):
     """The DF gradient's two-electron term (``fastdfgrad.jk_energy_per_atom``,
-    J - K/2 of the getjk workload's density) from the (H2O)64 cderi built on
+    j_factor 1, k_factor 0.5 on the getjk workload's density) from the (H2O)64 cderi built on
     the GPU: ``--warmup`` untimed calls, then ``--steps`` timed calls."""
     import torch
     from electrolites_amd import _dev, fastdf, fastdfgrad
@@ -551,7 +551,7 @@ def main():
             f'{t_dev * 1e3:.1f} ms; rel diff {host_line["rel_diff_vs_device"]:.1e}')
         del hj, hk, dj, dk, hdf, ddf, hpart, dev_part
 
-    # the DF gradient's two-electron term (J - K/2 of the same density) from
+    # the DF gradient's two-electron term (j_factor 1, k_factor 0.5) from
     # the same cderi: one warm call, then one timed call with its stages
     grad_line = None
     if not args.no_grad and not args.synthetic_cderi:
