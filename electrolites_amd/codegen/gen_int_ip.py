@@ -85,6 +85,14 @@ def _deriv(d, a, b, k, wrt):
 KCH = 16      # aux shells one thread of a three-centre kernel loops over
 QCH = 16      # aux shells one thread of a two-centre kernel loops over
 LDS_MAX_ATOMS = 1024
+#: a three-centre class whose Cartesian weight block times the aux components
+#: holds at most this many values computes every aux component m in one
+#: thread (the Rys roots and recurrences once instead of 2 lc + 1 times)
+ALLM_IP_MAX = 15
+
+
+def allm_ip(la, lb, lc):
+    return len(cart_powers(la)) * len(cart_powers(lb)) * (2 * lc + 1) <= ALLM_IP_MAX
 
 
 def _head(w, indent='    '):
@@ -122,12 +130,18 @@ def gen3(la, lb, lc):
       'double *grad, double *grad_aux) {')
     w('    const int nslot = 6;')
     _head(w)
+    am = allm_ip(la, lb, lc)
+    ms = list(range(2 * lc + 1)) if am else [None]
     w(f'    const int nkc = (nk + {KCH - 1}) / {KCH};')
     w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
-    w(f'    if (tid < (long)np * nkc * {2 * lc + 1}) {{')
-    w('    const int ip = (int)(tid % np);')
-    w('    const long rest = tid / np;')
-    w('    const int kc = (int)(rest % nkc), m = (int)(rest / nkc);')
+    if am:
+        w('    if (tid < (long)np * nkc) {')
+        w('    const int ip = (int)(tid % np), kc = (int)(tid / np);')
+    else:
+        w(f'    if (tid < (long)np * nkc * {2 * lc + 1}) {{')
+        w('    const int ip = (int)(tid % np);')
+        w('    const long rest = tid / np;')
+        w('    const int kc = (int)(rest % nkc), m = (int)(rest / nkc);')
     w('    const int ish = pairs[2 * ip], jsh = pairs[2 * ip + 1];')
     w('    const double Ax = ao.xyz[3 * ish], Ay = ao.xyz[3 * ish + 1], Az = ao.xyz[3 * ish + 2];')
     w('    const double Bx = ao.xyz[3 * jsh], By = ao.xyz[3 * jsh + 1], Bz = ao.xyz[3 * jsh + 2];')
@@ -150,18 +164,21 @@ def gen3(la, lb, lc):
     w(f'    for (int kq = kc * {KCH}; kq < k1; kq++) {{')
     w('    const int ksh = kl[kq];')
     w('    const double Cx = aux.xyz[3 * ksh], Cy = aux.xyz[3 * ksh + 1], Cz = aux.xyz[3 * ksh + 2];')
-    w('    const double *wrow = W + (long)(aux.aooff[ksh] + m) * ldw;')
-    for a in range(nca):
-        for b in range(ncb):
-            terms = []
-            for s in range(nsa):
-                if abs(Ca[s, a]) < 1e-15:
-                    continue
-                for t in range(nsb):
-                    if abs(Cb[t, b]) < 1e-15:
+    for mi in ms:
+        sfx = '' if mi is None else f'{mi}_'
+        mexpr = 'm' if mi is None else str(mi)
+        w(f'    const double *wrow{sfx.rstrip("_")} = W + (long)(aux.aooff[ksh] + {mexpr}) * ldw;')
+        for a in range(nca):
+            for b in range(ncb):
+                terms = []
+                for s_ in range(nsa):
+                    if abs(Ca[s_, a]) < 1e-15:
                         continue
-                    terms.append(f'{fmt(Ca[s, a] * Cb[t, b])} * wrow[col_{s}_{t}]')
-            w(f'    const double wc_{a}_{b} = fac * (' + (' + '.join(terms) if terms else '0.0') + ');')
+                    for t in range(nsb):
+                        if abs(Cb[t, b]) < 1e-15:
+                            continue
+                        terms.append(f'{fmt(Ca[s_, a] * Cb[t, b])} * wrow{sfx.rstrip("_")}[col_{s_}_{t}]')
+                w(f'    const double wc_{sfx}{a}_{b} = fac * (' + (' + '.join(terms) if terms else '0.0') + ');')
     w('    double hAx = 0., hAy = 0., hAz = 0., hBx = 0., hBy = 0., hBz = 0.;')
     w('    const int npc = aux.nprim[ksh], oc = aux.poff[ksh];')
     w('    for (int ia = 0; ia < npa; ia++) {')
@@ -195,26 +212,32 @@ def gen3(la, lb, lc):
         w(f'                    const double C00{d} = PA{d} - qf * PC{d}, C0p{d} = pf * PC{d};')
     ind = '                    '
     _vrr_hrr(w, lab + 1, lc, lb + 1, ind)
-    w(ind + 'switch (m) {')
-    for mi in range(2 * lc + 1):
-        w(ind + f'case {mi}: {{')
+    def products(mi, wname, indent):
         sup = [(k, Cc[mi, k]) for k in range(len(cc)) if abs(Cc[mi, k]) > 1e-15]
         for a_i, (ax, ay, az) in enumerate(ca):
             for b_i, (bx, by, bz) in enumerate(cb):
                 for k, coef in sup:
                     kx, ky, kz = cc[k]
-                    tw = f'wc_{a_i}_{b_i} * {fmt(coef)}'
+                    tw = f'{wname}{a_i}_{b_i} * {fmt(coef)}'
                     Ix, Iy, Iz = f'Ix_{ax}_{bx}_{kx}', f'Iy_{ay}_{by}_{ky}', f'Iz_{az}_{bz}_{kz}'
-                    w(ind + f'    {{ const double tw = {tw};')
-                    w(ind + f'      hAx += tw * {_deriv("x", ax, bx, kx, "A")} * {Iy} * {Iz};')
-                    w(ind + f'      hAy += tw * {Ix} * {_deriv("y", ay, by, ky, "A")} * {Iz};')
-                    w(ind + f'      hAz += tw * {Ix} * {Iy} * {_deriv("z", az, bz, kz, "A")};')
-                    w(ind + f'      hBx += tw * {_deriv("x", ax, bx, kx, "B")} * {Iy} * {Iz};')
-                    w(ind + f'      hBy += tw * {Ix} * {_deriv("y", ay, by, ky, "B")} * {Iz};')
-                    w(ind + f'      hBz += tw * {Ix} * {Iy} * {_deriv("z", az, bz, kz, "B")}; }}')
-        w(ind + '    break;')
+                    w(indent + f'{{ const double tw = {tw};')
+                    w(indent + f'  hAx += tw * {_deriv("x", ax, bx, kx, "A")} * {Iy} * {Iz};')
+                    w(indent + f'  hAy += tw * {Ix} * {_deriv("y", ay, by, ky, "A")} * {Iz};')
+                    w(indent + f'  hAz += tw * {Ix} * {Iy} * {_deriv("z", az, bz, kz, "A")};')
+                    w(indent + f'  hBx += tw * {_deriv("x", ax, bx, kx, "B")} * {Iy} * {Iz};')
+                    w(indent + f'  hBy += tw * {Ix} * {_deriv("y", ay, by, ky, "B")} * {Iz};')
+                    w(indent + f'  hBz += tw * {Ix} * {Iy} * {_deriv("z", az, bz, kz, "B")}; }}')
+    if am:
+        for mi in range(2 * lc + 1):
+            products(mi, f'wc_{mi}_', ind)
+    else:
+        w(ind + 'switch (m) {')
+        for mi in range(2 * lc + 1):
+            w(ind + f'case {mi}: {{')
+            products(mi, 'wc_', ind + '    ')
+            w(ind + '    break;')
+            w(ind + '}')
         w(ind + '}')
-    w(ind + '}')
     w('                }')
     w('            }')
     w('        }')
@@ -360,16 +383,19 @@ def main():
     parts.append('int launch_int3c_ip(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, '
                  'const int *pairs, int np, const int *kl, int nk, const double *W, long ldw, const int *cmap, '
                  'const int *ao_atom, const int *aux_atom, int natm, double *grad, double *grad_aux, hipStream_t st) {')
-    parts.append(f'    const long n = (long)np * ((nk + {KCH - 1}) / {KCH}) * (2 * lc + 1);')
+    parts.append(f'    const long nkc = (nk + {KCH - 1}) / {KCH};')
     parts.append(f'    const int use_lds = natm <= {LDS_MAX_ATOMS};')
     parts.append('    const size_t shm = use_lds ? 6 * (size_t)natm * sizeof(double) : 0;')
-    parts.append('    if (n == 0) return MDF_OK;')
-    parts.append(f'    const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})), b({THREADS});')
+    parts.append('    if ((long)np * nkc == 0) return MDF_OK;')
+    parts.append(f'    const dim3 b({THREADS});')
     parts.append(f'    switch ((la * {LMAX_AO + 1} + lb) * {LMAX_AUX + 1} + lc) {{')
     for (la, lb, lc), n in n3.items():
         key = (la * (LMAX_AO + 1) + lb) * (LMAX_AUX + 1) + lc
-        parts.append(f'    case {key}: hipLaunchKernelGGL({n}, g, b, shm, st, ao, aux, pairs, np, kl, nk, W, ldw, '
-                     'cmap, ao_atom, aux_atom, natm, use_lds, grad, grad_aux); break;')
+        mult = 1 if allm_ip(la, lb, lc) else 2 * lc + 1
+        parts.append(f'    case {key}: {{ const long n = (long)np * nkc * {mult}; '
+                     f'const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})); '
+                     f'hipLaunchKernelGGL({n}, g, b, shm, st, ao, aux, pairs, np, kl, nk, W, ldw, '
+                     'cmap, ao_atom, aux_atom, natm, use_lds, grad, grad_aux); } break;')
     parts.append('    default: set_error("int3c2e_ip: class (%d %d|%d) not generated", la, lb, lc); return MDF_EINVAL;')
     parts.append('    }')
     parts.append('    MDF_LAUNCHED("int3c_ip");')
