@@ -18,6 +18,10 @@ import pytest
 from oracle import oracle as O
 
 
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+
+
 def _copy(d):
     return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in d.items()}
 
@@ -161,3 +165,37 @@ def test_patched_method_runs_the_hip_path(mol):
     ref = fastdfgrad.jk_energy_per_atom(df, D, 1., .5)
     assert fastdfgrad.stats()['calls'] == before + 2
     assert np.abs(out - ref).max() < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('nao,naux,nocc,nL,screened,dtype', [(37, 19, 5, 19, False, 'float64'),
+                                                              (64, 130, 17, 41, True, 'float64'),
+                                                              (200, 77, 90, 77, True, 'float32')])
+def test_k_step1_entry(nao, naux, nocc, nL, screened, dtype):
+    """mdf_k_step1 (K's first step alone, the gradient's R_L = A_L F) equals
+    the unpacked rows times F (numpy) on ragged shapes, full-triangle and
+    screened pair lists, FP64 and FP32 storage (widened exactly)."""
+    import torch
+    from electrolites_amd import _dev, _lib, fastdf
+    rng = np.random.default_rng(nao + naux)
+    pa, _ = O.pair_addresses(nao)
+    if screened:
+        pa = np.sort(rng.choice(pa, size=len(pa) * 2 // 3, replace=False))
+    cderi = rng.standard_normal((naux, len(pa))).astype(dtype)
+    f = rng.standard_normal((nao, nocc))
+    ref = np.matmul(O.unpack(cderi.astype(np.float64), pa, nao)[:nL], f)
+
+    class DFo:
+        _cderi = {0: torch.from_numpy(cderi).cuda()}
+        _cderi_idx = (pa.astype(np.int32), None)
+
+    c = DFo._cderi[0]
+    lay = fastdf._layout(DFo(), c, nao)
+    fd = torch.from_numpy(f).cuda()
+    R = torch.full((nL, nao, nocc + 3), np.nan, dtype=torch.float64, device='cuda')
+    _lib.call('mdf_k_step1', _dev.ptr(c), fastdf._dtype_code(c), nL, c.shape[1], c.stride(0),
+              _dev.ptr(lay.map), nao, _dev.ptr(fd), fd.stride(0), nocc, _dev.ptr(R), nocc + 3,
+              nao * (nocc + 3), _dev.stream_ptr())
+    got = R.cpu().numpy()
+    assert np.all(np.isnan(got[..., nocc:]))              # nothing written past nocc
+    assert rel(got[..., :nocc], ref) < 1e-13
