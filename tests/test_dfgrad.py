@@ -199,3 +199,28 @@ def test_k_step1_entry(nao, naux, nocc, nL, screened, dtype):
     got = R.cpu().numpy()
     assert np.all(np.isnan(got[..., nocc:]))              # nothing written past nocc
     assert rel(got[..., :nocc], ref) < 1e-13
+
+
+@pytest.mark.gpu
+def test_gradient_eigen_route(mol, monkeypatch):
+    """The metric's eigen route (cderi = coef T with coef^T coef = M^-1, the
+    reference's branch for a metric that is not positive definite,
+    electrolites/fastdfcderi.py:203-210) gives the same gradient as the
+    Cholesky route: the build and the gradient take the same coefficients."""
+    import torch
+    import electrolites_amd.fastdfcderi as FC
+    from electrolites_amd import fastdfgrad, ints
+    from electrolites_amd.df import DF
+    D, _ = _density(mol.nao, 8, 8)
+    ref = fastdfgrad.jk_energy_per_atom(DF(mol).build(), D, 1., .5)
+
+    def eigen_route(aux, timer=None):
+        w, v = torch.linalg.eigh(ints.int2c2e(aux))
+        keep = w > FC.LINEAR_DEP_THR
+        return None, (v[:, keep] / torch.sqrt(w[keep])).T.contiguous()
+
+    monkeypatch.setattr(FC, 'decompose_j2c', eigen_route)
+    df = DF(mol).build()
+    g = fastdfgrad.jk_energy_per_atom(df, D, 1., .5)
+    assert df._mdf_grad_metric[2] is False                   # the general (non-triangular) branch ran
+    assert np.abs(g - ref).max() < 1e-7
