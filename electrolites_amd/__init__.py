@@ -8,16 +8,15 @@ over the two entry points the reference replaces (electrolites/__init__.py
     gpu4pyscf.df.df._cholesky_eri    -> electrolites_amd.fastdfcderi._cholesky_eri
     gpu4pyscf.df.df_jk.get_j         -> electrolites_amd.fastdfj3c.get_j
     gpu4pyscf.df.df.DF.loop          -> electrolites_amd.fastdffp32.patched_loop
-    gpu4pyscf.df.grad.{rhf,rks}.Gradients.jk_energy_per_atom
+    gpu4pyscf.df.grad.{rhf,rks,uhf,uks}.Gradients.jk_energy_per_atom
                                      -> electrolites_amd.fastdfgrad.patched_jk_energy_per_atom
 
 ('fastdffp32', as the reference's entry of that name (:164), gives the
 block iterator the tensors this build writes: FP32 storage widened, host
 residency streamed.)
 ('fastdfgrad', as the reference's entry of that name (:137-143), replaces the
-DF gradient's two-electron method on the restricted gradient classes, each
-of which binds its own copy of the function; the unrestricted ones keep
-GPU4PySCF's.)
+DF gradient's two-electron method on the four gradient classes, each of
+which binds its own copy of the function.)
 (the reference's 'fastdfj3c' entry patches the two Int3c2eOpt contractions
 under ``df_jk.get_j``, :127-132; this build replaces the whole J-without-a-
 cderi function, whose contractions are its own kernels)
@@ -44,8 +43,8 @@ _TARGETS = {
     'fastdfcderi': (('gpu4pyscf.df.df', '_cholesky_eri'),),
     'fastdfj3c': (('gpu4pyscf.df.df_jk', 'get_j'),),
     'fastdffp32': (('gpu4pyscf.df.df', 'DF.loop'),),
-    'fastdfgrad': (('gpu4pyscf.df.grad.rhf', 'Gradients.jk_energy_per_atom'),
-                   ('gpu4pyscf.df.grad.rks', 'Gradients.jk_energy_per_atom')),
+    'fastdfgrad': tuple((f'gpu4pyscf.df.grad.{k}', 'Gradients.jk_energy_per_atom')
+                        for k in ('rhf', 'rks', 'uhf', 'uks')),
 }
 MODULES = ('fastdf', 'fastdfcderi', 'fastdfj3c', 'fastdffp32', 'fastdfgrad')
 _patched = []
@@ -96,7 +95,7 @@ def _install(name):
         mod.DF.loop = fastdffp32.patched_loop
     elif name == 'fastdfgrad':
         from . import fastdfgrad
-        for mod_path in ('gpu4pyscf.df.grad.rhf', 'gpu4pyscf.df.grad.rks'):
+        for mod_path in [m for m, _ in _TARGETS['fastdfgrad']]:
             cls = importlib.import_module(mod_path).Gradients
             # each class binds the function object itself, so each is patched
             if cls not in fastdfgrad._orig_methods:

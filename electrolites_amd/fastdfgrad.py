@@ -8,6 +8,9 @@ the restricted two-electron energy
 
     E = 1/2 j_factor tr(D J[D]) - 1/4 k_factor tr(D K[D])
 
+(for an unrestricted density, D = D_a + D_b in J and -1/2 k_factor
+sum_s tr(D_s K[D_s]) in K: the sums below run over the two spin factors with
+twice the weight)
 with J and K density fitted over the metric M = (P|Q).  With the cderi
 ``B_L = sum_P A[L, P] T_P`` (T_P the three-centre integrals (ij|P); A the
 inverse Cholesky factor of M or the eigen filter's coefficients, so that
@@ -113,17 +116,23 @@ def _metric_factor(dfobj, mol, aux):
 
 
 def _factors(dm, hermi):
+    """``[(fl, fr)]`` per density: one for a restricted density ``(nao,
+    nao)``, two for the spin densities ``(2, nao, nao)`` of an unrestricted
+    one."""
     if hermi != 1:
         raise ValueError('jk_energy_per_atom: a symmetric density (hermi=1) is required')
-    dms = dm
-    if np.ndim(dm) == 2 or (hasattr(dm, 'dim') and dm.dim() == 2):
-        dms = dm[None] if not hasattr(dm, 'mo_coeff') else dm
+    nd = dm.dim() if hasattr(dm, 'dim') else np.ndim(dm)
+    if nd == 2:
+        dms = dm if hasattr(dm, 'mo_coeff') else dm[None]
+    elif nd == 3 and dm.shape[0] == 2:
+        dms = dm
+    else:
+        raise ValueError('jk_energy_per_atom: expected one density (nao, nao) or a spin pair (2, nao, nao)')
     fl, fr = fastdf.factorize_dm(dms, 1)
-    if fl.shape[0] != 1:
-        raise ValueError('jk_energy_per_atom: one restricted density expected')
-    fl = fl[0]
-    fr = fl if fr is None else fr[0]
-    return fl.contiguous(), fr.contiguous()
+    n = 1 if nd == 2 else 2
+    if fl.shape[0] != n:
+        raise ValueError('jk_energy_per_atom: density factors do not match the density shape')
+    return [(fl[s].contiguous(), (fl[s] if fr is None else fr[s]).contiguous()) for s in range(n)]
 
 
 class _Stages:
@@ -149,35 +158,12 @@ class _Stages:
             self.out[n] = self.out.get(n, 0.) + a.elapsed_time(b)
 
 
-def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_response=True, timings=None):
-    """``(natm, 3)`` nuclear derivative of ``1/2 j tr(D J) - 1/4 k tr(D K)``
-    for the restricted density ``dm`` (nao x nao; tagged with
-    ``mo_coeff``/``mo_occ`` or not), from the built cderi of ``dfobj``
-    (electrolites/fastdfgrad.py:1229).  ``auxbasis_response`` adds the
-    fitting functions' centres (the aux-centre share of the three-centre
-    term and the metric term), as GPU4PySCF's flag of that name.
-    ``timings``: a dict that receives the device milliseconds of each stage
-    ('metric', 'rhok', 'weights', 'ip3c', 'ip2c')."""
+def _rhok(dfobj, fl, fr, nao, nrow):
+    """``rhok[L] = F_r^T A_L F_l`` for every cderi row L."""
     t = _dev.torch()
-    tm = _Stages(timings)
-    _dev.require_gpu()
-    mol = dfobj.mol
-    if getattr(dfobj, '_cderi', None) is None:
-        dfobj.build()
-    nao, natm = mol.nao, mol.natm
-    fl, fr = _factors(dm, hermi)
     nocc = fl.shape[1]
-    D = fl @ fr.T
-    D = .5 * (D + D.T)
-    aux = fastdfcderi._basis(mol, 'aux')
-    ao = fastdfcderi._basis(mol, 'ao')
-    tm.mark('metric')
-    A, lower = _metric_factor(dfobj, mol, aux)            # cderi = A T
-    nrow, naux = A.shape
-    # rhok from the cderi's blocks; rho_B[L] = tr(A_L D) = tr(rhok[L])
     rhok = t.empty((nrow, nocc, nocc), dtype=t.float64, device=_dev.device())
     frT = fr.T.contiguous()
-    tm.mark('rhok')
     cderi = fastdf._cderi_of(dfobj)
     if isinstance(cderi, t.Tensor) and cderi.is_cuda and cderi.dtype in (t.float64, t.float32) \
             and cderi.dim() == 2 and cderi.stride(1) == 1:
@@ -194,53 +180,51 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
                       _dev.ptr(lay.map), nao, _dev.ptr(fl), fl.stride(0), nocc, _dev.ptr(R), nocc, nao * nocc,
                       _dev.stream_ptr())
             rhok[L0:L0 + nL] = t.matmul(frT, R[:nL])
-        del R
-    else:
-        L0 = 0
-        for dense, _ in dfobj.loop():
-            nL = dense.shape[0]
-            rhok[L0:L0 + nL] = t.matmul(frT, t.matmul(dense, fl))
-            L0 += nL
-        if L0 != nrow:
-            raise ValueError(f'jk_energy_per_atom: the cderi has {L0} rows, the metric factor {nrow}')
-    rho = t.diagonal(rhok, dim1=1, dim2=2).sum(dim=1)
-    tm.mark('weights')
-    c = A.T @ rho                                               # (naux,)
+        return rhok
+    L0 = 0
+    for dense, _ in dfobj.loop():
+        nL = dense.shape[0]
+        rhok[L0:L0 + nL] = t.matmul(frT, t.matmul(dense, fl))
+        L0 += nL
+    if L0 != nrow:
+        raise ValueError(f'jk_energy_per_atom: the cderi has {L0} rows, the metric factor {nrow}')
+    return rhok
+
+
+def _dmoo(A, lower, rhok, naux):
+    """``dmoo[P] = sum_L A[L, P] rhok[L]``, by block rows when A is lower
+    triangular (only L >= P contributes)."""
+    t = _dev.torch()
+    nrow, nocc = rhok.shape[0], rhok.shape[1]
     rk = rhok.reshape(nrow, nocc * nocc)
-    if lower:
-        # A lower triangular: dmoo[P] = sum_{L >= P} A[L, P] rhok[L]
-        dmoo = t.empty((naux, nocc * nocc), dtype=t.float64, device=_dev.device())
-        for P0 in range(0, naux, TBLOCK):
-            P1 = min(naux, P0 + TBLOCK)
-            dmoo[P0:P1] = A[P0:, P0:P1].T @ rk[P0:]
-        dmoo = dmoo.reshape(naux, nocc, nocc)
-    else:
-        dmoo = (A.T @ rk).reshape(naux, nocc, nocc)
-    del rhok, rk
-    Z = (-.5 * j_factor) * t.outer(c, c)
-    if k_factor != 0:
-        # Z_K = 1/4 k tr(dmoo_P dmoo_Q), symmetric: the lower block rows, mirrored
-        dm_flat = dmoo.reshape(naux, -1)
-        dmT_flat = dmoo.transpose(1, 2).contiguous().reshape(naux, -1)
-        ZK = t.zeros((naux, naux), dtype=t.float64, device=_dev.device())
-        for P0 in range(0, naux, TBLOCK):
-            P1 = min(naux, P0 + TBLOCK)
-            ZK[P0:P1, :P1] = dm_flat[P0:P1] @ dmT_flat[:P1].T
-        del dmT_flat
-        ZK = t.tril(ZK)
-        Z += (.25 * k_factor) * (ZK + t.tril(ZK, -1).T)
-        del ZK
-    # W in the cderi's compact columns
-    addr = np.asarray(dfobj._cderi_idx[0])
-    pairs, class_off, cmap = _class_sorted_pairs(mol, addr)
-    ncol = len(addr)
-    W = t.empty((naux, ncol), dtype=t.float64, device=_dev.device())
-    addr_d = _dev.to_device(addr.astype(np.int64), dtype=t.int64)
-    Dflat = D.reshape(-1)[addr_d]
-    # the exchange part only on the lower triangle, in row blocks of IBLOCK:
-    # block b holds the stored pairs (i, j), i in [i0, i1), j <= i, at local
-    # index (i - i0) * i1 + j of the (i1 - i0, i1) product
-    pi, pj = np.divmod(addr.astype(np.int64), nao)
+    if not lower:
+        return (A.T @ rk).reshape(naux, nocc, nocc)
+    dmoo = t.empty((naux, nocc * nocc), dtype=t.float64, device=_dev.device())
+    for P0 in range(0, naux, TBLOCK):
+        P1 = min(naux, P0 + TBLOCK)
+        dmoo[P0:P1] = A[P0:, P0:P1].T @ rk[P0:]
+    return dmoo.reshape(naux, nocc, nocc)
+
+
+def _trace_products(dmoo, naux):
+    """``tr(dmoo_P dmoo_Q)`` for all P, Q: the lower block rows, mirrored."""
+    t = _dev.torch()
+    dm_flat = dmoo.reshape(naux, -1)
+    dmT_flat = dmoo.transpose(1, 2).contiguous().reshape(naux, -1)
+    out = t.zeros((naux, naux), dtype=t.float64, device=_dev.device())
+    for P0 in range(0, naux, TBLOCK):
+        P1 = min(naux, P0 + TBLOCK)
+        out[P0:P1, :P1] = dm_flat[P0:P1] @ dmT_flat[:P1].T
+    out = t.tril(out)
+    return out + t.tril(out, -1).T
+
+
+def _lower_blocks(addr, nao):
+    """Row blocks of the stored pairs for the lower-triangle products: block b
+    holds the pairs (i, j), i in [i0, i1), j <= i, at local index
+    (i - i0) * i1 + j of an (i1 - i0, i1) product."""
+    t = _dev.torch()
+    pi, pj = np.divmod(np.asarray(addr, dtype=np.int64), nao)
     hi, lo = np.maximum(pi, pj), np.minimum(pi, pj)
     tri = []
     for i0 in range(0, nao, IBLOCK):
@@ -249,17 +233,75 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
         if len(sel):
             tri.append((i0, i1, _dev.to_device(sel, dtype=t.int64),
                         _dev.to_device((hi[sel] - i0) * i1 + lo[sel], dtype=t.int64)))
-    blk = max(1, min(naux, WBLOCK_BYTES // max(8 * nao * max(nocc, IBLOCK), 1)))
-    for P0 in range(0, naux, blk):
-        P1 = min(naux, P0 + blk)
-        Wp = (j_factor * c[P0:P1, None]) * Dflat[None, :]
-        if k_factor != 0:
-            Y = t.matmul(fl, dmoo[P0:P1])                       # (nP, nao, nocc)
-            for i0, i1, sel, loc in tri:
-                blkK = t.matmul(Y[:, i0:i1], frT[:, :i1])       # (nP, i1 - i0, i1)
-                Wp[:, sel] -= (.5 * k_factor) * blkK.reshape(P1 - P0, -1)[:, loc]
-        W[P0:P1] = Wp
-    del dmoo
+    return tri
+
+
+def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_response=True, timings=None):
+    """``(natm, 3)`` nuclear derivative of ``1/2 j tr(D J) - 1/4 k tr(D K)``
+    for the restricted density ``dm`` (nao x nao), or of ``1/2 j tr(D J) -
+    1/2 k sum_s tr(D_s K[D_s])`` (D = D_a + D_b) for the spin densities ``dm``
+    (2, nao, nao) of an unrestricted one (tagged with ``mo_coeff``/``mo_occ``
+    or not), from the built cderi of ``dfobj``
+    (electrolites/fastdfgrad.py:1229).  ``auxbasis_response`` adds the
+    fitting functions' centres (the aux-centre share of the three-centre
+    term and the metric term), as GPU4PySCF's flag of that name.
+    ``timings``: a dict that receives the device milliseconds of each stage
+    ('metric', 'rhok', 'weights', 'ip3c', 'ip2c')."""
+    t = _dev.torch()
+    tm = _Stages(timings)
+    _dev.require_gpu()
+    mol = dfobj.mol
+    if getattr(dfobj, '_cderi', None) is None:
+        dfobj.build()
+    nao, natm = mol.nao, mol.natm
+    facs = _factors(dm, hermi)
+    # K weight per factor set: E_K = -1/4 k tr(D K[D]) restricted,
+    # -1/2 k sum_s tr(D_s K[D_s]) over the spin densities
+    wk = .25 if len(facs) == 1 else .5
+    D = sum(fl @ fr.T for fl, fr in facs)
+    D = .5 * (D + D.T)
+    aux = fastdfcderi._basis(mol, 'aux')
+    ao = fastdfcderi._basis(mol, 'ao')
+    tm.mark('metric')
+    A, lower = _metric_factor(dfobj, mol, aux)            # cderi = A T
+    nrow, naux = A.shape
+    tm.mark('rhok')
+    # rhok_s from the cderi's blocks; rho_B[L] = tr(A_L D) = sum_s tr(rhok_s[L])
+    rhoks = [_rhok(dfobj, fl, fr, nao, nrow) for fl, fr in facs]
+    rho = sum(t.diagonal(r, dim1=1, dim2=2).sum(dim=1) for r in rhoks)
+    tm.mark('weights')
+    c = A.T @ rho                                               # (naux,)
+    Z = (-.5 * j_factor) * t.outer(c, c)
+    # W in the cderi's compact columns
+    addr = np.asarray(dfobj._cderi_idx[0])
+    pairs, class_off, cmap = _class_sorted_pairs(mol, addr)
+    ncol = len(addr)
+    W = t.empty((naux, ncol), dtype=t.float64, device=_dev.device())
+    addr_d = _dev.to_device(addr.astype(np.int64), dtype=t.int64)
+    Dflat = D.reshape(-1)[addr_d]
+    jb = max(1, min(naux, WBLOCK_BYTES // max(8 * ncol, 1)))
+    for P0 in range(0, naux, jb):
+        P1 = min(naux, P0 + jb)
+        W[P0:P1] = (j_factor * c[P0:P1, None]) * Dflat[None, :]
+    if k_factor != 0:
+        tri = _lower_blocks(addr, nao)
+        for si, (fl, fr) in enumerate(facs):
+            nocc = fl.shape[1]
+            dmoo = _dmoo(A, lower, rhoks[si], naux)
+            rhoks[si] = None
+            Z += (wk * k_factor) * _trace_products(dmoo, naux)
+            # the exchange part only on the lower triangle of (i, j)
+            frT = fr.T.contiguous()
+            blk = max(1, min(naux, WBLOCK_BYTES // max(8 * nao * max(nocc, IBLOCK), 1)))
+            for P0 in range(0, naux, blk):
+                P1 = min(naux, P0 + blk)
+                Y = t.matmul(fl, dmoo[P0:P1])                   # (nP, nao, nocc)
+                Wp = W[P0:P1]
+                for i0, i1, sel, loc in tri:
+                    blkK = t.matmul(Y[:, i0:i1], frT[:, :i1])   # (nP, i1 - i0, i1)
+                    Wp[:, sel] -= (2 * wk * k_factor) * blkK.reshape(P1 - P0, -1)[:, loc]
+            del dmoo
+    del rhoks
     grad = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())
     grad_aux = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())
     i32 = lambda a: _dev.to_device(np.ascontiguousarray(a, dtype=np.int32), dtype=t.int32)
@@ -297,12 +339,14 @@ def _decline_reason(mf, dm, omega):
     cderi = fastdf._cderi_of(with_df, missing_ok=True)
     if not isinstance(cderi, t.Tensor):
         return 'cderi not built by this package'
-    if np.ndim(dm) != 2 and not (hasattr(dm, 'dim') and dm.dim() == 2):
-        return 'not one restricted density'
+    nd = dm.dim() if hasattr(dm, 'dim') else np.ndim(dm)
+    if not (nd == 2 or (nd == 3 and dm.shape[0] == 2)):
+        return 'not one density or a spin pair'
     d = dm.detach().cpu().numpy() if hasattr(dm, 'detach') else np.asarray(dm)
     if np.iscomplexobj(d):
         return 'complex density'
-    if not np.allclose(d, d.T, rtol=0, atol=1e-12 * max(np.abs(d).max(), 1.)):
+    d = d.reshape(-1, d.shape[-2], d.shape[-1])
+    if not all(np.allclose(x, x.T, rtol=0, atol=1e-12 * max(np.abs(x).max(), 1.)) for x in d):
         return 'non-symmetric density'
     return None
 
@@ -316,10 +360,11 @@ def _orig_for(obj):
 
 def patched_jk_energy_per_atom(self, dm=None, j_factor=1, k_factor=1, omega=0, hermi=0, verbose=None):
     """GPU4PySCF's ``Gradients.jk_energy_per_atom`` for a restricted density
-    over a cderi this package built (electrolites/fastdfgrad.py:1229-1325
-    replaces the same method).  Anything else -- a long-range call, a spin
-    density pair, a cderi GPU4PySCF built itself, a non-symmetric density --
-    is counted and handed to the method this replaced."""
+    or an unrestricted spin pair over a cderi this package built
+    (electrolites/fastdfgrad.py:1229-1325 replaces the same method on the
+    same four classes).  Anything else -- a long-range call, a cderi
+    GPU4PySCF built itself, a non-symmetric density -- is counted and handed
+    to the method this replaced."""
     mf = self.base
     if dm is None:
         dm = mf.make_rdm1()

@@ -39,7 +39,7 @@ def fake_gpu4pyscf(monkeypatch):
 
     grad_pkg = types.ModuleType('gpu4pyscf.df.grad')
     grad_mods = {}
-    for kind in ('rhf', 'rks'):
+    for kind in ('rhf', 'rks', 'uhf', 'uks'):
         m = types.ModuleType(f'gpu4pyscf.df.grad.{kind}')
         m.Gradients = type('Gradients', (), {'jk_energy_per_atom': orig_jk_energy_per_atom})
         setattr(grad_pkg, kind, m)
@@ -54,8 +54,8 @@ def fake_gpu4pyscf(monkeypatch):
     df_pkg.df = df_mod
     for name, mod in [('gpu4pyscf', root), ('gpu4pyscf.df', df_pkg),
                       ('gpu4pyscf.df.df_jk', df_jk), ('gpu4pyscf.df.df', df_mod),
-                      ('gpu4pyscf.df.grad', grad_pkg), ('gpu4pyscf.df.grad.rhf', grad_mods['rhf']),
-                      ('gpu4pyscf.df.grad.rks', grad_mods['rks'])]:
+                      ('gpu4pyscf.df.grad', grad_pkg)] + \
+            [(f'gpu4pyscf.df.grad.{k}', m) for k, m in grad_mods.items()]:
         monkeypatch.setitem(sys.modules, name, mod)
     import electrolites_amd
     from electrolites_amd import fastdf, fastdfcderi, fastdfj3c, fastdffp32, fastdfgrad
@@ -191,13 +191,13 @@ def test_kept_pairs_cover_the_chunk_tasks():
 
 
 def test_gradient_method_patched_and_declines(fake_gpu4pyscf):
-    """Both restricted gradient classes get the method (each binds its own
-    copy, electrolites/__init__.py:133-143); a long-range call, a spin pair
-    and a cderi GPU4PySCF built itself go to the replaced method, counted."""
+    """All four gradient classes get the method (each binds its own copy,
+    electrolites/__init__.py:133-143); a long-range call and a cderi
+    GPU4PySCF built itself go to the replaced method, counted."""
     import electrolites_amd
     from electrolites_amd import fastdfgrad
     electrolites_amd.patch('fastdfgrad')
-    for kind in ('rhf', 'rks'):
+    for kind in ('rhf', 'rks', 'uhf', 'uks'):
         cls = fake_gpu4pyscf.grad[kind].Gradients
         assert cls.jk_energy_per_atom is fastdfgrad.patched_jk_energy_per_atom
         assert fastdfgrad._orig_methods[cls] is fake_gpu4pyscf.orig_jk_energy_per_atom

@@ -39,8 +39,18 @@ def energy_parts(ao, aux, D):
     return ej, ek
 
 
-def fd_gradient(mol, D, h=2e-3, atoms=None):
-    """(natm, 3, 2) five-point derivative of (E_J, E_K) per atom coordinate."""
+def energy_parts_spin(ao, aux, Da, Db):
+    """(E_J of Da + Db, -1/2 sum_s tr(D_s K[D_s])) of the oracle's DF energy
+    for an unrestricted density (each spin's exchange twice the restricted
+    form's -1/4 weight)."""
+    ej, _ = energy_parts(ao, aux, Da + Db)
+    return ej, 2 * (energy_parts(ao, aux, Da)[1] + energy_parts(ao, aux, Db)[1])
+
+
+def fd_gradient(mol, D, h=2e-3, atoms=None, energy=None):
+    """(natm, 3, 2) five-point derivative of (E_J, E_K) per atom coordinate
+    (``energy(ao, aux)``, default the restricted parts for D)."""
+    energy = energy or (lambda ao, aux: energy_parts(ao, aux, D))
     ao0, aux0 = mol.ao.as_dict(), mol.aux.as_dict()
     atoms = range(mol.natm) if atoms is None else atoms
     out = np.zeros((mol.natm, 3, 2))
@@ -51,7 +61,7 @@ def fd_gradient(mol, D, h=2e-3, atoms=None):
                 ao, aux = _copy(ao0), _copy(aux0)
                 ao['xyz'][ao['atom'] == a, x] += s * h
                 aux['xyz'][aux['atom'] == a, x] += s * h
-                vals.append(np.array(energy_parts(ao, aux, D)))
+                vals.append(np.array(energy(ao, aux)))
             out[a, x] = (vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h)
     return out
 
@@ -224,3 +234,37 @@ def test_gradient_eigen_route(mol, monkeypatch):
     g = fastdfgrad.jk_energy_per_atom(df, D, 1., .5)
     assert df._mdf_grad_metric[2] is False                   # the general (non-triangular) branch ran
     assert np.abs(g - ref).max() < 1e-7
+
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('tagged', [True, False])
+def test_unrestricted_gradient_matches_finite_differences(mol, tagged):
+    """Spin densities (2, nao, nao) with different occupations: J of the
+    total density, K per spin (-1/2 k sum_s tr(D_s K[D_s])), against the
+    five-point derivative of the oracle's energy; a closed-shell pair
+    (D_a = D_b = D/2) equals the restricted gradient of D."""
+    from electrolites_amd import fastdf, fastdfgrad
+    from electrolites_amd.df import DF
+    rng = np.random.default_rng(9)
+    q = np.linalg.qr(rng.standard_normal((mol.nao, 9)))[0]
+    ca, cb = q[:, :9], q[:, :7]
+    Da, Db = ca @ ca.T, cb @ cb.T
+    fd = fd_gradient(mol, None, energy=lambda ao, aux: energy_parts_spin(ao, aux, Da, Db))
+    dm = np.stack([Da, Db])
+    if tagged:
+        mo = np.stack([q[:, :9], q[:, :9]])
+        occ = np.array([[1.] * 9, [1.] * 7 + [0.] * 2])
+        dm = fastdf.tag_array(dm, mo_coeff=mo, mo_occ=occ)
+    df = DF(mol).build()
+    for jf, kf in [(1., 0.), (0., 1.), (1., .5)]:
+        g = fastdfgrad.jk_energy_per_atom(df, dm, j_factor=jf, k_factor=kf)
+        ref = jf * fd[..., 0] + kf * fd[..., 1]
+        err = np.abs(g - ref).max()
+        print(f'UHF j={jf} k={kf} max|g - fd| = {err:.2e}, max|g| = {np.abs(g).max():.3f}', flush=True)
+        assert err < 1e-7
+        assert np.abs(g.sum(axis=0)).max() < 1e-9
+    D, _ = _density(mol.nao, 8, 10)
+    gu = fastdfgrad.jk_energy_per_atom(df, np.stack([D / 2, D / 2]), 1., 1.)
+    gr = fastdfgrad.jk_energy_per_atom(df, D, 1., 1.)
+    assert np.abs(gu - gr).max() < 1e-10
