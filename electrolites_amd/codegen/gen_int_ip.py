@@ -82,7 +82,7 @@ def _deriv(d, a, b, k, wrt):
     return e
 
 
-KCH = 16      # aux shells one thread of a three-centre kernel loops over
+KCH = 4       # default aux shells one thread of a three-centre kernel loops over
 QCH = 16      # aux shells one thread of a two-centre kernel loops over
 LDS_MAX_ATOMS = 1024
 #: a three-centre class whose Cartesian weight block times the aux components
@@ -127,12 +127,12 @@ def gen3(la, lb, lc):
       'const int *__restrict__ pairs, int np, const int *__restrict__ kl, int nk, '
       'const double *__restrict__ W, long ldw, const int *__restrict__ cmap, '
       'const int *__restrict__ ao_atom, const int *__restrict__ aux_atom, int natm, int use_lds, '
-      'double *grad, double *grad_aux) {')
+      'int kch, double *grad, double *grad_aux) {')
     w('    const int nslot = 6;')
     _head(w)
     am = allm_ip(la, lb, lc)
     ms = list(range(2 * lc + 1)) if am else [None]
-    w(f'    const int nkc = (nk + {KCH - 1}) / {KCH};')
+    w('    const int nkc = (nk + kch - 1) / kch;')
     w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
     if am:
         w('    if (tid < (long)np * nkc) {')
@@ -160,8 +160,8 @@ def gen3(la, lb, lc):
     w('    int curC = -1;')
     w('    const int npa = ao.nprim[ish], oa = ao.poff[ish];')
     w('    const int npb = ao.nprim[jsh], ob = ao.poff[jsh];')
-    w(f'    const int k1 = min(nk, (kc + 1) * {KCH});')
-    w(f'    for (int kq = kc * {KCH}; kq < k1; kq++) {{')
+    w('    const int k1 = min(nk, (kc + 1) * kch);')
+    w('    for (int kq = kc * kch; kq < k1; kq++) {')
     w('    const int ksh = kl[kq];')
     w('    const double Cx = aux.xyz[3 * ksh], Cy = aux.xyz[3 * ksh + 1], Cz = aux.xyz[3 * ksh + 2];')
     for mi in ms:
@@ -358,7 +358,7 @@ def gen2(la, lc):
 def main():
     parts = ['// Generated by electrolites_amd/codegen/gen_int_ip.py; do not edit.',
              '// Nuclear-derivative Rys kernels of the DF gradient, one per angular class.',
-             '#include "mdf_int.h"', '', 'namespace mdf {', '',
+             '#include <cstdlib>', '#include "mdf_int.h"', '', 'namespace mdf {', '',
              '// one atom\'s (x, y, z) into slot part (0 orbital centres, 1 aux centres) of the',
              '// block\'s LDS sums, or straight into the gradient g without them',
              '__device__ __forceinline__ void ip_add(double *sg, int use_lds, int natm, double *g, int atom, '
@@ -367,6 +367,17 @@ def main():
              '    atomicAdd(d, x);',
              '    atomicAdd(d + 1, y);',
              '    atomicAdd(d + 2, z);',
+             '}', '',
+             '// aux shells one thread of a three-centre kernel loops over (MDF_IP_KCH, default '
+             f'{KCH})',
+             'static int ip_kch() {',
+             '    static int v = 0;',
+             '    if (!v) {',
+             '        const char *e = getenv("MDF_IP_KCH");',
+             f'        const int x = e ? atoi(e) : {KCH};',
+             f'        v = x >= 1 && x <= 4096 ? x : {KCH};',
+             '    }',
+             '    return v;',
              '}', '']
     n3, n2 = {}, {}
     for la in range(LMAX_AO + 1):
@@ -383,7 +394,8 @@ def main():
     parts.append('int launch_int3c_ip(int la, int lb, int lc, const BasisDev &ao, const BasisDev &aux, '
                  'const int *pairs, int np, const int *kl, int nk, const double *W, long ldw, const int *cmap, '
                  'const int *ao_atom, const int *aux_atom, int natm, double *grad, double *grad_aux, hipStream_t st) {')
-    parts.append(f'    const long nkc = (nk + {KCH - 1}) / {KCH};')
+    parts.append('    const int kch = ip_kch();')
+    parts.append('    const long nkc = (nk + kch - 1) / kch;')
     parts.append(f'    const int use_lds = natm <= {LDS_MAX_ATOMS};')
     parts.append('    const size_t shm = use_lds ? 6 * (size_t)natm * sizeof(double) : 0;')
     parts.append('    if ((long)np * nkc == 0) return MDF_OK;')
@@ -395,7 +407,7 @@ def main():
         parts.append(f'    case {key}: {{ const long n = (long)np * nkc * {mult}; '
                      f'const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})); '
                      f'hipLaunchKernelGGL({n}, g, b, shm, st, ao, aux, pairs, np, kl, nk, W, ldw, '
-                     'cmap, ao_atom, aux_atom, natm, use_lds, grad, grad_aux); } break;')
+                     'cmap, ao_atom, aux_atom, natm, use_lds, kch, grad, grad_aux); } break;')
     parts.append('    default: set_error("int3c2e_ip: class (%d %d|%d) not generated", la, lb, lc); return MDF_EINVAL;')
     parts.append('    }')
     parts.append('    MDF_LAUNCHED("int3c_ip");')
