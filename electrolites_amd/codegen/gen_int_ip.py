@@ -200,7 +200,9 @@ def gen3(la, lb, lc):
     w('                const double pref = TWO_PI_2P5 / (p * q * sqrt(pq)) * fab * aux.coefs[oc + ic];')
     w(f'                double ru[{nroots}], rw[{nroots}];')
     w(f'                rys_roots<{nroots}>(x, ru, rw);')
-    w('#pragma unroll')
+    # the roots one at a time in the classes whose terms keep every Rys factor
+    # live across the per-component switch (unrolled, they spilled)
+    w('#pragma unroll' if am else '#pragma unroll 1')
     w(f'                for (int r = 0; r < {nroots}; r++) {{')
     w('                    const double u = ru[r];')
     w('                    const double B00 = 0.5 * u * ipq;')
