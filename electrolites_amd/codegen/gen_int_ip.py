@@ -76,7 +76,9 @@ def _deriv(d, a, b, k, wrt):
         if a:
             e = f'({e} - {a}.0 * I{d}_{a-1}_{b}_{k})'
     else:
-        e = f'2.0 * eb * I{d}_{a}_{b+1}_{k}'
+        # I(a, b + 1) = I(a + 1, b) + AB I(a, b): the horizontal transfer one
+        # level short, so no I(., lb + 1) is kept live
+        e = f'2.0 * eb * (I{d}_{a+1}_{b}_{k} + AB{d} * I{d}_{a}_{b}_{k})'
         if b:
             e = f'({e} - {b}.0 * I{d}_{a}_{b-1}_{k})'
     return e
@@ -213,7 +215,7 @@ def gen3(la, lb, lc):
     for d in 'xyz':
         w(f'                    const double C00{d} = PA{d} - qf * PC{d}, C0p{d} = pf * PC{d};')
     ind = '                    '
-    _vrr_hrr(w, lab + 1, lc, lb + 1, ind)
+    _vrr_hrr(w, lab + 1, lc, lb, ind)
     def products(mi, wname, indent):
         sup = [(k, Cc[mi, k]) for k in range(len(cc)) if abs(Cc[mi, k]) > 1e-15]
         for a_i, (ax, ay, az) in enumerate(ca):
