@@ -46,6 +46,7 @@ class DF:
         self._cderi_idx = None
         self.__dict__.pop('_mdf_layout', None)
         self.__dict__.pop('_mdf_grad_metric', None)
+        self.__dict__.pop('_mdf_grad_pairs', None)
         return self
 
     def get_jk(self, dm, hermi=1, with_j=True, with_k=True, omega=None):

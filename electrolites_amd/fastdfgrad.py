@@ -94,15 +94,24 @@ def _class_sorted_pairs(mol, addr):
     return pairs, off, cmap
 
 
+def _cache_key(dfobj, mol):
+    """What the cached per-cderi tables depend on: the tensor, its pair list
+    and the basis (positions, exponents, coefficients of both sets)."""
+    cderi = fastdf._cderi_of(dfobj)
+    idx = getattr(dfobj, '_cderi_idx', None)
+    basis = b''.join(np.ascontiguousarray(a).tobytes() for sh in (mol.ao, mol.aux)
+                     for a in (sh.shl_xyz, sh.exps, sh.coefs))
+    return (id(cderi), tuple(cderi.shape), None if idx is None else id(idx[0]), basis)
+
+
 def _metric_factor(dfobj, mol, aux):
     """``(A, lower)``: the matrix the cderi is defined by (cderi = A T; the
     inverse Cholesky factor of the metric, lower triangular, or the eigen
     filter's coefficients), cached on the DF object for its current cderi
-    and geometry (the reference reuses the build's ``aux_coef`` the same
-    way, electrolites/fastdfgrad.py:665-714)."""
+    and basis (the reference reuses the build's ``aux_coef`` the same way,
+    electrolites/fastdfgrad.py:665-714)."""
     t = _dev.torch()
-    cderi = fastdf._cderi_of(dfobj)
-    key = (id(cderi), tuple(cderi.shape), np.asarray(mol.aux.shl_xyz).tobytes())
+    key = _cache_key(dfobj, mol)
     cached = getattr(dfobj, '_mdf_grad_metric', None)
     if cached is not None and cached[0] == key:
         return cached[1], cached[2]
@@ -113,6 +122,33 @@ def _metric_factor(dfobj, mol, aux):
     except AttributeError:
         pass
     return A, lower
+
+
+def _pair_tables(dfobj, mol):
+    """The device tables of the derivative launch and the weight build for
+    the cderi's pair list (class-sorted shell pairs, packed-pair -> column
+    map, column addresses, lower-triangle blocks, shell atoms), cached on the
+    DF object like the metric factor."""
+    t = _dev.torch()
+    key = _cache_key(dfobj, mol)
+    cached = getattr(dfobj, '_mdf_grad_pairs', None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    addr = np.asarray(dfobj._cderi_idx[0])
+    pairs, class_off, cmap = _class_sorted_pairs(mol, addr)
+    i32 = lambda a: _dev.to_device(np.ascontiguousarray(a, dtype=np.int32), dtype=t.int32)
+    tabs = {
+        'ncol': len(addr), 'class_off': class_off,
+        'pairs': i32(pairs.reshape(-1)) if len(pairs) else t.zeros(2, dtype=t.int32, device=_dev.device()),
+        'cmap': i32(cmap), 'addr': _dev.to_device(addr.astype(np.int64), dtype=t.int64),
+        'tri': _lower_blocks(addr, mol.nao),
+        'ao_atom': i32(mol.ao.shl_atom), 'aux_atom': i32(mol.aux.shl_atom),
+    }
+    try:
+        dfobj._mdf_grad_pairs = (key, tabs)
+    except AttributeError:
+        pass
+    return tabs
 
 
 def _factors(dm, hermi):
@@ -273,18 +309,16 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     c = A.T @ rho                                               # (naux,)
     Z = (-.5 * j_factor) * t.outer(c, c)
     # W in the cderi's compact columns
-    addr = np.asarray(dfobj._cderi_idx[0])
-    pairs, class_off, cmap = _class_sorted_pairs(mol, addr)
-    ncol = len(addr)
+    tabs = _pair_tables(dfobj, mol)
+    ncol = tabs['ncol']
     W = t.empty((naux, ncol), dtype=t.float64, device=_dev.device())
-    addr_d = _dev.to_device(addr.astype(np.int64), dtype=t.int64)
-    Dflat = D.reshape(-1)[addr_d]
+    Dflat = D.reshape(-1)[tabs['addr']]
     jb = max(1, min(naux, WBLOCK_BYTES // max(8 * ncol, 1)))
     for P0 in range(0, naux, jb):
         P1 = min(naux, P0 + jb)
         W[P0:P1] = (j_factor * c[P0:P1, None]) * Dflat[None, :]
     if k_factor != 0:
-        tri = _lower_blocks(addr, nao)
+        tri = tabs['tri']
         for si, (fl, fr) in enumerate(facs):
             nocc = fl.shape[1]
             dmoo = _dmoo(A, lower, rhoks[si], naux)
@@ -304,10 +338,8 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     del rhoks
     grad = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())
     grad_aux = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())
-    i32 = lambda a: _dev.to_device(np.ascontiguousarray(a, dtype=np.int32), dtype=t.int32)
-    pairs_d = i32(pairs.reshape(-1)) if len(pairs) else t.zeros(2, dtype=t.int32, device=_dev.device())
-    cmap_d = i32(cmap)
-    ao_atom, aux_atom = i32(mol.ao.shl_atom), i32(mol.aux.shl_atom)
+    pairs_d, cmap_d, class_off = tabs['pairs'], tabs['cmap'], tabs['class_off']
+    ao_atom, aux_atom = tabs['ao_atom'], tabs['aux_atom']
     lp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
     if ao.desc.lmax > ints.LMAX_AO or aux.desc.lmax > ints.LMAX_AUX:
         raise ValueError('jk_energy_per_atom: angular momentum beyond the generated classes')
