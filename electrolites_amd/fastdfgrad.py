@@ -194,6 +194,28 @@ class _Stages:
             self.out[n] = self.out.get(n, 0.) + a.elapsed_time(b)
 
 
+def _on_device(cderi):
+    t = _dev.torch()
+    return isinstance(cderi, t.Tensor) and cderi.is_cuda and cderi.dtype in (t.float64, t.float32) \
+        and cderi.dim() == 2 and cderi.stride(1) == 1
+
+
+def _rho_packed(dfobj, cderi, D, nao, nrow):
+    """``rho_B[L] = sum_ij B_L,ij D_ij`` from the packed rows (``mdf_j_rho``
+    over the density on the stored pairs, off-diagonal pairs doubled)."""
+    t = _dev.torch()
+    if cderi.shape[0] != nrow:
+        raise ValueError(f'jk_energy_per_atom: the cderi has {cderi.shape[0]} rows, the metric factor {nrow}')
+    lay = fastdf._layout(dfobj, cderi, nao)
+    npairs = cderi.shape[1]
+    d = fastdf._sparse_dm(D[None].contiguous(), lay, 1)
+    rho = t.empty(nrow, dtype=t.float64, device=_dev.device())
+    nbytes = _lib.query('mdf_get_j_workspace', nrow, npairs, 1)
+    work = _dev.WORKSPACE.get(nbytes)
+    _lib.call('mdf_j_rho', _dev.ptr(cderi), fastdf._dtype_code(cderi), nrow, npairs, cderi.stride(0),
+              _dev.ptr(d), 1, npairs, _dev.ptr(rho), nrow, 0, _dev.ptr(work), nbytes, _dev.stream_ptr())
+    return rho
+
 def _rhok(dfobj, fl, fr, nao, nrow):
     """``rhok[L] = F_r^T A_L F_l`` for every cderi row L."""
     t = _dev.torch()
@@ -201,8 +223,7 @@ def _rhok(dfobj, fl, fr, nao, nrow):
     rhok = t.empty((nrow, nocc, nocc), dtype=t.float64, device=_dev.device())
     frT = fr.T.contiguous()
     cderi = fastdf._cderi_of(dfobj)
-    if isinstance(cderi, t.Tensor) and cderi.is_cuda and cderi.dtype in (t.float64, t.float32) \
-            and cderi.dim() == 2 and cderi.stride(1) == 1:
+    if _on_device(cderi):
         # R_L = A_L F_l by the K engine's first step, straight from the packed rows
         if cderi.shape[0] != nrow:
             raise ValueError(f'jk_energy_per_atom: the cderi has {cderi.shape[0]} rows, the metric factor {nrow}')
@@ -302,9 +323,14 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     A, lower = _metric_factor(dfobj, mol, aux)            # cderi = A T
     nrow, naux = A.shape
     tm.mark('rhok')
-    # rhok_s from the cderi's blocks; rho_B[L] = tr(A_L D) = sum_s tr(rhok_s[L])
-    rhoks = [_rhok(dfobj, fl, fr, nao, nrow) for fl, fr in facs]
-    rho = sum(t.diagonal(r, dim1=1, dim2=2).sum(dim=1) for r in rhoks)
+    cderi = fastdf._cderi_of(dfobj)
+    if k_factor == 0 and _on_device(cderi):
+        # Coulomb only: rho_B = B d over the packed rows (J's first pass)
+        rhoks, rho = [], _rho_packed(dfobj, cderi, D, nao, nrow)
+    else:
+        # rhok_s from the cderi's blocks; rho_B[L] = tr(A_L D) = sum_s tr(rhok_s[L])
+        rhoks = [_rhok(dfobj, fl, fr, nao, nrow) for fl, fr in facs]
+        rho = sum(t.diagonal(r, dim1=1, dim2=2).sum(dim=1) for r in rhoks)
     tm.mark('weights')
     c = A.T @ rho                                               # (naux,)
     Z = (-.5 * j_factor) * t.outer(c, c)
