@@ -4,10 +4,18 @@ Torch is used only to own device memory and to name the current HIP stream;
 all arithmetic of the path is in libmi355df.
 """
 import ctypes
+import threading
 
 import numpy as np
 
 _torch = None
+
+#: Held around every device dense-solver call of this package (eigh, SVD,
+#: the DIIS solve).  Several host threads issuing hipSOLVER ``syevd`` at once
+#: (a serving loop absorbing requests from a thread pool, one HIP stream
+#: each) crashed inside ``hipsolverDsyevd`` with a SIGSEGV; with the lock no
+#: two threads are ever inside the solver library together.
+SOLVER_LOCK = threading.RLock()
 
 
 def torch():
@@ -40,19 +48,86 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def is_foreign_device_array(x):
+    """A device array of another library (CuPy, Numba, ...): anything that is
+    not a torch tensor and exposes the CUDA array interface."""
+    return not isinstance(x, torch().Tensor) and hasattr(x, '__cuda_array_interface__')
+
+
 def is_device_array(x):
     t = torch()
-    return isinstance(x, t.Tensor) and x.is_cuda
+    return (isinstance(x, t.Tensor) and x.is_cuda) or is_foreign_device_array(x)
+
+
+def as_torch(x):
+    """A torch view of ``x`` without a copy where ``x`` is a device array:
+    torch tensors as they are, objects exposing ``__cuda_array_interface__``
+    (CuPy's arrays, GPU4PySCF's ``_cderi`` and densities) wrapped in place,
+    strides kept.  Anything else is returned unchanged."""
+    t = torch()
+    if is_foreign_device_array(x):
+        cai = x.__cuda_array_interface__
+        if cai.get('mask') is not None:
+            raise ValueError('masked device arrays are not supported')
+        return t.as_tensor(x, device=device())
+    return x
+
+
+def to_host(x):
+    """numpy array of a numpy/torch array or of a CUDA-array-interface
+    object (copied to the host)."""
+    x = as_torch(x)
+    if isinstance(x, torch().Tensor):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+def shape_of(x):
+    """Shape of a numpy/torch array or of a CUDA-array-interface object."""
+    if is_foreign_device_array(x) and not hasattr(x, 'shape'):
+        return tuple(x.__cuda_array_interface__['shape'])
+    return tuple(x.shape)
 
 
 def to_device(x, dtype=None):
-    """numpy / torch -> contiguous device tensor (float64 by default)."""
+    """numpy / torch / CUDA-array-interface object -> contiguous device tensor
+    (float64 by default)."""
     t = torch()
     dtype = dtype or t.float64
+    x = as_torch(x)
     if isinstance(x, t.Tensor):
         return x.to(device=device(), dtype=dtype).contiguous()
     return t.from_numpy(np.ascontiguousarray(x)).to(device=device(),
                                                     dtype=dtype).contiguous()
+
+
+def like_input(x, ref):
+    """Return the device tensor ``x`` in the family of the input ``ref``:
+    a CuPy array for a CuPy input (zero copy), the torch tensor itself for a
+    torch input or any other object exposing the CUDA array interface (a
+    torch tensor exposes that interface too), and a numpy array for host
+    input."""
+    if isinstance(ref, torch().Tensor):
+        return x
+    if is_foreign_device_array(ref):
+        mod = type(ref).__module__.split('.')[0]
+        if mod == 'cupy':
+            import cupy
+            return cupy.asarray(x)
+        return x
+    return x.cpu().numpy()
+
+
+def is_oom(exc):
+    """Whether ``exc`` reports device memory exhaustion (torch's allocator,
+    the HIP runtime through this package's library, or CuPy's pool)."""
+    t = torch()
+    if isinstance(exc, getattr(t.cuda, 'OutOfMemoryError', ())) or isinstance(exc, MemoryError):
+        return True
+    name = type(exc).__name__
+    msg = str(exc)
+    return name == 'OutOfMemoryError' or 'out of memory' in msg.lower() or \
+        'hipErrorOutOfMemory' in msg or 'cudaErrorMemoryAllocation' in msg
 
 
 class Workspace:

@@ -9,7 +9,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, 'libmi355df.so')
-ABI_VERSION = 7
+ABI_VERSION = 8
 DTYPE_F64, DTYPE_F32 = 0, 1
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -37,6 +37,7 @@ _SIGS = {
     'mdf_cast_f32': (I, [P, L, L, L, P, L, P]),
     'mdf_widen_f32': (I, [P, L, L, L, P, L, P]),
     'mdf_copy_2d': (I, [P, L, P, L, L, L, P]),
+    'mdf_transpose': (I, [P, I, L, L, L, P, L, P]),
     'mdf_j_rho': (I, [P, I, L, L, L, P, I, L, P, L, I, P, SZ, P]),
     'mdf_j_vj': (I, [P, I, L, L, L, P, I, L, P, L, I, P, SZ, P]),
     'mdf_int3c2e': (I, [P, P, P, P, P, P, L, P, P, L, P]),

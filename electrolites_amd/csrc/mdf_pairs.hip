@@ -195,3 +195,47 @@ extern "C" int mdf_build_pair_map(const int *pair_addr, long npairs, int nao, in
     MDF_LAUNCHED("pair_map_kernel");
     return MDF_OK;
 }
+
+namespace {
+// dst[c][r] = src[r][c] through a 64 x 64 LDS tile (one pad column keeps
+// the column reads off a single bank); both sides read and write whole
+// 64-element row segments.  Grid-strided over tiles in both directions.
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_kernel(const T *__restrict__ src, long rows, long cols, long lds,
+                                                        T *__restrict__ dst, long ldd) {
+    __shared__ T tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (long rt = blockIdx.y; rt * 64 < rows; rt += gridDim.y) {
+        for (long ct = blockIdx.x; ct * 64 < cols; ct += gridDim.x) {
+            const long r0 = rt * 64, c0 = ct * 64;
+            for (int k = ty; k < 64; k += 4) {
+                const long r = r0 + k, c = c0 + tx;
+                if (r < rows && c < cols) tile[k][tx] = src[r * lds + c];
+            }
+            __syncthreads();
+            for (int k = ty; k < 64; k += 4) {
+                const long c = c0 + k, r = r0 + tx;
+                if (c < cols && r < rows) dst[c * ldd + r] = tile[tx][k];
+            }
+            __syncthreads();
+        }
+    }
+}
+}  // namespace
+
+extern "C" int mdf_transpose(const void *src, int dtype, long rows, long cols, long lds, void *dst, long ldd,
+                             void *stream) {
+    MDF_CHECK_ARG(rows >= 0 && cols >= 0 && lds >= cols && ldd >= rows, "mdf_transpose: bad shape");
+    MDF_CHECK_ARG((src && dst) || rows * cols == 0, "mdf_transpose: null pointer");
+    MDF_CHECK_ARG(dtype == MDF_DTYPE_F64 || dtype == MDF_DTYPE_F32, "mdf_transpose: dtype %d", dtype);
+    if (rows * cols == 0) return MDF_OK;
+    const dim3 g((unsigned)std::min(mdf::cdiv(cols, 64L), 4096L), (unsigned)std::min(mdf::cdiv(rows, 64L), 4096L));
+    if (dtype == MDF_DTYPE_F64)
+        hipLaunchKernelGGL(transpose_kernel<double>, g, dim3(256), 0, mdf::as_stream(stream),
+                           static_cast<const double *>(src), rows, cols, lds, static_cast<double *>(dst), ldd);
+    else
+        hipLaunchKernelGGL(transpose_kernel<float>, g, dim3(256), 0, mdf::as_stream(stream),
+                           static_cast<const float *>(src), rows, cols, lds, static_cast<float *>(dst), ldd);
+    MDF_LAUNCHED("transpose_kernel");
+    return MDF_OK;
+}

@@ -107,7 +107,8 @@ def decompose_j2c(aux, timer=None):
         tm.mark('trtri:1')
         return linv, None
     STATS['eigen_fallback'] += 1
-    w, v = t.linalg.eigh(j2c)
+    with _dev.SOLVER_LOCK:
+        w, v = t.linalg.eigh(j2c)
     keep = w > LINEAR_DEP_THR
     coef = (v[:, keep] / t.sqrt(w[keep])).T.contiguous()
     return None, coef
@@ -149,7 +150,7 @@ def kept_pairs(mol, bound=None):
     return keep, rows, cols
 
 
-def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'):
+def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device', layout='pair'):
     """Build the cderi of ``mol`` (orbital basis ``mol.ao``, fitting basis
     ``mol.aux``) on the current device.  Returns ``(cderi, (pair_addr,
     diag_idx))``.  ``timer`` (a :class:`_Timer`) collects per-stage device
@@ -163,7 +164,13 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
     the tensor is the c-th kept pair in packed order, and the returned
     addresses ``i * nao + j`` name them.  Each chunk's integrals are written
     straight into those columns (``mdf_int3c2e`` with a column map), so the
-    metric solve and every later pass touch no screened column."""
+    metric solve and every later pass touch no screened column.
+
+    ``layout`` 'pair' (default) stores the rows pair-contiguous; 'aux' (a
+    device tensor only) returns the reference's device layout, the
+    ``(naux, npairs)`` transpose of a C-order ``(npairs, naux)`` buffer
+    (electrolites/fastdfcderi.py:648, :690): each chunk is computed as for
+    'pair' and written out transposed (``mdf_transpose``)."""
     _dev.require_gpu()
     t = _dev.torch()
     tm = timer or _NoTimer()
@@ -182,8 +189,15 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
     if storage not in ('device', 'host'):
         raise ValueError(f'storage {storage!r}: expected device or host')
     on_host = storage == 'host'
+    if layout not in ('pair', 'aux') or (layout == 'aux' and on_host):
+        raise ValueError(f'layout {layout!r} with storage {storage!r}')
+    auxc = layout == 'aux'
     sdt = t.float32 if fp32 else t.float64
-    cderi = None if on_host else t.empty((nrow, npairs), dtype=sdt, device=_dev.device())
+    cderi = None
+    if auxc:
+        cbuf = t.empty((npairs, nrow), dtype=sdt, device=_dev.device())
+    elif not on_host:
+        cderi = t.empty((nrow, npairs), dtype=sdt, device=_dev.device())
     budget = chunk_bytes or CHUNK_BYTES
     if fp32:
         budget //= 2                       # an FP64 output chunk beside the integrals
@@ -198,7 +212,8 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
 
     widest = max(max(span(I0, I1)[3] - span(I0, I1)[2] for I0, I1 in chunks), 1)
     buf = t.empty(naux * widest, dtype=t.float64, device=_dev.device())
-    out64 = t.empty(nrow * widest, dtype=t.float64, device=_dev.device()) if fp32 else None
+    out64 = t.empty(nrow * widest, dtype=t.float64, device=_dev.device()) if fp32 or auxc else None
+    out32 = t.empty(nrow * widest, dtype=t.float32, device=_dev.device()) if fp32 and auxc else None
     store = None
     if on_host:
         from . import dfstore
@@ -220,10 +235,13 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
         tm.mark('int3c2e:1')
         if on_host:
             dst, ldd = store.slot(width), width
+        elif auxc:
+            dst, ldd = (out32 if fp32 else out64), width
         else:
             dst, ldd = cderi[:, c0:], npairs
         # FP64 result straight into the tensor, or into a chunk then rounded
-        out, ldo = (out64, width) if fp32 else (dst, ldd)
+        # and/or transposed
+        out, ldo = (out64, width) if fp32 or auxc else (dst, ldd)
         tm.mark('trmm:0')
         if coef is None:
             _lib.call('mdf_trmm', _dev.ptr(linv), naux, naux, _dev.ptr(blk), width, width,
@@ -235,12 +253,17 @@ def cholesky_eri(mol, chunk_bytes=None, timer=None, dtype=None, storage='device'
         if fp32:
             _lib.call('mdf_cast_f32', _dev.ptr(out64), width, nrow, width, _dev.ptr(dst), ldd,
                       _dev.stream_ptr())
+        if auxc:
+            _lib.call('mdf_transpose', _dev.ptr(dst), 1 if fp32 else 0, nrow, width, width,
+                      _dev.ptr(cbuf[c0:]), nrow, _dev.stream_ptr())
         if on_host:
             store.write(dst, c0)
         STATS['chunks'] += 1
     if on_host:
         cderi = store.finish()
         STATS['host_builds'] = STATS.get('host_builds', 0) + 1
+    if auxc:
+        cderi = cbuf.T
     STATS['builds'] += 1
     rk, ck = rows[keep], cols[keep]
     pair_addr = (rk * nao + ck).astype(np.int32)
@@ -268,16 +291,73 @@ def tile_fill(mol, tile=128):
     return float(hit[np.tril_indices(nt)].mean())
 
 
+def _session_molecules(intopt):
+    """``(orbital, auxiliary)`` PySCF molecules of a GPU4PySCF
+    ``Int3c2eOpt`` (its ``mol`` and ``auxmol`` are sorted wrappers whose
+    ``.mol`` is the original, electrolites/fastdfcderi.py:380-386), or None
+    when ``intopt`` is not shaped that way."""
+    aux = getattr(intopt, 'auxmol', None)
+    mol = getattr(intopt, 'mol', None)
+    if aux is None or mol is None or hasattr(mol, 'ao'):
+        return None
+    mol = getattr(mol, 'mol', mol)
+    aux = getattr(aux, 'mol', aux)
+    if not all(hasattr(m, a) for m in (mol, aux) for a in ('_atm', '_bas', '_env')):
+        return None
+    return mol, aux
+
+
+def _as_session_array(x):
+    """A CuPy array over the same device memory when CuPy is importable
+    (a GPU4PySCF session), else the torch tensor (which exposes the CUDA
+    array interface)."""
+    try:
+        import cupy
+    except ImportError:
+        return x
+    return cupy.asarray(x)
+
+
 def _cholesky_eri(intopt, omega=None, use_gpu_memory=None):
-    """``df._cholesky_eri``'s signature.  ``intopt`` is this package's
-    :class:`~electrolites_amd.df.DF` (or anything with a ``mol`` carrying
-    ``ao``/``aux`` shell tables).  Range separation is not covered."""
+    """``df._cholesky_eri``'s signature (electrolites/fastdfcderi.py:352).
+
+    ``intopt`` is either this package's :class:`~electrolites_amd.df.DF`
+    (anything with a ``mol`` carrying ``ao``/``aux`` shell tables), which
+    gets the pair-contiguous tensor as a torch tensor, or GPU4PySCF's
+    ``Int3c2eOpt``, whose original molecules' libcint arrays give the shell
+    tables (:mod:`electrolites_amd.gto.libcint`) and which gets the
+    reference's return values: ``[cderi]`` with the device tensor in the
+    aux-contiguous layout (``out.T``, :690) or, for ``use_gpu_memory=False``,
+    a C-order host array (:576-577), and ``(pair_addr, diag_idx)`` as device
+    arrays (:446-449) -- CuPy arrays when CuPy is importable.  Range
+    separation is not covered."""
     if omega not in (None, 0, 0.):
         raise NotImplementedError('range-separated cderi (omega != 0) is not '
                                   'covered by the MI355X build')
-    mol = getattr(intopt, 'mol', intopt)
-    # use_gpu_memory False: GPU4PySCF's host-resident tensor (df/df.py), here
-    # a page-locked host tensor the J/K calls stream from
-    storage = 'host' if use_gpu_memory is False else getattr(intopt, 'storage', 'device')
-    cderi, idx = cholesky_eri(mol, storage=storage)
-    return [cderi], idx
+    pair = _session_molecules(intopt)
+    if pair is None:
+        mol = getattr(intopt, 'mol', intopt)
+        # use_gpu_memory False: GPU4PySCF's host-resident tensor (df/df.py),
+        # here a page-locked host tensor the J/K calls stream from
+        storage = 'host' if use_gpu_memory is False else getattr(intopt, 'storage', 'device')
+        cderi, idx = cholesky_eri(mol, storage=storage)
+        return [cderi], idx
+    from .gto.libcint import LibcintPair
+    mol = getattr(intopt, '_mdf_molecules', None)
+    if mol is None or mol.pyscf_mol is not pair[0]:
+        mol = LibcintPair(*pair)
+        try:
+            intopt._mdf_molecules = mol
+        except AttributeError:
+            pass
+    t = _dev.torch()
+    STATS['session_builds'] = STATS.get('session_builds', 0) + 1
+    if use_gpu_memory is False:
+        cderi, (pa, diag) = cholesky_eri(mol, storage='host')
+        cderi = cderi.numpy()
+    else:
+        cderi, (pa, diag) = cholesky_eri(mol, layout='aux')
+        cderi = _as_session_array(cderi)
+    pa = _as_session_array(_dev.to_device(pa, dtype=t.int32))
+    diag = _as_session_array(_dev.to_device(diag, dtype=t.int64))
+    return [cderi], (pa, diag)

@@ -134,7 +134,7 @@ def _pair_tables(dfobj, mol):
     cached = getattr(dfobj, '_mdf_grad_pairs', None)
     if cached is not None and cached[0] == key:
         return cached[1]
-    addr = np.asarray(dfobj._cderi_idx[0])
+    addr = _dev.to_host(dfobj._cderi_idx[0])
     pairs, class_off, cmap = _class_sorted_pairs(mol, addr)
     i32 = lambda a: _dev.to_device(np.ascontiguousarray(a, dtype=np.int32), dtype=t.int32)
     tabs = {
@@ -400,7 +400,7 @@ def _decline_reason(mf, dm, omega):
     nd = dm.dim() if hasattr(dm, 'dim') else np.ndim(dm)
     if not (nd == 2 or (nd == 3 and dm.shape[0] == 2)):
         return 'not one density or a spin pair'
-    d = dm.detach().cpu().numpy() if hasattr(dm, 'detach') else np.asarray(dm)
+    d = _dev.to_host(dm)
     if np.iscomplexobj(d):
         return 'complex density'
     d = d.reshape(-1, d.shape[-2], d.shape[-1])

@@ -104,7 +104,6 @@ def get_j(dfobj, dm, hermi=1):
     STATS['get_j'] += 1
     st = _state(dfobj)
     nao, naux, npairs = st.nao, st.naux, st.npairs
-    out_device = _dev.is_device_array(dm)
     dms3 = _dev.to_device(dm).reshape(-1, nao, nao)
     n = dms3.shape[0]
     d = t.empty((n, npairs), dtype=t.float64, device=_dev.device())
@@ -138,4 +137,4 @@ def get_j(dfobj, dm, hermi=1):
     vj = t.empty((n, nao, nao), dtype=t.float64, device=_dev.device())
     _lib.call('mdf_unpack_pairs', _dev.ptr(vjp), n, npairs, _dev.ptr(st.addr), npairs, nao, _dev.ptr(vj),
               nao * nao, _dev.stream_ptr())
-    return fastdf._finish(vj, dm, out_device)
+    return fastdf._finish(vj, _dev.shape_of(dm), dm)

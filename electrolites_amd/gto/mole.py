@@ -142,6 +142,37 @@ class ShellTable:
         self.lmax = int(self.shl_l.max()) if len(l_) else 0
         self.ao_loc = np.append(self.shl_ao_off, ao).astype(np.int32)
 
+    @classmethod
+    def from_shells(cls, atom_xyz, shells):
+        """A table from ``(atom, l, exps, coefs)`` tuples whose coefficients
+        are already normalised (libcint's stored form)."""
+        self = cls.__new__(cls)
+        l_, atom_, nprim_, off_, aoff_, exps, coefs = [], [], [], [], [], [], []
+        ao = 0
+        for ia, l, e, c in shells:
+            l_.append(l)
+            atom_.append(ia)
+            nprim_.append(len(e))
+            off_.append(len(exps))
+            aoff_.append(ao)
+            exps.extend(e)
+            coefs.extend(c)
+            ao += 2 * l + 1
+        self.shl_l = np.array(l_, dtype=np.int32)
+        self.shl_atom = np.array(atom_, dtype=np.int32)
+        self.shl_nprim = np.array(nprim_, dtype=np.int32)
+        self.shl_prim_off = np.array(off_, dtype=np.int32)
+        self.shl_ao_off = np.array(aoff_, dtype=np.int32)
+        self.exps = np.array(exps, dtype=np.float64)
+        self.coefs = np.array(coefs, dtype=np.float64)
+        self.shl_xyz = np.ascontiguousarray(
+            np.asarray(atom_xyz, dtype=np.float64).reshape(-1, 3)[self.shl_atom])
+        self.nbas = len(l_)
+        self.nao = ao
+        self.lmax = int(self.shl_l.max()) if len(l_) else 0
+        self.ao_loc = np.append(self.shl_ao_off, ao).astype(np.int32)
+        return self
+
     def as_dict(self):
         """Plain arrays, for code that must not depend on this class."""
         return dict(l=self.shl_l, atom=self.shl_atom, nprim=self.shl_nprim,

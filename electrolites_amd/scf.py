@@ -82,13 +82,16 @@ def host_eigh(a, concurrent=False):
 class RHF:
     def __init__(self, mol, conv_tol=1e-10, conv_tol_grad=None, max_cycle=100, diis_space=8,
                  df=None, verbose=0, k_precision='fp64', switch_err=1e-5, low_precision='fp32',
-                 host_diag_max=0):
+                 host_diag_max=0, precision_report=False):
         if k_precision not in ('fp64', 'fp32', 'bf16x3', 'mixed'):
             raise ValueError(f'k_precision {k_precision!r}')
         if low_precision not in ('fp32', 'bf16x3'):
             raise ValueError(f'low_precision {low_precision!r}')
         self.k_precision = k_precision
         self.low_precision = low_precision
+        # diagnostic: every low-precision cycle also builds the FP64 J/K of
+        # the same density and records the relative errors in ``history``
+        self.precision_report = precision_report
         self.host_diag_max = host_diag_max
         self.concurrent_diag = False      # set by a caller that holds single_thread_blas
         self.switch_err = switch_err
@@ -119,7 +122,8 @@ class RHF:
             mol._mdf_pair_bound = bound
         self.S, T, V = ints.int1e(ao, mol, bound)
         self.H = T + V
-        w, v = t.linalg.eigh(self.S)
+        with _dev.SOLVER_LOCK:
+            w, v = t.linalg.eigh(self.S)
         keep = w > 1e-9
         self.X = (v[:, keep] / t.sqrt(w[keep])).contiguous()
         self.enuc = mol.energy_nuc()
@@ -164,7 +168,8 @@ class RHF:
         B[:n, :n] = self._bm[:n, :n]
         rhs = t.zeros(n + 1, dtype=F.dtype, device=F.device)
         rhs[n] = -1.
-        coef = t.linalg.solve_ex(B, rhs)[0][:n]           # no host check: no sync
+        with _dev.SOLVER_LOCK:
+            coef = t.linalg.solve_ex(B, rhs)[0][:n]       # no host check: no sync
         return (coef.view(-1, 1, 1) * self._fh[:n]).sum(dim=0)
 
     def _diag(self, F):
@@ -180,7 +185,11 @@ class RHF:
             e = t.from_numpy(e).to(A.device)
             c = t.from_numpy(c).to(A.device)
         else:
-            e, c = t.linalg.eigh(A)
+            # one thread at a time inside the solver library (a serving loop
+            # absorbs requests from a thread pool; concurrent hipSOLVER syevd
+            # calls from several threads crashed the process)
+            with _dev.SOLVER_LOCK:
+                e, c = t.linalg.eigh(A)
         return e, self.X @ c
 
     # ------------------------------------------------------------- loop
@@ -190,9 +199,30 @@ class RHF:
         while not self.done:
             self.with_df.k_precision = self.prec
             vj, vk = self.with_df.get_jk(self.dm, hermi=1)
+            report = None
+            if self.precision_report and self.prec != 'fp64':
+                report = self._precision_errors(vj, vk)
             self.absorb(vj, vk)
+            if report is not None:
+                self.history[-1].update(report)
         self.with_df.k_precision = 'fp64'
         return self.e_tot
+
+    def _precision_errors(self, vj, vk):
+        """``max|X - X_fp64| / max|X_fp64|`` for J and K of the current
+        density, X from the low-precision call and X_fp64 from an FP64 call
+        on the same cderi (J is FP64 arithmetic in every mode, so its error
+        is that of the cderi's storage type alone)."""
+        t = _dev.torch()
+        prec = self.with_df.k_precision
+        self.with_df.k_precision = 'fp64'
+        try:
+            vj64, vk64 = self.with_df.get_jk(self.dm, hermi=1)
+        finally:
+            self.with_df.k_precision = prec
+        err = t.stack([(vj - vj64).abs().max() / vj64.abs().max(),
+                       (vk - vk64).abs().max() / vk64.abs().max()]).tolist()
+        return {'j_rel_err': err[0], 'k_rel_err': err[1]}
 
     def start(self, dm0=None):
         """Initial density; afterwards the caller alternates J/K of ``self.dm``

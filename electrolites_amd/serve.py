@@ -63,12 +63,15 @@ HBM_BUDGET = 250e9
 
 
 def footprint(mol, diis_space=8):
-    """Resident device bytes of one request: the packed cderi (upper bound:
-    every pair stored) plus its nao x nao matrices (S, H, X, DIIS Fock and
-    error history, J/K buffers)."""
+    """Resident device bytes of one request: the cderi over the pairs the
+    build stores (the screened-in pairs, counted on the host from the
+    shell-pair bounds before the build, :func:`fastdfcderi.kept_pairs`) plus
+    its nao x nao matrices (S, H, X, DIIS Fock and error history, J/K
+    buffers, the pair map)."""
+    from .fastdfcderi import kept_pairs
     nao, naux = mol.nao, mol.naux
-    npairs = nao * (nao + 1) // 2
-    return 8 * (naux * npairs + (2 * diis_space + 10) * nao * nao + 2 * npairs + naux)
+    npairs = int(kept_pairs(mol)[0].sum())
+    return 8 * (naux * npairs + (2 * diis_space + 10) * nao * nao + 2 * npairs + naux) + 4 * nao * nao
 
 
 def molecule_mix(n, seed=0, lo=3, hi=40):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 7
+#define MDF_ABI_VERSION 8
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
@@ -177,6 +177,14 @@ int mdf_widen_f32(const float *src, long lds, long rows, long cols, double *dst,
  * store's chunk write-out and block prefetch (electrolites/_dfstore.py, the
  * cudaMemcpy2DAsync of its 'stage' mode; electrolites/fastdf.py:611-671). */
 int mdf_copy_2d(void *dst, long dpitch, const void *src, long spitch, long width, long rows, void *stream);
+/* dst (cols, rows), row stride ldd, = the transpose of src (rows, cols), row
+ * stride lds; MDF_DTYPE_F64 or _F32 on both sides.  Reads the reference's
+ * aux-contiguous device cderi, the (naux, npairs) transpose of a C-order
+ * (npairs, naux) buffer (electrolites/fastdfcderi.py:648, :690), into the
+ * pair-contiguous rows the J and K kernels stream: once for a kept copy, or
+ * a block of aux rows at a time; and writes the build's pair-contiguous
+ * chunks out in that layout. */
+int mdf_transpose(const void *src, int dtype, long rows, long cols, long lds, void *dst, long ldd, void *stream);
 
 /* ---- Gaussian integrals (Rys quadrature, one kernel per angular class) --
  * A basis is a set of contracted shells; all arrays are DEVICE memory:

@@ -159,3 +159,49 @@ def test_fp32_storage_build(built, chunk_bytes):
     assert c32.dtype == torch.float32 and c32.shape == c64.shape
     assert torch.equal(c32, c64.float())
     assert np.array_equal(idx[0], fastdfcderi.cholesky_eri(mol, chunk_bytes=chunk_bytes)[1][0])
+
+
+@pytest.mark.gpu
+def test_eigen_fallback_on_singular_metric(monkeypatch):
+    """A fitting set with one shell duplicated has an exactly singular
+    metric; shifted by -1e-11 (planted on the host, so the pivot is negative
+    whatever the rounding) the Cholesky factorisation fails and the build
+    takes the eigen filter (electrolites/fastdfcderi.py:203-210): its cderi
+    spans the same fit as the oracle's eigen route (the Gram matrix
+    cderi^T cderi and J/K agree; the rows differ by an orthogonal rotation)."""
+    import torch
+    from electrolites_amd import fastdfcderi, fastdf, ints
+    from electrolites_amd.gto import Mole, basis as B
+    from electrolites_amd.gto.cluster import water_cluster_atoms
+    atoms = water_cluster_atoms(2, seed=3, side=5.0)
+    aux = B.load('etjk', ['O', 'H'])
+    aux['H'] = aux['H'] + [aux['H'][1]]               # one shell twice on every H
+    mol = Mole(atoms, basis='def2-svp', auxbasis=aux)
+    shift = 1e-11
+    orig = ints.int2c2e
+    monkeypatch.setattr(ints, 'int2c2e', lambda b: orig(b) - shift * torch.eye(b.nao, dtype=torch.float64,
+                                                                              device='cuda'))
+    n0 = fastdfcderi.STATS['eigen_fallback']
+    cderi, idx = fastdfcderi.cholesky_eri(mol)
+    assert fastdfcderi.STATS['eigen_fallback'] == n0 + 1
+    got = cderi.cpu().numpy()
+    ao_d, aux_d = mol.ao.as_dict(), mol.aux.as_dict()
+    j2c = O.int2c2e(aux_d) - shift * np.eye(mol.naux)
+    with pytest.raises(np.linalg.LinAlgError):
+        np.linalg.cholesky(j2c)
+    pa = np.asarray(idx[0]).astype(np.int64)
+    ref = O.cholesky_eri(O.int3c2e(ao_d, aux_d), j2c, pa)
+    assert got.shape == ref.shape and got.shape[0] < mol.naux      # the null directions dropped
+    assert rel(got.T @ got, ref.T @ ref) < 1e-9
+    rng = np.random.default_rng(2)
+    nocc = mol.nelectron // 2
+    c = np.linalg.qr(rng.standard_normal((mol.nao, nocc)))[0]
+    dm = fastdf.tag_array(2 * c @ c.T, mo_coeff=c, mo_occ=np.full(nocc, 2.))
+
+    class D:
+        _cderi = {0: cderi}
+        _cderi_idx = idx
+
+    vj, vk = fastdf.get_jk(D(), dm, hermi=1)
+    rj, rk = O.get_jk(ref, pa, np.asarray(dm))
+    assert rel(vj, rj) < 1e-9 and rel(vk, rk) < 1e-9

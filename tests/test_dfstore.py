@@ -103,6 +103,11 @@ def test_host_resident_jk(mol, monkeypatch, dtype, frac, max_rows, tagged, n_dm)
         vj, vk = host.get_jk(dm, hermi=1)
         rj, rk = dev.get_jk(dm, hermi=1)
         assert rel(vj, rj) < 1e-12 and rel(vk, rk) < 1e-12
+    # and against the oracle on the stored values (widened for FP32)
+    stored = host.cderi().double().numpy()
+    pa = np.asarray(host._cderi_idx[0]).astype(np.int64)
+    oj, ok = O.get_jk(stored, pa, np.asarray(dm))
+    assert rel(vj, oj) < 1e-12 and rel(vk, ok) < 1e-12
     st = dfstore.stats()
     naux = host.cderi().shape[0]
     res = 0 if frac == 0 else (min(max_rows, naux) if max_rows else naux)
