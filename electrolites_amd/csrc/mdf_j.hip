@@ -56,39 +56,79 @@ __global__ __launch_bounds__(J_THREADS) void j_vj_reduce(const double *__restric
     vj_reduce_range(part, ngroup, ndm, npairs, vj, ldvj, accumulate, e0, min(e0 + J_THREADS, (long)ndm * npairs));
 }
 
+// The same tiles under their own names for the two halves called one
+// chunk at a time (mdf_j_rho / mdf_j_vj: the J build without a stored cderi
+// and the DF gradient's Coulomb density), so a kernel trace separates them
+// from the full J of mdf_get_j.
+template <int NDM, int VEC, typename S>
+__global__ __launch_bounds__(J_THREADS) void jh_rho_partial(const S *__restrict__ L, long ld, long naux,
+                                                            long npairs, const double *__restrict__ d, long ldd,
+                                                            double *__restrict__ part) {
+    rho_partial_tile<NDM, VEC, S>(L, ld, naux, npairs, d, ldd, NDM, part, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(J_THREADS) void jh_rho_reduce(const double *__restrict__ part, long nchunk, int ndm,
+                                                           long naux, double *__restrict__ rho, long ldrho,
+                                                           int accumulate) {
+    const long e0 = (long)blockIdx.x * J_THREADS;
+    rho_reduce_range(part, nchunk, ndm, naux, rho, ldrho, e0, min(e0 + J_THREADS, (long)ndm * naux), accumulate);
+}
+
+template <int NDM, int VEC, typename S>
+__global__ __launch_bounds__(J_THREADS) void jh_vj_pass(const S *__restrict__ L, long ld, long naux,
+                                                        long npairs, const double *__restrict__ rho, long ldrho,
+                                                        long rows_per_group, double *__restrict__ out, long ldo,
+                                                        long group_stride) {
+    const long r0 = (long)blockIdx.y * rows_per_group;
+    const long r1 = min(r0 + rows_per_group, naux);
+    vj_pass_tile<NDM, VEC, S>(L, ld, npairs, rho, ldrho, NDM, r0, r1, out + (long)blockIdx.y * group_stride, ldo,
+                              blockIdx.x);
+}
+
+__global__ __launch_bounds__(J_THREADS) void jh_vj_reduce(const double *__restrict__ part, long ngroup, int ndm,
+                                                          long npairs, double *__restrict__ vj, long ldvj,
+                                                          int accumulate) {
+    const long e0 = (long)blockIdx.x * J_THREADS;
+    vj_reduce_range(part, ngroup, ndm, npairs, vj, ldvj, accumulate, e0, min(e0 + J_THREADS, (long)ndm * npairs));
+}
+
 // passes: 1 = rho only (rho += with accumulate_rho), 2 = vj only (from the
-// given rho), 3 = both (rho written, then vj)
+// given rho), 3 = both (rho written, then vj); halves: launch the jh_* names
 template <int NDM, int VEC, typename S>
 int run_j_group(const S *L, long ld, long naux, long npairs, const double *d, long ldd,
                 double *rho, long ldrho, double *vj, long ldvj, int accumulate,
-                double *work, const JPlan &pl, hipStream_t st, int passes = 3, int accumulate_rho = 0) {
+                double *work, const JPlan &pl, hipStream_t st, int passes = 3, int accumulate_rho = 0,
+                bool halves = false) {
     double *part1 = work;
     double *part2 = work + align_up(pl.part1_words, 64);
     long nt;
     if (passes & 1) {
         dim3 g1((unsigned)pl.nchunk, (unsigned)pl.nrblock);
-        hipLaunchKernelGGL((j_rho_partial<NDM, VEC, S>), g1, dim3(J_THREADS), 0, st, L, ld, naux, npairs, d, ldd,
-                           part1);
+        auto rho_partial = halves ? jh_rho_partial<NDM, VEC, S> : j_rho_partial<NDM, VEC, S>;
+        hipLaunchKernelGGL(rho_partial, g1, dim3(J_THREADS), 0, st, L, ld, naux, npairs, d, ldd, part1);
         MDF_LAUNCHED("j_rho_partial");
         nt = (long)NDM * naux;
-        hipLaunchKernelGGL(j_rho_reduce, dim3((unsigned)cdiv(nt, (long)J_THREADS)), dim3(J_THREADS), 0, st,
-                           part1, pl.nchunk, NDM, naux, rho, ldrho, accumulate_rho);
+        auto rho_reduce = halves ? jh_rho_reduce : j_rho_reduce;
+        hipLaunchKernelGGL(rho_reduce, dim3((unsigned)cdiv(nt, (long)J_THREADS)), dim3(J_THREADS), 0, st, part1,
+                           pl.nchunk, NDM, naux, rho, ldrho, accumulate_rho);
         MDF_LAUNCHED("j_rho_reduce");
     }
     if (!(passes & 2)) return MDF_OK;
     dim3 g2((unsigned)pl.ncblock, (unsigned)pl.ngroup);
+    auto vj_pass = halves ? jh_vj_pass<NDM, VEC, S> : j_vj_pass<NDM, VEC, S>;
     if (pl.ngroup == 1 && !accumulate) {
-        hipLaunchKernelGGL((j_vj_pass<NDM, VEC, S>), g2, dim3(J_THREADS), 0, st,
+        hipLaunchKernelGGL(vj_pass, g2, dim3(J_THREADS), 0, st,
                            L, ld, naux, npairs, rho, ldrho, pl.rows_per_group, vj, ldvj, 0L);
         MDF_LAUNCHED("j_vj_pass");
         return MDF_OK;
     }
-    hipLaunchKernelGGL((j_vj_pass<NDM, VEC, S>), g2, dim3(J_THREADS), 0, st,
+    hipLaunchKernelGGL(vj_pass, g2, dim3(J_THREADS), 0, st,
                        L, ld, naux, npairs, rho, ldrho, pl.rows_per_group, part2, npairs, (long)NDM * npairs);
     MDF_LAUNCHED("j_vj_pass");
     nt = (long)NDM * npairs;
-    hipLaunchKernelGGL(j_vj_reduce, dim3((unsigned)cdiv(nt, (long)J_THREADS)), dim3(J_THREADS), 0, st,
-                       part2, pl.ngroup, NDM, npairs, vj, ldvj, accumulate);
+    auto vj_reduce = halves ? jh_vj_reduce : j_vj_reduce;
+    hipLaunchKernelGGL(vj_reduce, dim3((unsigned)cdiv(nt, (long)J_THREADS)), dim3(J_THREADS), 0, st, part2,
+                       pl.ngroup, NDM, npairs, vj, ldvj, accumulate);
     MDF_LAUNCHED("j_vj_reduce");
     return MDF_OK;
 }
@@ -96,12 +136,12 @@ int run_j_group(const S *L, long ld, long naux, long npairs, const double *d, lo
 template <int VEC, typename S>
 int dispatch_j(int nd, const S *L, long ld, long naux, long npairs, const double *d, long ldd,
                double *rho, long ldrho, double *vj, long ldvj, int accumulate, double *work,
-               const JPlan &pl, hipStream_t st, int passes = 3, int acc_rho = 0) {
+               const JPlan &pl, hipStream_t st, int passes = 3, int acc_rho = 0, bool halves = false) {
     switch (nd) {
-    case 1: return run_j_group<1, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho);
-    case 2: return run_j_group<2, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho);
-    case 3: return run_j_group<3, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho);
-    default: return run_j_group<4, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho);
+    case 1: return run_j_group<1, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho, halves);
+    case 2: return run_j_group<2, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho, halves);
+    case 3: return run_j_group<3, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho, halves);
+    default: return run_j_group<4, VEC, S>(L, ld, naux, npairs, d, ldd, rho, ldrho, vj, ldvj, accumulate, work, pl, st, passes, acc_rho, halves);
     }
 }
 
@@ -128,6 +168,7 @@ template <typename S>
 static int get_j_typed(const S *cderi, int vec, long naux, long npairs, long ld, const double *dm_pairs, int n_dm,
                        long ldd, double *rho, long ldrho, double *vj_pairs, long ldvj, int accumulate, void *work,
                        hipStream_t st, int passes = 3, int acc_rho = 0) {
+    const bool halves = passes != 3;
     for (int n0 = 0; n0 < n_dm; n0 += 4) {
         const int nd = n_dm - n0 < 4 ? n_dm - n0 : 4;
         JPlan pl = plan_j(naux, npairs, nd, vec);
@@ -136,13 +177,13 @@ static int get_j_typed(const S *cderi, int vec, long naux, long npairs, long ld,
         double *r = rho + n0 * ldrho, *v = vj_pairs + n0 * ldvj;
         if (vec == 1)
             rc = dispatch_j<1, S>(nd, cderi, ld, naux, npairs, d, ldd, r, ldrho, v, ldvj, accumulate, (double *)work,
-                                  pl, st, passes, acc_rho);
+                                  pl, st, passes, acc_rho, halves);
         else if constexpr (sizeof(S) == 8)
             rc = dispatch_j<2, S>(nd, cderi, ld, naux, npairs, d, ldd, r, ldrho, v, ldvj, accumulate, (double *)work,
-                                  pl, st, passes, acc_rho);
+                                  pl, st, passes, acc_rho, halves);
         else
             rc = dispatch_j<4, S>(nd, cderi, ld, naux, npairs, d, ldd, r, ldrho, v, ldvj, accumulate, (double *)work,
-                                  pl, st, passes, acc_rho);
+                                  pl, st, passes, acc_rho, halves);
         if (rc) return rc;
     }
     return MDF_OK;
