@@ -212,8 +212,15 @@ def test_session_cholesky_eri_layout_and_values(fp32, monkeypatch):
     (cderi,), (pa, diag) = fastdfcderi._cholesky_eri(opt)
     assert cderi.is_cuda and cderi.stride(0) == 1 and cderi.stride(1) == cderi.shape[0]
     assert cderi.dtype == (torch.float32 if fp32 else torch.float64)
-    ref_c, (ref_pa, ref_diag) = fastdfcderi.cholesky_eri(mol, dtype='float32' if fp32 else 'float64')
-    assert torch.equal(cderi, ref_c)
+    dt = 'float32' if fp32 else 'float64'
+    # the same tensor as the pair-contiguous build from the same shell tables
+    same, _ = fastdfcderi.cholesky_eri(opt._mdf_molecules, dtype=dt)
+    assert torch.equal(cderi, same)
+    # and, to rounding of the coefficients, as the build from the package's
+    # own molecule (its normalisation of the basis data was done separately)
+    ref_c, (ref_pa, ref_diag) = fastdfcderi.cholesky_eri(mol, dtype=dt)
+    assert float((cderi.double() - ref_c.double()).abs().max() / ref_c.double().abs().max()) < \
+        (1e-6 if fp32 else 1e-12)
     assert np.array_equal(pa.cpu().numpy(), ref_pa) and np.array_equal(diag.cpu().numpy(), ref_diag)
 
     class SessionDF:
@@ -225,7 +232,7 @@ def test_session_cholesky_eri_layout_and_values(fp32, monkeypatch):
     c = np.linalg.qr(rng.standard_normal((mol.nao, nocc)))[0]
     dm = fastdf.tag_array(2 * c @ c.T, mo_coeff=c, mo_occ=np.full(nocc, 2.))
     vj, vk = fastdf.get_jk(SessionDF(), dm, hermi=1)
-    store = ref_c.double().cpu().numpy()
+    store = cderi.double().cpu().numpy()
     rj, rk = O.get_jk(store, ref_pa.astype(np.int64), np.asarray(dm))
     assert np.abs(vj - rj).max() / np.abs(rj).max() < 1e-12
     assert np.abs(vk - rk).max() / np.abs(rk).max() < 1e-12
