@@ -28,6 +28,7 @@ _SIGS = {
                             ctypes.c_char_p, I]),
     'mdf_get_j_workspace': (SZ, [L, L, I]),
     'mdf_get_j': (I, [P, I, L, L, L, P, I, L, P, L, P, L, I, P, SZ, P]),
+    'mdf_get_j_noalloc': (I, [P, I, L, L, L, P, I, L, P, L, P, L, I, P]),
     'mdf_get_k_workspace': (SZ, [L, I, I, I, I, L]),
     'mdf_get_k': (I, [P, I, L, L, L, P, I, P, P, I, L, I, P, L, I, P, SZ, P]),
     'mdf_pack_dm': (I, [P, I, I, L, P, L, I, P, L, P]),

@@ -273,3 +273,76 @@ extern "C" int mdf_j_vj(const void *B, int dtype, long naux, long width, long ld
                         void *stream) {
     return j_half(B, dtype, naux, width, ld, c, n_dm, ldc, vj, ldvj, accumulate, work, work_bytes, stream, 2);
 }
+// ---- J with no allocation: the partial sums in a file-scope device array ----
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops, harnesses that forbid hipMalloc).  The
+// partials of both passes (the only scratch mdf_get_j uses) live in
+// `scratch`, MDF_NOALLOC_SCRATCH_BYTES per device, held from module load to
+// process end.  Its device address is looked up once per device with that
+// device current (hipGetSymbolAddress) and only that address is handed to
+// kernels.  Every partial is written before it is read, so nothing needs
+// clearing between calls.  Calls that overlap on one device race on the
+// array.
+#include <mutex>
+
+static __device__ __align__(256) double scratch[MDF_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_of_current_device(void **out) {
+    int dev = 0;
+    MDF_HIP(hipGetDevice(&dev));
+    MDF_CHECK_ARG(dev >= 0 && dev < MAX_DEVICES, "mdf_get_j_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        MDF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        MDF_CHECK_ARG(p != nullptr, "mdf_get_j_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MDF_OK;
+}
+
+// bytes of partials mdf_get_j's plan needs for these sizes (its first,
+// largest group of at most four densities)
+size_t j_need_bytes(long naux, long npairs, int n_dm, int vec) {
+    const int nd = n_dm < 4 ? n_dm : 4;
+    JPlan p = plan_j(naux, npairs, nd, vec);
+    return (align_up(p.part1_words, 64) + p.part2_words) * sizeof(double);
+}
+}  // namespace
+
+extern "C" int mdf_get_j_noalloc(const void *cderi, int cderi_dtype, long naux, long npairs, long ld,
+                                 const double *dm_pairs, int n_dm, long ldd, double *rho, long ldrho,
+                                 double *vj_pairs, long ldvj, int accumulate, void *stream) {
+    MDF_CHECK_ARG(naux >= 0 && npairs >= 0 && n_dm >= 1, "mdf_get_j_noalloc: bad sizes naux=%ld npairs=%ld n_dm=%d",
+                  naux, npairs, n_dm);
+    MDF_CHECK_ARG(ld >= npairs && ldd >= npairs && ldvj >= npairs && ldrho >= naux,
+                  "mdf_get_j_noalloc: leading dimensions too small");
+    MDF_CHECK_ARG(cderi && dm_pairs && rho && vj_pairs, "mdf_get_j_noalloc: null pointer");
+    MDF_CHECK_ARG(cderi_dtype == MDF_DTYPE_F64 || cderi_dtype == MDF_DTYPE_F32,
+                  "mdf_get_j_noalloc: cderi dtype %d", cderi_dtype);
+    if (npairs == 0 || naux == 0)
+        return mdf_get_j(cderi, cderi_dtype, naux, npairs, ld, dm_pairs, n_dm, ldd, rho, ldrho, vj_pairs, ldvj,
+                         accumulate, nullptr, 0, stream);
+    const int vec = j_vec(cderi, ld, cderi_dtype);
+    const size_t need = j_need_bytes(naux, npairs, n_dm, vec);
+    if (need > (size_t)MDF_NOALLOC_SCRATCH_BYTES) {
+        set_error("mdf_get_j_noalloc: naux %ld x npairs %ld with %d densities needs %zu bytes of partials, "
+                  "above the %zu-byte scratch array; use mdf_get_j with a workspace",
+                  naux, npairs, n_dm, need, (size_t)MDF_NOALLOC_SCRATCH_BYTES);
+        return MDF_EINVAL;
+    }
+    void *work = nullptr;
+    if (int rc = scratch_of_current_device(&work)) return rc;
+    hipStream_t st = as_stream(stream);
+    if (cderi_dtype == MDF_DTYPE_F32)
+        return get_j_typed((const float *)cderi, vec, naux, npairs, ld, dm_pairs, n_dm, ldd, rho, ldrho, vj_pairs,
+                           ldvj, accumulate, work, st);
+    return get_j_typed((const double *)cderi, vec, naux, npairs, ld, dm_pairs, n_dm, ldd, rho, ldrho, vj_pairs, ldvj,
+                       accumulate, work, st);
+}

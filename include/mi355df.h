@@ -58,6 +58,19 @@ int mdf_get_j(const void *cderi, int cderi_dtype, long naux, long npairs, long l
               const double *dm_pairs, int n_dm, long ldd,
               double *rho, long ldrho, double *vj_pairs, long ldvj, int accumulate,
               void *work, size_t work_bytes, void *stream);
+/* mdf_get_j with no allocation and no workspace argument: the partial sums
+ * of both passes live in a file-scope device array of
+ * MDF_NOALLOC_SCRATCH_BYTES (256 MiB) per device, held from library load to
+ * process end.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Sizes whose partials exceed the
+ * array are rejected (MDF_EINVAL, message names the need); the (H2O)64
+ * tensor (naux 6976, 378,596 pairs) fits with any number of densities.
+ * Calls that overlap on one device race on the array. */
+#define MDF_NOALLOC_SCRATCH_BYTES (256L << 20)
+int mdf_get_j_noalloc(const void *cderi, int cderi_dtype, long naux, long npairs, long ld,
+                      const double *dm_pairs, int n_dm, long ldd, double *rho, long ldrho,
+                      double *vj_pairs, long ldvj, int accumulate, void *stream);
 
 /* The two halves of mdf_get_j for a three-centre block B (naux, width) that
  * is produced and consumed chunk by chunk -- the J build without a stored
