@@ -152,11 +152,16 @@ def test_library_error_message():
 def test_tile_fill_dense_configs():
     """The block-sparse tiled step 1 of the reference (taken only at AO tile
     fill <= 0.75) does not apply to the dense 1 g/cm^3 clusters: every
-    128 x 128 AO block holds kept shell pairs."""
+    128 x 128 AO block holds kept shell pairs, and more than 90 % of the
+    16 x 16 blocks the reference's gate counts."""
     from electrolites_amd import fastdfcderi
     from electrolites_amd.gto import water_cluster
     for n in (32, 64):
-        assert fastdfcderi.tile_fill(water_cluster(n, seed=0)) == 1.0
+        mol = water_cluster(n, seed=0)
+        assert fastdfcderi.tile_fill(mol) == 1.0
+        # at the reference's own 16-AO tile (electrolites/fastdf.py:216) the
+        # fill is still far above its 0.75 gate (0.977 at (H2O)64)
+        assert fastdfcderi.tile_fill(mol, tile=16) > .9
     # a sparse arrangement (two waters 60 A apart) is not dense
     from electrolites_amd.gto.mole import Mole
     far = Mole([('O', (0., 0., 0.)), ('H', (0.76, 0.59, 0.)), ('H', (-0.76, 0.59, 0.)),

@@ -152,19 +152,41 @@ def test_p_functions_are_center_derivatives():
             return O.int3c2e(sh, aux)[0, 1, 0] / (gto_norm(0, a) * y00)
         if kind == 'nuc':
             return O.int1e('nuc', sh, np.array([1.]), C[None])[0, 1] / (gto_norm(0, a) * y00)
-        return O.int1e('ovlp', sh)[0, 1] / (gto_norm(0, a) * y00)
+        return O.int1e(kind, sh)[0, 1] / (gto_norm(0, a) * y00)
 
     shp = one_shell([1, 0], [A, B], [a, b])
     np_ = gto_norm(1, a) * y1
-    for kind in ('ovlp', 'nuc', '3c'):
+    for kind in ('ovlp', 'kin', 'nuc', '3c'):
         if kind == '3c':
             px = O.int3c2e(shp, aux)[0, 3, 0]
         elif kind == 'nuc':
             px = O.int1e('nuc', shp, np.array([1.]), C[None])[0, 3]
         else:
-            px = O.int1e('ovlp', shp)[0, 3]
+            px = O.int1e(kind, shp)[0, 3]
         ref = _fd(lambda x: s_int(x, kind), A[0]) / (2 * a)
         assert math.isclose(px / np_, ref, rel_tol=2e-8), kind
+
+
+def test_kinetic_d_function_second_derivative():
+    """The kinetic integral's d_xy component is (1/4a^2) d^2/dA_x dA_y of the
+    s one (the operator does not depend on A): pins the kinetic machinery
+    for l = 2 beyond the closed-form s case."""
+    from electrolites_amd.gto.mole import gto_norm
+    a, b = .9, 1.3
+    A, B = np.array([.1, .2, -.3]), np.array([.5, -.4, .2])
+    y00 = 1 / math.sqrt(4 * math.pi)
+
+    def s_int(ax, ay):
+        sh = one_shell([0, 0], [[ax, ay, A[2]], B], [a, b])
+        return O.int1e('kin', sh)[0, 1] / (gto_norm(0, a) * y00)
+
+    h = 2e-4
+    d2 = (s_int(A[0] + h, A[1] + h) - s_int(A[0] + h, A[1] - h) -
+          s_int(A[0] - h, A[1] + h) + s_int(A[0] - h, A[1] - h)) / (4 * h * h)
+    shd = one_shell([2, 0], [A, B], [a, b])
+    dxy = O.int1e('kin', shd)[0, 5]             # m = -2 is d_xy
+    coef = O.cart2sph(2)[0, 1] * gto_norm(2, a)
+    assert math.isclose(dxy / coef, d2 / (4 * a * a), rel_tol=1e-6)
 
 
 def test_d_function_second_derivative():
