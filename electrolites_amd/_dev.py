@@ -10,11 +10,12 @@ import numpy as np
 
 _torch = None
 
-#: Held around every device dense-solver call of this package (eigh, SVD,
-#: the DIIS solve).  Several host threads issuing hipSOLVER ``syevd`` at once
-#: (a serving loop absorbing requests from a thread pool, one HIP stream
-#: each) crashed inside ``hipsolverDsyevd`` with a SIGSEGV; with the lock no
-#: two threads are ever inside the solver library together.
+#: Held around every call into the device dense-solver library (the solver
+#: thread's batched syevd, torch's SVD of a non-symmetric density).  Several
+#: host threads issuing hipSOLVER ``syevd`` at once (a serving loop absorbing
+#: requests from a thread pool, one HIP stream each) crashed inside
+#: ``hipsolverDsyevd`` with a SIGSEGV; with the lock no two threads are ever
+#: inside the solver library together.
 SOLVER_LOCK = threading.RLock()
 
 

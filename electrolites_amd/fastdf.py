@@ -36,6 +36,7 @@ import numpy as np
 from . import _dev
 from . import _lib
 from . import dfstore
+from . import eigsolve
 
 STATS = {'calls': 0, 'declined': 0, 'j_only': 0, 'mode0': 0, 'mode1': 0,
          'k_calls': 0}
@@ -344,8 +345,7 @@ def factorize_dm(dms, hermi):
     fls, frs = [], []
     for i in range(d.shape[0]):
         if hermi == 1:
-            with _dev.SOLVER_LOCK:
-                w, v = t.linalg.eigh(.5 * (d[i] + d[i].T))
+            w, v = eigsolve.eigh(.5 * (d[i] + d[i].T))
             keep = w.abs() > FACTOR_CUTOFF * max(float(w.abs().max()), 1e-300)
             w, v = w[keep], v[:, keep]
             s = w.abs().sqrt()

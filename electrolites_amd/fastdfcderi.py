@@ -30,6 +30,7 @@ import numpy as np
 
 from . import _dev
 from . import _lib
+from . import eigsolve
 from . import ints
 
 STATS = {'builds': 0, 'eigen_fallback': 0, 'chunks': 0, 'pairs_screened': 0,
@@ -107,8 +108,7 @@ def decompose_j2c(aux, timer=None):
         tm.mark('trtri:1')
         return linv, None
     STATS['eigen_fallback'] += 1
-    with _dev.SOLVER_LOCK:
-        w, v = t.linalg.eigh(j2c)
+    w, v = eigsolve.eigh(j2c)
     keep = w > LINEAR_DEP_THR
     coef = (v[:, keep] / t.sqrt(w[keep])).T.contiguous()
     return None, coef

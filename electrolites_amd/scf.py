@@ -31,6 +31,7 @@ import numpy as np
 from threadpoolctl import threadpool_limits
 
 from . import _dev
+from . import eigsolve
 from . import fastdf
 from . import ints
 from .df import DF
@@ -122,8 +123,7 @@ class RHF:
             mol._mdf_pair_bound = bound
         self.S, T, V = ints.int1e(ao, mol, bound)
         self.H = T + V
-        with _dev.SOLVER_LOCK:
-            w, v = t.linalg.eigh(self.S)
+        w, v = eigsolve.eigh(self.S)
         keep = w > 1e-9
         self.X = (v[:, keep] / t.sqrt(w[keep])).contiguous()
         self.enuc = mol.energy_nuc()
@@ -163,13 +163,21 @@ class RHF:
         self._bm[:n, k] = row
         if n < 2:
             return F
-        B = -t.ones((n + 1, n + 1), dtype=F.dtype, device=F.device)
+        # the (n+1) x (n+1) Pulay system on the host (LAPACK; a system this
+        # small runs on one thread): no solver-library call from the serving
+        # loop's worker threads
+        B = -np.ones((n + 1, n + 1))
         B[n, n] = 0.
-        B[:n, :n] = self._bm[:n, :n]
-        rhs = t.zeros(n + 1, dtype=F.dtype, device=F.device)
+        B[:n, :n] = self._bm[:n, :n].cpu().numpy()
+        rhs = np.zeros(n + 1)
         rhs[n] = -1.
-        with _dev.SOLVER_LOCK:
-            coef = t.linalg.solve_ex(B, rhs)[0][:n]       # no host check: no sync
+        try:
+            coef = np.linalg.solve(B, rhs)[:n]
+        except np.linalg.LinAlgError:
+            # linearly dependent history (error vectors that repeat near
+            # convergence): the least-squares solution
+            coef = np.linalg.lstsq(B, rhs, rcond=None)[0][:n]
+        coef = t.from_numpy(coef).to(F.device)
         return (coef.view(-1, 1, 1) * self._fh[:n]).sum(dim=0)
 
     def _diag(self, F):
@@ -185,11 +193,10 @@ class RHF:
             e = t.from_numpy(e).to(A.device)
             c = t.from_numpy(c).to(A.device)
         else:
-            # one thread at a time inside the solver library (a serving loop
-            # absorbs requests from a thread pool; concurrent hipSOLVER syevd
-            # calls from several threads crashed the process)
-            with _dev.SOLVER_LOCK:
-                e, c = t.linalg.eigh(A)
+            # through the solver thread (eigsolve): concurrent hipSOLVER syevd
+            # calls from a serving loop's worker threads crashed the process;
+            # there, the requests diagonalising at once share one batched call
+            e, c = eigsolve.eigh(A)
         return e, self.X @ c
 
     # ------------------------------------------------------------- loop

@@ -3,42 +3,44 @@
 A stream of DF-RHF energy requests (molecules of uneven size) is served by a
 loop that keeps every admitted request's state in HBM -- its cderi, one-
 electron matrices, density factors, DIIS history and the device buffers of
-its J/K item -- and advances all of them one SCF cycle per tick:
+its J/K item -- and advances each of them one SCF cycle at a time:
 
 1. admission: queued requests are admitted in arrival order while the
-   resident bytes (cderi + per-request matrices, :func:`footprint`) stay
-   within ``hbm_budget`` and the batch has a free slot; admitting builds the
-   request's cderi on the GPU (:mod:`electrolites_amd.fastdfcderi`);
-2. one grouped J/K run per precision group (:class:`batch.GroupedJK`): every
-   active request's current density goes into one persistent launch per
-   phase, so a request that arrived while others were mid-solve shares their
-   launches from its first cycle on;
-3. each request absorbs its J/K (:meth:`scf.RHF.absorb`: Fock, energy, DIIS,
-   diagonalisation, next density, precision switch); finished requests leave
-   and free their bytes for the queue.
+   resident bytes (the stored cderi + per-request matrices,
+   :func:`footprint`) stay within ``hbm_budget`` and the batch has a free
+   slot; an admitted request's cderi and one-electron matrices are built by
+   a builder thread on a stream of its own while the loop goes on, and the
+   request joins at the first step after its build finished;
+2. J/K: every request whose next density is ready goes into one grouped
+   launch per precision group (:class:`batch.GroupedJK`), so requests that
+   arrived while others were mid-solve share launches with them;
+3. absorb (:meth:`scf.RHF.absorb`: Fock, energy, DIIS, diagonalisation, next
+   density, precision switch) on a worker thread and the request's own HIP
+   stream (default priority), right after its J/K was launched; finished
+   requests leave and free their bytes for the queue.
 
-Step 3 is latency-bound (small matrices, one FP64 diagonalisation per
-request), so the requests absorb concurrently: each has its own HIP stream
-(default priority) and a worker thread of a pool issues its operations
-there; the torch operators and the library calls release the interpreter
-lock while they run, so the diagonalisations of different requests overlap.
-Orthogonal dimensions up to ``host_diag_max`` are diagonalised by LAPACK on
-the host (single-threaded, serialised: :func:`scf.host_eigh`), larger ones
-on the GPU.  ``pipeline=True`` splits the requests into two groups whose
-J/K and absorb phases alternate (measured no faster here: the loop is
-GPU-bound).  Each stream waits on
-an event recorded after the grouped J/K launches, and the next tick's
-launches wait on events recorded after every request's absorb.
+Schedules.  ``'async'`` (the default): no request waits for another; a step
+launches the J/K of every request whose absorb finished since the last step,
+so while the host diagonalises one request's Fock matrix the GPU runs the
+others' J/K.  ``'lockstep'``: every active request gets its J/K in one launch,
+then all absorb, then the next tick (``pipeline=True`` alternates two
+groups).  Measured on the 48-request mix at 8 req/s: async 19-23 thousand
+solves/hr against 13-15 thousand lockstep.
 
-Admission is asynchronous: an admitted request's cderi and one-electron
-matrices are built by a builder thread on a stream of its own while the
-ticks go on, and the request joins the batch at the first tick after its
-build finished.
+Diagonalisation.  Orthogonal dimensions up to ``host_diag_max`` (default
+2000, every molecule of the mix) go to LAPACK on the host, one thread per
+call, many calls side by side (:func:`scf.host_eigh`); larger ones to the
+device eigensolver thread (:mod:`electrolites_amd.eigsolve`), which batches
+the requests waiting at once.  Several worker threads calling hipSOLVER
+``syevd`` at the same time crashed the process (SIGSEGV inside
+``hipsolverDsyevd``); no worker thread calls a device solver library now.
+A device ``syevd`` is a chain of small kernels (about 18 ms at 816 x 816
+alone), so on this workload the host calls, running in parallel, are faster.
 
 Each request runs exactly the cycle sequence of a stand-alone
-:class:`scf.RHF` with the same precision schedule, and its grouped J/K item
-is cut exactly as the single-item entries cut it, so its energy is the same
-alone and in any mix (bit-identical; the tests compare to 1e-12 Eh).
+:class:`scf.RHF` with the same options, and its grouped J/K item is cut
+exactly as the single-item entries cut it, so its energy is the same alone
+and in any mix (bit-identical; the tests compare to 1e-12 Eh).
 
 The reference has no serving loop (one ``get_jk`` per call,
 electrolites/fastdf.py:805-1026); this is the caller-side batching built on
@@ -142,6 +144,7 @@ class _Slot:
         self.stream = t.cuda.Stream()
         self.jk_done = None           # event after this slot's launched J/K
         self.ready = None             # event after this slot's last absorb
+        self.future = None            # its running absorb (asynchronous schedule)
 
     def load_density(self):
         """Density factor and packed density of the current ``scf.dm``, the
@@ -169,17 +172,25 @@ class Server:
     ``serve`` drives a timed arrival stream to completion."""
 
     def __init__(self, hbm_budget=HBM_BUDGET, max_batch=64, conv_tol=1e-10, max_cycle=100,
-                 graphs=False, workers=16, host_diag_max=400, async_build=True,
-                 low_precision='bf16x3', pipeline=False):
+                 graphs=False, workers=16, host_diag_max=2000, async_build=True,
+                 low_precision='bf16x3', pipeline=False, schedule='async'):
+        if schedule not in ('async', 'lockstep'):
+            raise ValueError(f'schedule {schedule!r}: expected async or lockstep')
         self.hbm_budget = float(hbm_budget)
-        self.pipeline = pipeline
+        self.schedule = schedule
+        self.pipeline = pipeline and schedule == 'lockstep'
+        if schedule == 'async':
+            # an absorb holds its worker while it waits for its J/K on the
+            # device, so every admitted request may need one
+            workers = max(workers, max_batch)
         self._plan_tick = {}
         self._weight = [0., 0.]
         self._work = _dev.Workspace(grow=2.0)
         self.workers = workers
         self.host_diag_max = host_diag_max
         self.low_precision = low_precision
-        self._pool = concurrent.futures.ThreadPoolExecutor(workers) if workers > 1 else None
+        self._pool = concurrent.futures.ThreadPoolExecutor(workers) \
+            if workers > 1 or schedule == 'async' else None
         self._builder = concurrent.futures.ThreadPoolExecutor(1) if async_build else None
         self._building = []            # (future, request) of admitted requests being built
         self.timing = collections.Counter()
@@ -323,6 +334,74 @@ class Server:
         return [s for s, d in zip(slots, done) if d]
 
     def step(self):
+        """Admit, then advance the batch: :meth:`_step_async` (the default
+        schedule) or one lockstep tick (:meth:`_step_lockstep`)."""
+        if self.schedule == 'async':
+            return self._step_async()
+        return self._step_lockstep()
+
+    def _absorb_one(self, s):
+        """A worker thread's part of one cycle of one request: wait (on the
+        request's stream) for its J/K, absorb it, record the event the next
+        launch waits on; True when the request finished."""
+        t = _dev.torch()
+        with single_thread_blas():
+            with t.cuda.stream(s.stream):
+                s.stream.wait_event(s.jk_done)
+                finished = s.absorb()
+                ev = t.cuda.Event()
+                ev.record(s.stream)
+        s.ready = ev
+        return finished
+
+    def _step_async(self):
+        """Asynchronous cycles: every request whose absorb has finished since
+        the last call (its next density is being written on its stream) joins
+        one grouped J/K launch per precision group at once, and its absorb is
+        handed to a worker right after the launch, where it waits for that
+        J/K on the device.  No request waits for another's diagonalisation:
+        while the host diagonalises one request's Fock matrix, the GPU runs
+        the others' J/K.  Each request still runs its stand-alone cycle
+        sequence, so its result is the same alone and in the mix."""
+        t0 = time.perf_counter()
+        self._admit()
+        t1 = time.perf_counter()
+        self.timing['admit_s'] += t1 - t0
+        if not self.active:
+            return 0
+        finished = []
+        for s in self.active:
+            f = s.future
+            if f is not None and f.done():
+                s.future = None
+                if f.result():
+                    finished.append(s)
+        for s in finished:
+            s.stream.synchronize()
+            self._retire(s)
+            self._weight[s.group] -= s.weight
+        if finished:
+            gone = set(map(id, finished))
+            self.active = [s for s in self.active if id(s) not in gone]
+        ready = [s for s in self.active if s.future is None]
+        if ready:
+            self.ticks += 1
+            ev = self._launch(ready)
+            for s in ready:
+                s.jk_done = ev
+                s.future = self._pool.submit(self._absorb_one, s)
+            self.timing['launch_s'] += time.perf_counter() - t1
+            self.trace.append((self.now(), len(self.active), len(self.queue), self.resident))
+        elif not finished:
+            tw = time.perf_counter()
+            pending = [s.future for s in self.active if s.future is not None]
+            if pending:
+                concurrent.futures.wait(pending, timeout=.005,
+                                        return_when=concurrent.futures.FIRST_COMPLETED)
+            self.timing['wait_s'] += time.perf_counter() - tw
+        return len(self.active)
+
+    def _step_lockstep(self):
         """Admit, then one tick.  Lockstep (``pipeline=False``): every active
         request gets its J/K, then absorbs it.  Pipelined: the requests are
         in two groups; a tick launches one group's J/K and, while the GPU

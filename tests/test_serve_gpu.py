@@ -37,13 +37,15 @@ def test_footprint_and_streams():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('schedule', ['async', 'lockstep'])
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
-def test_alone_equals_mix(precision):
+def test_alone_equals_mix(precision, schedule):
     from electrolites_amd import serve
     from electrolites_amd.scf import RHF
     mols = _mols()
-    # in the mix: a new request joins every other tick while earlier ones run
-    srv = serve.Server(conv_tol=1e-10)
+    # in the mix: a new request joins every other J/K launch while earlier
+    # ones run (built at admission, so it is active at once)
+    srv = serve.Server(conv_tol=1e-10, schedule=schedule, async_build=False)
     reqs = [serve.Request(m, rid=i, k_precision=precision) for i, m in enumerate(mols)]
     pending = list(reqs)
     peak = 0
@@ -58,7 +60,7 @@ def test_alone_equals_mix(precision):
     assert peak >= 3, 'requests did not overlap'
     for r in reqs:
         assert r.converged
-        alone = serve.Server(conv_tol=1e-10)
+        alone = serve.Server(conv_tol=1e-10, schedule=schedule)
         a = serve.Request(r.mol, k_precision=precision)
         alone.serve([a])
         # the stand-alone driver with the server's diagonalisation choice
