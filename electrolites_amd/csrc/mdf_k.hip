@@ -47,24 +47,26 @@ __global__ __launch_bounds__(Step1Shape<BN>::THREADS) void k_step1(
                               lds);
 }
 
+// 1-D grid of ntiles_grid * ksplit workgroups, (tile, split) by step2_coords.
 template <bool SYM, bool VEC, typename T>
 __global__ __launch_bounds__(G_THREADS, 2) void k_step2(
     const T *__restrict__ Rl, const T *__restrict__ Rr, int nao, int nocc,
-    long ldr, long strideL, int nL, int ntile, double *__restrict__ part, long part_stride,
-    int beta) {
+    long ldr, long strideL, int nL, int ntile, int ntiles_grid, int ksplit, double *__restrict__ part,
+    long part_stride, int beta) {
     __shared__ T lds[TileShape<S2_B, S2_B>::LDS_ELEMS];
-    int ti, tj;
+    int tile, split, ti, tj;
+    step2_coords(blockIdx.x, ntiles_grid, tile, split);
     if (SYM) {
-        tri_tile(blockIdx.x, ti, tj);
+        tri_tile(tile, ti, tj);
     } else {
-        ti = blockIdx.x / ntile;
-        tj = blockIdx.x % ntile;
+        ti = tile / ntile;
+        tj = tile % ntile;
     }
     const int kchunks = (nocc + G_BK - 1) / G_BK;
     const long nsteps = (long)nL * kchunks;
-    const long s0 = nsteps * blockIdx.y / gridDim.y, s1 = nsteps * (blockIdx.y + 1) / gridDim.y;
+    const long s0 = nsteps * split / ksplit, s1 = nsteps * (split + 1) / ksplit;
     step2_tile<VEC, T>(Rl, Rr, nao, nocc, ldr, strideL, ti * S2_B, tj * S2_B, s0, s1,
-                       part + (long)blockIdx.y * part_stride, beta, lds);
+                       part + (long)split * part_stride, beta, lds);
 }
 
 __global__ __launch_bounds__(256) void k_finalize(
@@ -114,19 +116,21 @@ void launch_step1_b3(int nL, hipStream_t st, const Src *blk, long ld, const int 
 template <bool SYM>
 __global__ __launch_bounds__(G_THREADS, 2) void k_step2_b3(
     const b3::u16 *lhi, const b3::u16 *llo, const b3::u16 *rhi, const b3::u16 *rlo, int nao, long ldr,
-    long strideL, int nL, int ntile, double *__restrict__ part, long part_stride, int beta) {
+    long strideL, int nL, int ntile, int ntiles_grid, int ksplit, double *__restrict__ part, long part_stride,
+    int beta) {
     __shared__ b3::u16 lds[b3::Shape<S2_B, S2_B>::LDS_ELEMS];
-    int ti, tj;
+    int tile, split, ti, tj;
+    step2_coords(blockIdx.x, ntiles_grid, tile, split);
     if (SYM) {
-        tri_tile(blockIdx.x, ti, tj);
+        tri_tile(tile, ti, tj);
     } else {
-        ti = blockIdx.x / ntile;
-        tj = blockIdx.x % ntile;
+        ti = tile / ntile;
+        tj = tile % ntile;
     }
     const long nsteps = (long)nL * (ldr / b3::BK);
-    const long s0 = nsteps * blockIdx.y / gridDim.y, s1 = nsteps * (blockIdx.y + 1) / gridDim.y;
+    const long s0 = nsteps * split / ksplit, s1 = nsteps * (split + 1) / ksplit;
     kb3::step2_tile(lhi, llo, rhi, rlo, nao, ldr, strideL, ti * S2_B, tj * S2_B, s0, s1,
-                    part + (long)blockIdx.y * part_stride, beta, lds);
+                    part + (long)split * part_stride, beta, lds);
 }
 
 template <int BN, typename T, typename Src>
@@ -161,20 +165,20 @@ int step1(const KPlan &p, int nL, hipStream_t st, const Src *blk, long ld, const
 template <typename T>
 int step2(const KPlan &p, int nL, bool two, hipStream_t st, const T *rl, const T *rr, int nao, int nocc,
           long ldr, long strideL, double *part, long part_stride, int beta) {
-    dim3 g2((unsigned)p.ntiles_grid, (unsigned)p.ksplit);
+    const dim3 g2((unsigned)((long)p.ntiles_grid * p.ksplit));
     const bool vec = (ldr % (16 / (long)sizeof(T)) == 0) && ((uintptr_t)rl % 16 == 0) && ((uintptr_t)rr % 16 == 0);
     if (two && vec)
         hipLaunchKernelGGL((k_step2<false, true, T>), g2, dim3(G_THREADS), 0, st, rl, rr, nao, nocc, ldr,
-                           strideL, nL, p.ntile, part, part_stride, beta);
+                           strideL, nL, p.ntile, p.ntiles_grid, p.ksplit, part, part_stride, beta);
     else if (two)
         hipLaunchKernelGGL((k_step2<false, false, T>), g2, dim3(G_THREADS), 0, st, rl, rr, nao, nocc, ldr,
-                           strideL, nL, p.ntile, part, part_stride, beta);
+                           strideL, nL, p.ntile, p.ntiles_grid, p.ksplit, part, part_stride, beta);
     else if (vec)
         hipLaunchKernelGGL((k_step2<true, true, T>), g2, dim3(G_THREADS), 0, st, rl, rl, nao, nocc, ldr,
-                           strideL, nL, p.ntile, part, part_stride, beta);
+                           strideL, nL, p.ntile, p.ntiles_grid, p.ksplit, part, part_stride, beta);
     else
         hipLaunchKernelGGL((k_step2<true, false, T>), g2, dim3(G_THREADS), 0, st, rl, rl, nao, nocc, ldr,
-                           strideL, nL, p.ntile, part, part_stride, beta);
+                           strideL, nL, p.ntile, p.ntiles_grid, p.ksplit, part, part_stride, beta);
     MDF_LAUNCHED("k_step2");
     return MDF_OK;
 }
@@ -251,14 +255,14 @@ int run_k_b3(const KPlan &p, const Src *cderi, long naux, long ld, const int *pa
             }
             MDF_LAUNCHED("k_step1_b3");
         }
-        dim3 g2((unsigned)p.ntiles_grid, (unsigned)p.ksplit);
+        const dim3 g2((unsigned)((long)p.ntiles_grid * p.ksplit));
         const kb3::Side &a = sd[0], &b = sd[two ? 1 : 0];
         if (two)
             hipLaunchKernelGGL(k_step2_b3<false>, g2, dim3(G_THREADS), 0, st, a.rhi, a.rlo, b.rhi, b.rlo, nao, y.ldr,
-                               strideL, nL, p.ntile, part, part_stride, (int)(L0 > 0));
+                               strideL, nL, p.ntile, p.ntiles_grid, p.ksplit, part, part_stride, (int)(L0 > 0));
         else
             hipLaunchKernelGGL(k_step2_b3<true>, g2, dim3(G_THREADS), 0, st, a.rhi, a.rlo, b.rhi, b.rlo, nao, y.ldr,
-                               strideL, nL, p.ntile, part, part_stride, (int)(L0 > 0));
+                               strideL, nL, p.ntile, p.ntiles_grid, p.ksplit, part, part_stride, (int)(L0 > 0));
         MDF_LAUNCHED("k_step2_b3");
     }
     long nt = (long)nao * nao;

@@ -282,19 +282,36 @@ inline int resident_step2_wgs() {
     return cached[dev];
 }
 
-// Split of the (L, k) reduction: the ksplit in 1..32 whose grid fills whole
-// rounds of resident workgroups best (ties: the larger split).
+// Split of the (L, k) reduction: a multiple of the XCD count, so that each
+// XCD takes whole splits (step2_coords) and the output tiles running on one
+// XCD at a time read the same R panels through its L2; among 8, 16, 24, 32
+// the one whose per-XCD share fills whole rounds of resident workgroups
+// best (ties: the larger split).
 inline int pick_ksplit(int ntiles_grid) {
-    const int res = resident_step2_wgs();
-    int best = 1;
+    const int slots = resident_step2_wgs() / N_XCD;
+    int best = N_XCD;
     double best_eff = -1.;
-    for (int ks = 1; ks <= 32; ks++) {
-        const long wgs = (long)ntiles_grid * ks;
-        const long rounds = (wgs + res - 1) / res;
-        const double eff = (double)wgs / (double)(rounds * res);
+    for (int ks = N_XCD; ks <= 4 * N_XCD; ks += N_XCD) {
+        const long items = (long)ntiles_grid * ks / N_XCD;
+        const long rounds = (items + slots - 1) / slots;
+        const double eff = (double)items / (double)(rounds * slots);
         if (eff >= best_eff - 1e-9) { best_eff = eff; best = ks; }
     }
     return best;
+}
+
+// (output tile, split) of workgroup w of a 1-D step-2 grid of ntiles_grid *
+// ksplit workgroups (ksplit a multiple of N_XCD).  Workgroups reach the XCDs
+// round-robin (w % N_XCD); XCD x takes splits x, x + N_XCD, ..., every
+// output tile of one split before the next, so the tiles resident on an XCD
+// together mostly share one split's K range: each R panel of that range is
+// fetched into the XCD's L2 once and read there by every tile of its row
+// and column.
+__device__ __forceinline__ void step2_coords(long w, int ntiles_grid, int &tile, int &split) {
+    const int xcd = (int)(w % N_XCD);
+    const long l = w / N_XCD;
+    split = xcd + N_XCD * (int)(l / ntiles_grid);
+    tile = (int)(l % ntiles_grid);
 }
 
 // Aux rows per pass: the transformed blocks of one pass (R_l and R_r) stay
