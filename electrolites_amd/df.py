@@ -31,6 +31,9 @@ class DF:
         return None if self._cderi is None else next(iter(self._cderi.values())).shape[0]
 
     def build(self, omega=None):
+        # tables of a previous tensor go with it (they are also keyed on it)
+        for name in ('_mdf_layout', '_mdf_auxc', '_mdf_grad_metric', '_mdf_grad_pairs'):
+            self.__dict__.pop(name, None)
         cderi, idx = fastdfcderi._cholesky_eri(self, omega=omega)
         dev = _dev.torch().cuda.current_device()
         self._cderi = {dev: cderi[0]}
@@ -44,9 +47,8 @@ class DF:
         self.__dict__.pop('_mdf_resident', None)
         self._cderi = None
         self._cderi_idx = None
-        self.__dict__.pop('_mdf_layout', None)
-        self.__dict__.pop('_mdf_grad_metric', None)
-        self.__dict__.pop('_mdf_grad_pairs', None)
+        for name in ('_mdf_layout', '_mdf_auxc', '_mdf_grad_metric', '_mdf_grad_pairs'):
+            self.__dict__.pop(name, None)
         return self
 
     def get_jk(self, dm, hermi=1, with_j=True, with_k=True, omega=None):

@@ -95,13 +95,15 @@ def _class_sorted_pairs(mol, addr):
 
 
 def _cache_key(dfobj, mol):
-    """What the cached per-cderi tables depend on: the tensor, its pair list
+    """What the cached per-cderi tables depend on: the stored tensor (its
+    identity and memory, held weakly: ``fastdf._cache_key``), its pair list
     and the basis (positions, exponents, coefficients of both sets)."""
     cderi = fastdf._cderi_of(dfobj)
     idx = getattr(dfobj, '_cderi_idx', None)
     basis = b''.join(np.ascontiguousarray(a).tobytes() for sh in (mol.ao, mol.aux)
                      for a in (sh.shl_xyz, sh.exps, sh.coefs))
-    return (id(cderi), tuple(cderi.shape), None if idx is None else id(idx[0]), basis)
+    x = _dev.torch().from_numpy(cderi) if isinstance(cderi, np.ndarray) else cderi
+    return fastdf._cache_key(dfobj, x, None if idx is None else id(idx[0]), basis)
 
 
 def _metric_factor(dfobj, mol, aux):
@@ -112,16 +114,12 @@ def _metric_factor(dfobj, mol, aux):
     electrolites/fastdfgrad.py:665-714)."""
     t = _dev.torch()
     key = _cache_key(dfobj, mol)
-    cached = getattr(dfobj, '_mdf_grad_metric', None)
-    if cached is not None and cached[0] == key:
-        return cached[1], cached[2]
+    cached = fastdf._cache_get(dfobj, '_mdf_grad_metric', key)
+    if cached is not None:
+        return cached
     linv, coef = fastdfcderi.decompose_j2c(aux)
     A, lower = (t.tril(linv), True) if coef is None else (coef, False)
-    try:
-        dfobj._mdf_grad_metric = (key, A, lower)
-    except AttributeError:
-        pass
-    return A, lower
+    return fastdf._cache_put(dfobj, '_mdf_grad_metric', key, (A, lower))
 
 
 def _pair_tables(dfobj, mol):
@@ -131,9 +129,9 @@ def _pair_tables(dfobj, mol):
     DF object like the metric factor."""
     t = _dev.torch()
     key = _cache_key(dfobj, mol)
-    cached = getattr(dfobj, '_mdf_grad_pairs', None)
-    if cached is not None and cached[0] == key:
-        return cached[1]
+    cached = fastdf._cache_get(dfobj, '_mdf_grad_pairs', key)
+    if cached is not None:
+        return cached
     addr = _dev.to_host(dfobj._cderi_idx[0])
     pairs, class_off, cmap = _class_sorted_pairs(mol, addr)
     i32 = lambda a: _dev.to_device(np.ascontiguousarray(a, dtype=np.int32), dtype=t.int32)
@@ -144,10 +142,7 @@ def _pair_tables(dfobj, mol):
         'tri': _lower_blocks(addr, mol.nao),
         'ao_atom': i32(mol.ao.shl_atom), 'aux_atom': i32(mol.aux.shl_atom),
     }
-    try:
-        dfobj._mdf_grad_pairs = (key, tabs)
-    except AttributeError:
-        pass
+    fastdf._cache_put(dfobj, '_mdf_grad_pairs', key, tabs)
     return tabs
 
 
