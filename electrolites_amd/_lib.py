@@ -48,6 +48,7 @@ _SIGS = {
     'mdf_int2c2e_ip': (I, [P, P, P, P, L, P, I, P, P]),
     'mdf_int1e': (I, [P, I, P, P, I, P, P, P, P, P, P]),
     'mdf_gemm': (I, [I, I, I, L, L, L, D, P, L, P, L, D, P, L, P]),
+    'mdf_gemm_batched': (I, [I, I, I, L, L, L, D, P, L, L, P, L, L, D, P, L, L, L, P]),
     'mdf_potrf_workspace': (SZ, [L]),
     'mdf_potrf': (I, [P, L, L, P, P, SZ, P]),
     'mdf_trtri_workspace': (SZ, [L]),

@@ -25,12 +25,18 @@ namespace {
 
 constexpr int LB = 128;   // panel / tile width
 
-template <int BN, bool BT, bool TRI_A, bool LOWER_C>
+// TRI: 0 = A general; 1 = A lower triangular (row block m reads k < m0 + 128);
+// 2 = A upper triangular (row block m reads k >= m0).  blockIdx.z is the
+// batch index (operand strides sA, sB, sC).
+template <int BN, bool BT, int TRI, bool LOWER_C>
 __global__ __launch_bounds__(G_THREADS, 2) void gemm_f64_kernel(
-    int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
-    const double *__restrict__ B, long ldb, double beta, double *C, long ldc) {
+    int M, int N, int K, double alpha, const double *__restrict__ A, long lda, long sA,
+    const double *__restrict__ B, long ldb, long sB, double beta, double *C, long ldc, long sC) {
     using S = TileShape<128, BN>;
     __shared__ double lds[S::LDS_DOUBLES];
+    A += (long)blockIdx.z * sA;
+    B += (long)blockIdx.z * sB;
+    C += (long)blockIdx.z * sC;
     int ti, tj;
     if (LOWER_C) {
         const int x = blockIdx.x;
@@ -43,16 +49,17 @@ __global__ __launch_bounds__(G_THREADS, 2) void gemm_f64_kernel(
         tj = blockIdx.x;
     }
     const int m0 = ti * 128, n0 = tj * BN;
-    const int kmax = TRI_A ? min(K, m0 + 128) : K;
-    RowsLoader<128> la{A, lda, m0, M, 0, kmax};
+    const int k0 = TRI == 2 ? min(K, m0) : 0;
+    const int kmax = TRI == 1 ? min(K, m0 + 128) : K;
+    RowsLoader<128> la{A, lda, m0, M, k0, kmax};
     Acc<S::MI, S::NI> acc;
     acc.zero();
-    const int nk = (kmax + G_BK - 1) / G_BK;
+    const int nk = (kmax - k0 + G_BK - 1) / G_BK;
     if (BT) {
-        RowsLoader<BN> lb{B, ldb, n0, N, 0, kmax};
+        RowsLoader<BN> lb{B, ldb, n0, N, k0, kmax};
         gemm_mainloop<128, BN>(la, lb, nk, lds, acc);
     } else {
-        ColsLoader<BN> lb{B, ldb, n0, N, 0, kmax};
+        ColsLoader<BN> lb{B, ldb, n0, N, k0, kmax};
         gemm_mainloop<128, BN>(la, lb, nk, lds, acc);
     }
     gemm_epilogue<128, BN>(acc, [&](int r, int c, double v) {
@@ -64,21 +71,36 @@ __global__ __launch_bounds__(G_THREADS, 2) void gemm_f64_kernel(
     });
 }
 
-template <int BN, bool BT, bool TRI_A, bool LOWER_C>
-int launch_gemm(int M, int N, int K, double alpha, const double *A, long lda, const double *B, long ldb,
-                double beta, double *C, long ldc, hipStream_t st) {
-    if (M <= 0 || N <= 0) return MDF_OK;
-    dim3 g;
-    if (LOWER_C) {
-        const int t = (M + 127) / 128;
-        g = dim3((unsigned)(t * (t + 1) / 2));
-    } else {
-        g = dim3((unsigned)((N + BN - 1) / BN), (unsigned)((M + 127) / 128));
+constexpr long MAX_GRID_Z = 65535;
+
+template <int BN, bool BT, int TRI, bool LOWER_C>
+int launch_gemm(int M, int N, int K, double alpha, const double *A, long lda, long sA, const double *B, long ldb,
+                long sB, double beta, double *C, long ldc, long sC, long batch, hipStream_t st) {
+    if (M <= 0 || N <= 0 || batch <= 0) return MDF_OK;
+    for (long b0 = 0; b0 < batch; b0 += MAX_GRID_Z) {
+        const unsigned nz = (unsigned)(batch - b0 < MAX_GRID_Z ? batch - b0 : MAX_GRID_Z);
+        dim3 g;
+        if (LOWER_C) {
+            const int t = (M + 127) / 128;
+            g = dim3((unsigned)(t * (t + 1) / 2), 1, nz);
+        } else {
+            g = dim3((unsigned)((N + BN - 1) / BN), (unsigned)((M + 127) / 128), nz);
+        }
+        hipLaunchKernelGGL((gemm_f64_kernel<BN, BT, TRI, LOWER_C>), g, dim3(G_THREADS), 0, st, M, N, K, alpha,
+                           A + b0 * sA, lda, sA, B + b0 * sB, ldb, sB, beta, C + b0 * sC, ldc, sC);
+        MDF_LAUNCHED("gemm_f64_kernel");
     }
-    hipLaunchKernelGGL((gemm_f64_kernel<BN, BT, TRI_A, LOWER_C>), g, dim3(G_THREADS), 0, st, M, N, K, alpha,
-                       A, lda, B, ldb, beta, C, ldc);
-    MDF_LAUNCHED("gemm_f64_kernel");
     return MDF_OK;
+}
+
+template <bool BT, bool LOWER_C>
+int gemm_tri(int tri, int m, int n, int k, double alpha, const double *A, long lda, long sA, const double *B,
+             long ldb, long sB, double beta, double *C, long ldc, long sC, long batch, hipStream_t st) {
+    if (tri == 1)
+        return launch_gemm<128, BT, 1, LOWER_C>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st);
+    if (tri == 2)
+        return launch_gemm<128, BT, 2, LOWER_C>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st);
+    return launch_gemm<128, BT, 0, LOWER_C>(m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, st);
 }
 
 // Cholesky of one diagonal block (n <= 128, lower, in place) by one
@@ -152,26 +174,34 @@ __global__ __launch_bounds__(DIAG_THREADS) void potrf_diag_kernel(double *A, lon
 
 }  // namespace
 
+extern "C" int mdf_gemm_batched(int transb, int tri_a, int lower_c, long M, long N, long K, double alpha,
+                                const double *A, long lda, long strideA, const double *B, long ldb, long strideB,
+                                double beta, double *C, long ldc, long strideC, long batch, void *stream) {
+    MDF_CHECK_ARG(M >= 0 && N >= 0 && K >= 0 && M < (1L << 31) && N < (1L << 31) && K < (1L << 31) && batch >= 0,
+                  "mdf_gemm: bad sizes");
+    MDF_CHECK_ARG((A && B && C) || batch == 0, "mdf_gemm: null pointer");
+    MDF_CHECK_ARG(!lower_c || M == N, "mdf_gemm: lower_c needs a square C");
+    MDF_CHECK_ARG(tri_a >= 0 && tri_a <= 2 && (tri_a == 0 || M == K), "mdf_gemm: tri_a %d needs a square A", tri_a);
+    MDF_CHECK_ARG(lda >= K && ldc >= N && ldb >= (transb ? K : N), "mdf_gemm: leading dimension too small");
+    MDF_CHECK_ARG(strideA >= 0 && strideB >= 0 && strideC >= 0, "mdf_gemm: negative batch stride");
+    hipStream_t st = as_stream(stream);
+    const int m = (int)M, n = (int)N, k = (int)K;
+    if (lower_c)
+        return transb ? gemm_tri<true, true>(tri_a, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                                             strideC, batch, st)
+                      : gemm_tri<false, true>(tri_a, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                                              strideC, batch, st);
+    return transb ? gemm_tri<true, false>(tri_a, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                                          strideC, batch, st)
+                  : gemm_tri<false, false>(tri_a, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                                           strideC, batch, st);
+}
+
 extern "C" int mdf_gemm(int transb, int tri_a, int lower_c, long M, long N, long K, double alpha,
                         const double *A, long lda, const double *B, long ldb, double beta, double *C, long ldc,
                         void *stream) {
-    MDF_CHECK_ARG(M >= 0 && N >= 0 && K >= 0 && M < (1L << 31) && N < (1L << 31) && K < (1L << 31),
-                  "mdf_gemm: bad sizes");
-    MDF_CHECK_ARG(A && B && C, "mdf_gemm: null pointer");
-    MDF_CHECK_ARG(!lower_c || M == N, "mdf_gemm: lower_c needs a square C");
-    MDF_CHECK_ARG(lda >= K && ldc >= N && ldb >= (transb ? K : N), "mdf_gemm: leading dimension too small");
-    hipStream_t st = as_stream(stream);
-    const int m = (int)M, n = (int)N, k = (int)K;
-    if (lower_c) {
-        if (transb) return tri_a ? launch_gemm<128, true, true, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st)
-                                 : launch_gemm<128, true, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
-        return tri_a ? launch_gemm<128, false, true, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st)
-                     : launch_gemm<128, false, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
-    }
-    if (transb) return tri_a ? launch_gemm<128, true, true, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st)
-                             : launch_gemm<128, true, false, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
-    return tri_a ? launch_gemm<128, false, true, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st)
-                 : launch_gemm<128, false, false, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    return mdf_gemm_batched(transb, tri_a, lower_c, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1,
+                            stream);
 }
 
 // workspace: the diagonal-block inverses D (nblocks x 128 x 128, kept for
@@ -196,13 +226,13 @@ extern "C" int mdf_potrf(double *A, long n, long lda, int *info, void *work, siz
         if (m2 <= 0) break;
         double *A21 = A + (k0 + kb) * lda + k0;
         // A21 <- A21 inv(L11)^T, through the panel buffer
-        int rc = launch_gemm<128, true, false, false>((int)m2, kb, kb, 1., A21, lda, Dk, LB, 0., P, LB, st);
+        int rc = launch_gemm<128, true, 0, false>((int)m2, kb, kb, 1., A21, lda, 0, Dk, LB, 0, 0., P, LB, 0, 1, st);
         if (rc) return rc;
         MDF_HIP(hipMemcpy2DAsync(A21, lda * sizeof(double), P, LB * sizeof(double), kb * sizeof(double), m2,
                                  hipMemcpyDeviceToDevice, st));
         // A22 <- A22 - A21 A21^T (lower tiles)
         double *A22 = A + (k0 + kb) * lda + k0 + kb;
-        rc = launch_gemm<128, true, false, true>((int)m2, (int)m2, kb, -1., A21, lda, A21, lda, 1., A22, lda, st);
+        rc = launch_gemm<128, true, 0, true>((int)m2, (int)m2, kb, -1., A21, lda, 0, A21, lda, 0, 1., A22, lda, 0, 1, st);
         if (rc) return rc;
     }
     return MDF_OK;
@@ -230,10 +260,11 @@ extern "C" int mdf_trtri(const double *L, long n, long ldl, const double *D, dou
                                  kb * sizeof(double), kb, hipMemcpyDeviceToDevice, st));
         if (b0 == 0) continue;
         // T = L[b, 0:b0] X[0:b0, 0:b0];  X[b, 0:b0] = -D_b T
-        int rc = launch_gemm<128, false, false, false>(kb, (int)b0, (int)b0, 1., L + b0 * ldl, ldl, X, ldx, 0.,
-                                                       T, b0, st);
+        int rc = launch_gemm<128, false, 0, false>(kb, (int)b0, (int)b0, 1., L + b0 * ldl, ldl, 0, X, ldx, 0, 0.,
+                                                   T, b0, 0, 1, st);
         if (rc) return rc;
-        rc = launch_gemm<128, false, false, false>(kb, (int)b0, kb, -1., Db, LB, T, b0, 0., X + b0 * ldx, ldx, st);
+        rc = launch_gemm<128, false, 0, false>(kb, (int)b0, kb, -1., Db, LB, 0, T, b0, 0, 0., X + b0 * ldx, ldx, 0,
+                                               1, st);
         if (rc) return rc;
     }
     return MDF_OK;
@@ -245,6 +276,6 @@ extern "C" int mdf_trmm(const double *Linv, long n, long ldl, const double *B, l
     MDF_CHECK_ARG(Linv && B && C && n >= 0 && ncols >= 0 && ldl >= n && ldb >= ncols && ldc >= ncols,
                   "mdf_trmm: bad arguments");
     MDF_CHECK_ARG(n < (1L << 31) && ncols < (1L << 31), "mdf_trmm: sizes beyond int range");
-    return launch_gemm<128, false, true, false>((int)n, (int)ncols, (int)n, 1., Linv, ldl, B, ldb, 0., C, ldc,
-                                                as_stream(stream));
+    return launch_gemm<128, false, 1, false>((int)n, (int)ncols, (int)n, 1., Linv, ldl, 0, B, ldb, 0, 0., C, ldc, 0,
+                                             1, as_stream(stream));
 }

@@ -54,14 +54,31 @@ _orig_methods = {}
 RBLOCK_BYTES = 1 << 31
 #: bytes of one block of (nao, nocc) rows F_l dmoo_P of the weight build
 WBLOCK_BYTES = 1 << 30
-#: orbital rows per lower-triangle block of the weight build
-IBLOCK = 128
-#: aux rows per block of the triangular metric products
+#: aux rows per block of the triangular metric products (host-stored path)
 TBLOCK = 1024
 
 
 def stats():
     return dict(STATS)
+
+
+def _gemm(A, B, C, transb=False, tri_a=0, lower_c=False, alpha=1., beta=0.):
+    """``C = alpha A op(B) + beta C`` on the package's FP64 MFMA engine
+    (``mdf_gemm``): row-major device matrices with unit column stride."""
+    M, K = A.shape
+    N = B.shape[0] if transb else B.shape[1]
+    _lib.call('mdf_gemm', int(transb), int(tri_a), int(lower_c), M, N, K, float(alpha), _dev.ptr(A),
+              A.stride(0), _dev.ptr(B), B.stride(0), float(beta), _dev.ptr(C), C.stride(0), _dev.stream_ptr())
+    return C
+
+
+def _gemm_batched(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, transb=False, lower_c=False,
+                  alpha=1., beta=0.):
+    """``mdf_gemm_batched`` over raw device addresses (ints) with strides in
+    elements."""
+    _lib.call('mdf_gemm_batched', int(transb), 0, int(lower_c), M, N, K, float(alpha), ctypes.c_void_p(A), lda,
+              sA, ctypes.c_void_p(B), ldb, sB, float(beta), ctypes.c_void_p(C), ldc, sC, batch,
+              _dev.stream_ptr())
 
 
 def _class_sorted_pairs(mol, addr):
@@ -107,11 +124,11 @@ def _cache_key(dfobj, mol):
 
 
 def _metric_factor(dfobj, mol, aux):
-    """``(A, lower)``: the matrix the cderi is defined by (cderi = A T; the
-    inverse Cholesky factor of the metric, lower triangular, or the eigen
-    filter's coefficients), cached on the DF object for its current cderi
-    and basis (the reference reuses the build's ``aux_coef`` the same way,
-    electrolites/fastdfgrad.py:665-714)."""
+    """``(A, lower, At)``: the matrix the cderi is defined by (cderi = A T;
+    the inverse Cholesky factor of the metric, lower triangular, or the
+    eigen filter's coefficients) and its transpose, cached on the DF object
+    for its current cderi and basis (the reference reuses the build's
+    ``aux_coef`` the same way, electrolites/fastdfgrad.py:665-714)."""
     t = _dev.torch()
     key = _cache_key(dfobj, mol)
     cached = fastdf._cache_get(dfobj, '_mdf_grad_metric', key)
@@ -119,7 +136,7 @@ def _metric_factor(dfobj, mol, aux):
         return cached
     linv, coef = fastdfcderi.decompose_j2c(aux)
     A, lower = (t.tril(linv), True) if coef is None else (coef, False)
-    return fastdf._cache_put(dfobj, '_mdf_grad_metric', key, (A, lower))
+    return fastdf._cache_put(dfobj, '_mdf_grad_metric', key, (A, lower, A.T.contiguous()))
 
 
 def _pair_tables(dfobj, mol):
@@ -139,7 +156,7 @@ def _pair_tables(dfobj, mol):
         'ncol': len(addr), 'class_off': class_off,
         'pairs': i32(pairs.reshape(-1)) if len(pairs) else t.zeros(2, dtype=t.int32, device=_dev.device()),
         'cmap': i32(cmap), 'addr': _dev.to_device(addr.astype(np.int64), dtype=t.int64),
-        'tri': _lower_blocks(addr, mol.nao),
+        'lowidx': _lower_index(addr, mol.nao),
         'ao_atom': i32(mol.ao.shl_atom), 'aux_atom': i32(mol.aux.shl_atom),
     }
     fastdf._cache_put(dfobj, '_mdf_grad_pairs', key, tabs)
@@ -231,7 +248,9 @@ def _rhok(dfobj, fl, fr, nao, nrow):
             _lib.call('mdf_k_step1', _dev.ptr(cderi[L0]), code, nL, cderi.shape[1], cderi.stride(0),
                       _dev.ptr(lay.map), nao, _dev.ptr(fl), fl.stride(0), nocc, _dev.ptr(R), nocc, nao * nocc,
                       _dev.stream_ptr())
-            rhok[L0:L0 + nL] = t.matmul(frT, R[:nL])
+            # rhok[L] = F_r^T R_L, one batched product over the block's rows
+            _gemm_batched(nocc, nocc, nao, frT.data_ptr(), nao, 0, R.data_ptr(), nocc, nao * nocc,
+                          rhok[L0].data_ptr(), nocc, nocc * nocc, nL)
         return rhok
     L0 = 0
     for dense, _ in dfobj.loop():
@@ -243,49 +262,36 @@ def _rhok(dfobj, fl, fr, nao, nrow):
     return rhok
 
 
-def _dmoo(A, lower, rhok, naux):
-    """``dmoo[P] = sum_L A[L, P] rhok[L]``, by block rows when A is lower
-    triangular (only L >= P contributes)."""
+def _dmoo(A, lower, At, rhok, naux):
+    """``dmoo[P] = sum_L A[L, P] rhok[L]`` = A^T rhok as one product; A^T is
+    upper triangular on the Cholesky route (only L >= P contributes), and
+    the product skips the zero blocks."""
     t = _dev.torch()
     nrow, nocc = rhok.shape[0], rhok.shape[1]
     rk = rhok.reshape(nrow, nocc * nocc)
-    if not lower:
-        return (A.T @ rk).reshape(naux, nocc, nocc)
     dmoo = t.empty((naux, nocc * nocc), dtype=t.float64, device=_dev.device())
-    for P0 in range(0, naux, TBLOCK):
-        P1 = min(naux, P0 + TBLOCK)
-        dmoo[P0:P1] = A[P0:, P0:P1].T @ rk[P0:]
+    _gemm(At, rk, dmoo, tri_a=2 if lower else 0)
     return dmoo.reshape(naux, nocc, nocc)
 
 
 def _trace_products(dmoo, naux):
-    """``tr(dmoo_P dmoo_Q)`` for all P, Q: the lower block rows, mirrored."""
+    """``tr(dmoo_P dmoo_Q)`` for all P, Q: the lower 128-tiles of one
+    product, mirrored."""
     t = _dev.torch()
     dm_flat = dmoo.reshape(naux, -1)
     dmT_flat = dmoo.transpose(1, 2).contiguous().reshape(naux, -1)
-    out = t.zeros((naux, naux), dtype=t.float64, device=_dev.device())
-    for P0 in range(0, naux, TBLOCK):
-        P1 = min(naux, P0 + TBLOCK)
-        out[P0:P1, :P1] = dm_flat[P0:P1] @ dmT_flat[:P1].T
+    out = t.empty((naux, naux), dtype=t.float64, device=_dev.device())
+    _gemm(dm_flat, dmT_flat, out, transb=True, lower_c=True)
     out = t.tril(out)
     return out + t.tril(out, -1).T
 
 
-def _lower_blocks(addr, nao):
-    """Row blocks of the stored pairs for the lower-triangle products: block b
-    holds the pairs (i, j), i in [i0, i1), j <= i, at local index
-    (i - i0) * i1 + j of an (i1 - i0, i1) product."""
+def _lower_index(addr, nao):
+    """``hi * nao + lo`` for every stored pair (hi >= lo): where its value
+    sits in the lower triangle of an nao x nao matrix."""
     t = _dev.torch()
     pi, pj = np.divmod(np.asarray(addr, dtype=np.int64), nao)
-    hi, lo = np.maximum(pi, pj), np.minimum(pi, pj)
-    tri = []
-    for i0 in range(0, nao, IBLOCK):
-        i1 = min(nao, i0 + IBLOCK)
-        sel = np.nonzero((hi >= i0) & (hi < i1))[0]
-        if len(sel):
-            tri.append((i0, i1, _dev.to_device(sel, dtype=t.int64),
-                        _dev.to_device((hi[sel] - i0) * i1 + lo[sel], dtype=t.int64)))
-    return tri
+    return _dev.to_device(np.maximum(pi, pj) * nao + np.minimum(pi, pj), dtype=t.int64)
 
 
 def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_response=True, timings=None):
@@ -310,12 +316,14 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     # K weight per factor set: E_K = -1/4 k tr(D K[D]) restricted,
     # -1/2 k sum_s tr(D_s K[D_s]) over the spin densities
     wk = .25 if len(facs) == 1 else .5
-    D = sum(fl @ fr.T for fl, fr in facs)
+    D = t.empty((nao, nao), dtype=t.float64, device=_dev.device())
+    for si, (fl, fr) in enumerate(facs):
+        _gemm(fl, fr, D, transb=True, beta=0. if si == 0 else 1.)
     D = .5 * (D + D.T)
     aux = fastdfcderi._basis(mol, 'aux')
     ao = fastdfcderi._basis(mol, 'ao')
     tm.mark('metric')
-    A, lower = _metric_factor(dfobj, mol, aux)            # cderi = A T
+    A, lower, At = _metric_factor(dfobj, mol, aux)        # cderi = A T
     nrow, naux = A.shape
     tm.mark('rhok')
     cderi = fastdf._cderi_of(dfobj)
@@ -327,7 +335,9 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
         rhoks = [_rhok(dfobj, fl, fr, nao, nrow) for fl, fr in facs]
         rho = sum(t.diagonal(r, dim1=1, dim2=2).sum(dim=1) for r in rhoks)
     tm.mark('weights')
-    c = A.T @ rho                                               # (naux,)
+    c = t.empty((1, naux), dtype=t.float64, device=_dev.device())
+    _gemm(rho.reshape(1, -1), A, c)                             # c = A^T rho
+    c = c.reshape(-1)
     Z = (-.5 * j_factor) * t.outer(c, c)
     # W in the cderi's compact columns
     tabs = _pair_tables(dfobj, mol)
@@ -339,23 +349,28 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
         P1 = min(naux, P0 + jb)
         W[P0:P1] = (j_factor * c[P0:P1, None]) * Dflat[None, :]
     if k_factor != 0:
-        tri = tabs['tri']
+        lowidx = tabs['lowidx']
         for si, (fl, fr) in enumerate(facs):
             nocc = fl.shape[1]
-            dmoo = _dmoo(A, lower, rhoks[si], naux)
+            dmoo = _dmoo(A, lower, At, rhoks[si], naux)
             rhoks[si] = None
             Z += (wk * k_factor) * _trace_products(dmoo, naux)
-            # the exchange part only on the lower triangle of (i, j)
-            frT = fr.T.contiguous()
-            blk = max(1, min(naux, WBLOCK_BYTES // max(8 * nao * max(nocc, IBLOCK), 1)))
+            # the exchange part at the stored pairs: G_P = F_l dmoo_P F_r^T
+            # as H_P = F_r dmoo_P^T for a block of P (one product), then
+            # the lower 128-tiles of F_l H_P^T per P (one batched product),
+            # gathered into the cderi's compact columns
+            blk = max(1, min(naux, WBLOCK_BYTES // max(8 * nao * nao, 1)))
+            H = t.empty((nao, blk * nocc), dtype=t.float64, device=_dev.device())
+            G = t.empty((blk, nao, nao), dtype=t.float64, device=_dev.device())
             for P0 in range(0, naux, blk):
                 P1 = min(naux, P0 + blk)
-                Y = t.matmul(fl, dmoo[P0:P1])                   # (nP, nao, nocc)
-                Wp = W[P0:P1]
-                for i0, i1, sel, loc in tri:
-                    blkK = t.matmul(Y[:, i0:i1], frT[:, :i1])   # (nP, i1 - i0, i1)
-                    Wp[:, sel] -= (2 * wk * k_factor) * blkK.reshape(P1 - P0, -1)[:, loc]
-            del dmoo
+                nP = P1 - P0
+                Hb = H[:, :nP * nocc]
+                _gemm(fr, dmoo[P0:P1].reshape(nP * nocc, nocc), Hb, transb=True)
+                _gemm_batched(nao, nao, nocc, fl.data_ptr(), nocc, 0, Hb.data_ptr(), Hb.stride(0), nocc,
+                              G.data_ptr(), nao, nao * nao, nP, transb=True, lower_c=True)
+                W[P0:P1] -= (2 * wk * k_factor) * G[:nP].reshape(nP, nao * nao)[:, lowidx]
+            del dmoo, H, G
     del rhoks
     grad = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())
     grad_aux = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())

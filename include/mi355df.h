@@ -285,9 +285,15 @@ int mdf_int1e(const mdf_basis *ao, int nao, const int *pairs, const long *pair_c
  * Replace the cuSOLVER/cuBLAS calls of electrolites/fastdfcderi.py:
  * potrf (:196), explicit L^-1 by solve_triangular (:479-481) and the
  * triangular multiply of _trmm_right_lowerT (:237-261).  Row-major.
- * mdf_gemm: C = alpha A op(B) + beta C; transb: B is (N, K); tri_a: A is
- *   lower triangular (K loop bounded per row tile); lower_c: only the lower
- *   128x128 tiles of a square C are written.
+ * mdf_gemm: C = alpha A op(B) + beta C; transb: B is (N, K); tri_a 1: A is
+ *   lower triangular (K loop bounded per row tile), 2: upper triangular (the
+ *   K loop of row tile m starts at its first row), either needing M == K;
+ *   lower_c: only the lower 128x128 tiles of a square C are written.
+ * mdf_gemm_batched: the same over `batch` problems, operand i at A + i
+ *   strideA, B + i strideB, C + i strideC (a stride of 0 shares the
+ *   operand); the DF gradient's contractions of the metric factor and the
+ *   occupied-space blocks (electrolites/fastdfgrad.py:870-1182, there on
+ *   cuBLAS/cuTENSOR).
  * mdf_potrf: A = L L^T in place (lower; the strict upper part of each
  *   diagonal 128-block is zeroed).  *info (device int, zero it first) gets
  *   the 1-based index of the first non-positive pivot.  The workspace keeps
@@ -298,6 +304,9 @@ int mdf_int1e(const mdf_basis *ao, int nao, const int *pairs, const long *pair_c
 int mdf_gemm(int transb, int tri_a, int lower_c, long M, long N, long K, double alpha,
              const double *A, long lda, const double *B, long ldb, double beta, double *C, long ldc,
              void *stream);
+int mdf_gemm_batched(int transb, int tri_a, int lower_c, long M, long N, long K, double alpha,
+                     const double *A, long lda, long strideA, const double *B, long ldb, long strideB,
+                     double beta, double *C, long ldc, long strideC, long batch, void *stream);
 size_t mdf_potrf_workspace(long n);
 int mdf_potrf(double *A, long n, long lda, int *info, void *work, size_t work_bytes, void *stream);
 size_t mdf_trtri_workspace(long n);

@@ -205,3 +205,37 @@ def test_eigen_fallback_on_singular_metric(monkeypatch):
     vj, vk = fastdf.get_jk(D(), dm, hermi=1)
     rj, rk = O.get_jk(ref, pa, np.asarray(dm))
     assert rel(vj, rj) < 1e-9 and rel(vk, rk) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('transb,tri_a,lower_c', [(0, 2, 0), (1, 2, 0), (1, 0, 1), (0, 0, 0)])
+def test_gemm_batched(transb, tri_a, lower_c):
+    """mdf_gemm_batched against numpy: upper-triangular A (the K loop of a
+    row tile starts at its first row), shared operands (stride 0), ragged
+    sizes, padded leading dimensions, beta accumulation."""
+    import torch
+    from electrolites_amd import _dev, _lib
+    rng = np.random.default_rng(10 * transb + 3 * tri_a + lower_c)
+    nb, M = 5, 261
+    N = M if lower_c else 300
+    K = M if tri_a else 147
+    a = rng.standard_normal((nb, M, K + 3))
+    if tri_a == 2:
+        a[:, :, :K] = np.triu(a[:, :, :K])
+    b = rng.standard_normal((N, K + 2) if transb else (K, N + 2))          # shared: stride 0
+    c = rng.standard_normal((nb, M, N + 1))
+    A, B, C = (torch.from_numpy(x.copy()).cuda() for x in (a, b, c))
+    _lib.call('mdf_gemm_batched', transb, tri_a, lower_c, M, N, K, ctypes.c_double(.7),
+              _dev.ptr(A), K + 3, M * (K + 3), _dev.ptr(B), b.shape[1], 0, ctypes.c_double(-1.5),
+              _dev.ptr(C), N + 1, M * (N + 1), nb, None)
+    got = C.cpu().numpy()
+    bb = b[:, :K].T if transb else b[:K, :N]
+    for i in range(nb):
+        ref = .7 * a[i, :, :K] @ bb + -1.5 * c[i, :, :N]
+        g = got[i, :, :N]
+        if lower_c:
+            mask = (np.arange(M)[:, None] // 128) >= (np.arange(N)[None, :] // 128)
+            assert rel(g[mask], ref[mask]) < 1e-13
+        else:
+            assert rel(g, ref) < 1e-13
+        assert np.array_equal(got[i, :, N:], c[i, :, N:])          # the padding column untouched

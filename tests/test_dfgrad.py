@@ -232,7 +232,7 @@ def test_gradient_eigen_route(mol, monkeypatch):
     monkeypatch.setattr(FC, 'decompose_j2c', eigen_route)
     df = DF(mol).build()
     g = fastdfgrad.jk_energy_per_atom(df, D, 1., .5)
-    assert df._mdf_grad_metric[2] is False                   # the general (non-triangular) branch ran
+    assert df._mdf_grad_metric[2][1] is False                # the general (non-triangular) branch ran
     assert np.abs(g - ref).max() < 1e-7
 
 
