@@ -80,10 +80,32 @@ def host_eigh(a, concurrent=False):
         return np.linalg.eigh(a)
 
 
+def _cholesky_inverse(M):
+    """``(L^-1, ok)`` for the SPD device matrix M = L L^T (``mdf_potrf`` and
+    ``mdf_trtri``, the build's own kernels); ok False for a pivot below
+    1e-10 of the largest diagonal (M near singular)."""
+    from . import _lib
+    t = _dev.torch()
+    n = M.shape[0]
+    low = M.clone()
+    info = t.zeros(1, dtype=t.int32, device=M.device)
+    work = t.empty(_lib.query('mdf_potrf_workspace', n), dtype=t.uint8, device=M.device)
+    _lib.call('mdf_potrf', _dev.ptr(low), n, n, _dev.ptr(info), _dev.ptr(work), work.numel(),
+              _dev.stream_ptr())
+    dmin, dmax, bad = t.stack([low.diagonal().min(), low.diagonal().max(), info[0].double()]).tolist()
+    if bad != 0 or not dmin > 1e-5 * dmax:
+        return None, False
+    linv = t.empty((n, n), dtype=t.float64, device=M.device)
+    w2 = t.empty(_lib.query('mdf_trtri_workspace', n), dtype=t.uint8, device=M.device)
+    _lib.call('mdf_trtri', _dev.ptr(low), n, n, _dev.ptr(work), _dev.ptr(linv), n, _dev.ptr(w2), w2.numel(),
+              _dev.stream_ptr())
+    return linv, True
+
+
 class RHF:
     def __init__(self, mol, conv_tol=1e-10, conv_tol_grad=None, max_cycle=100, diis_space=8,
                  df=None, verbose=0, k_precision='fp64', switch_err=1e-5, low_precision='fp32',
-                 host_diag_max=0, precision_report=False):
+                 host_diag_max=0, precision_report=False, purify_min=None):
         if k_precision not in ('fp64', 'fp32', 'bf16x3', 'mixed'):
             raise ValueError(f'k_precision {k_precision!r}')
         if low_precision not in ('fp32', 'bf16x3'):
@@ -94,6 +116,9 @@ class RHF:
         # the same density and records the relative errors in ``history``
         self.precision_report = precision_report
         self.host_diag_max = host_diag_max
+        # orthogonal dimensions above this take the occupied space of each
+        # cycle's Fock matrix by purification instead of a diagonalisation
+        self.purify_min = purify_min
         self.concurrent_diag = False      # set by a caller that holds single_thread_blas
         self.switch_err = switch_err
         self.mol = mol
@@ -131,12 +156,70 @@ class RHF:
         return self
 
     def _density(self, C):
+        """The closed-shell density of the first ``nocc`` columns of C,
+        tagged with them (one-factor exchange)."""
         t = _dev.torch()
         nocc = self.nocc
         co = C[:, :nocc].contiguous()
         dm = 2. * co @ co.T
         occ = t.full((nocc,), 2., dtype=t.float64, device=C.device)
         return fastdf.tag_array(dm, mo_coeff=co, mo_occ=occ)
+
+    def _purify_occ(self, A):
+        """An orthonormal basis (n x nocc) of the space of the nocc lowest
+        eigenvalues of the symmetric ``A`` without diagonalising it, or None.
+
+        SP2 purification (Niklasson, Phys. Rev. B 66, 155115 (2002)): the
+        Gershgorin bounds map A's spectrum into [0, 1] reversed, X0 = (e_max
+        - A) / (e_max - e_min); each step takes X^2 or 2X - X^2, whichever
+        trace lies closer to nocc, and drives the eigenvalues to 1 (the nocc
+        lowest of A) or 0, until X is the spectral projector P (FP64
+        products only).  The basis is
+        P R (R^T P R)^-1/2 by a Cholesky factor (``mdf_potrf`` /
+        ``mdf_trtri``), R the previous cycle's occupied orbitals: no solver
+        library is called.  None (the caller diagonalises) when the steps do
+        not converge (a vanishing gap at nocc) or P R loses rank."""
+        t = _dev.torch()
+        n, nocc = A.shape[0], self.nocc
+        R = getattr(self, '_q_occ', None)
+        if R is None or R.shape != (n, nocc) or nocc == 0:
+            return None
+        d = A.diagonal()
+        off = A.abs().sum(dim=1) - d.abs()
+        emin, emax = t.stack([(d - off).min(), (d + off).max()]).tolist()
+        if not emax > emin:
+            return None
+        X = A * (-1. / (emax - emin))
+        X.diagonal().add_(emax / (emax - emin))
+        # the step choice is made on the device (no host round trip per
+        # step); convergence is read back once per PURIFY_BLOCK steps, and
+        # the steps run until a block ends with tr(X - X^2) below 1e-12
+        # (further steps leave a converged projector as it is)
+        done = False
+        for _ in range(0, self.PURIFY_MAXIT, self.PURIFY_BLOCK):
+            for _ in range(self.PURIFY_BLOCK):
+                X2 = X @ X
+                trX, trX2 = X.diagonal().sum(), X2.diagonal().sum()
+                up = (trX2 - nocc).abs() >= (2. * trX - trX2 - nocc).abs()
+                X = t.where(up, 2. * X - X2, X2)
+            if abs(float(trX - trX2)) < 1e-12:
+                done = True
+                break
+        if not done:
+            return None
+        if abs(float(X.diagonal().sum()) - nocc) > 1e-6:
+            return None
+        PR = X @ R
+        M = (R.T @ PR).contiguous()
+        L, ok = _cholesky_inverse(M)
+        if not ok:
+            return None
+        return PR @ L.T
+
+    #: most SP2 steps before falling back to a diagonalisation
+    PURIFY_MAXIT = 96
+    #: SP2 steps between reads of the convergence measure
+    PURIFY_BLOCK = 8
 
     # ------------------------------------------------------------- DIIS
     def _diis_reset(self):
@@ -180,7 +263,28 @@ class RHF:
         coef = t.from_numpy(coef).to(F.device)
         return (coef.view(-1, 1, 1) * self._fh[:n]).sum(dim=0)
 
-    def _diag(self, F):
+    def _occupied(self, F):
+        """The next cycle's occupied AO coefficients (nao x nocc) from the
+        DIIS-extrapolated Fock matrix: purified (``purify_min``) or from a
+        diagonalisation; the occupied orbitals in the orthogonal basis are
+        kept for the next purification."""
+        t = _dev.torch()
+        if self.purify_min is not None and self.X.shape[1] > self.purify_min:
+            A = self.X.T @ F @ self.X
+            A = .5 * (A + A.T)
+            q = self._purify_occ(A)
+            if q is not None:
+                self._q_occ = q
+                self.purified = getattr(self, 'purified', 0) + 1
+                self.mo_energy = None
+                self.mo_coeff = self.X @ q
+                return self.mo_coeff
+        e, C, c = self._diag(F, orth=True)
+        self._q_occ = c[:, :self.nocc].contiguous()
+        self.mo_energy, self.mo_coeff = e, C
+        return C
+
+    def _diag(self, F, orth=False):
         """FP64 eigenpairs of ``X^T F X``: on the device, or with LAPACK on
         the host when the orthogonal dimension is at most ``host_diag_max``
         (small symmetric eigenproblems are latency-bound on the GPU; a serving
@@ -197,6 +301,8 @@ class RHF:
             # calls from a serving loop's worker threads crashed the process;
             # there, the requests diagonalising at once share one batched call
             e, c = eigsolve.eigh(A)
+        if orth:
+            return e, self.X @ c, c
         return e, self.X @ c
 
     # ------------------------------------------------------------- loop
@@ -239,7 +345,8 @@ class RHF:
         if not hasattr(self, 'H'):
             self.build()
         if dm0 is None:
-            _, C = self._diag(self.H)
+            _, C, c = self._diag(self.H, orth=True)
+            self._q_occ = c[:, :self.nocc].contiguous()
             dm0 = self._density(C)
         self.dm = dm0
         self._diis_reset()
@@ -270,7 +377,7 @@ class RHF:
         err = self.X.T @ (err - err.T) @ self.X
         F = self._diis(F, err)
         # diagonalise first; the scalars come back in one transfer afterwards
-        self.mo_energy, self.mo_coeff = self._diag(F)
+        self._occupied(F)
         e_el, err_max = t.stack([e_el, err.abs().max()]).tolist()
         e_tot = e_el + self.enuc
         de = e_tot - self._e_old

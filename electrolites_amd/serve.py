@@ -36,6 +36,10 @@ the requests waiting at once.  Several worker threads calling hipSOLVER
 ``hipsolverDsyevd``); no worker thread calls a device solver library now.
 A device ``syevd`` is a chain of small kernels (about 18 ms at 816 x 816
 alone), so on this workload the host calls, running in parallel, are faster.
+Requests above ``purify_min`` (default 600) skip the diagonalisation: their
+occupied space comes from SP2 purification of the Fock matrix on the GPU
+(:meth:`scf.RHF._purify_occ`, FP64 products only), which is faster than a
+single-threaded host diagonalisation at that size.
 
 Each request runs exactly the cycle sequence of a stand-alone
 :class:`scf.RHF` with the same options, and its grouped J/K item is cut
@@ -123,12 +127,12 @@ PLAN_KEEP_TICKS = 32
 class _Slot:
     """An admitted request: its SCF state and its grouped J/K item."""
 
-    def __init__(self, req, conv_tol, max_cycle, host_diag_max=0, low_precision='fp32'):
+    def __init__(self, req, conv_tol, max_cycle, host_diag_max=0, low_precision='fp32', purify_min=None):
         t = _dev.torch()
         self.req = req
         self.scf = RHF(req.mol, conv_tol=conv_tol, max_cycle=max_cycle,
                        k_precision=req.k_precision, host_diag_max=host_diag_max,
-                       low_precision=low_precision)
+                       low_precision=low_precision, purify_min=purify_min)
         # the server constructs and absorbs slots inside single_thread_blas
         self.scf.concurrent_diag = True
         self.scf.build()
@@ -173,7 +177,7 @@ class Server:
 
     def __init__(self, hbm_budget=HBM_BUDGET, max_batch=64, conv_tol=1e-10, max_cycle=100,
                  graphs=False, workers=16, host_diag_max=2000, async_build=True,
-                 low_precision='bf16x3', pipeline=False, schedule='async'):
+                 low_precision='bf16x3', pipeline=False, schedule='async', purify_min=600):
         if schedule not in ('async', 'lockstep'):
             raise ValueError(f'schedule {schedule!r}: expected async or lockstep')
         self.hbm_budget = float(hbm_budget)
@@ -188,6 +192,7 @@ class Server:
         self._work = _dev.Workspace(grow=2.0)
         self.workers = workers
         self.host_diag_max = host_diag_max
+        self.purify_min = purify_min
         self.low_precision = low_precision
         self._pool = concurrent.futures.ThreadPoolExecutor(workers) \
             if workers > 1 or schedule == 'async' else None
@@ -225,7 +230,8 @@ class Server:
         t = _dev.torch()
         stream = t.cuda.Stream()
         with t.cuda.stream(stream), single_thread_blas():
-            slot = _Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max, self.low_precision)
+            slot = _Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max, self.low_precision,
+                         self.purify_min)
             ev = t.cuda.Event()
             ev.record(stream)
         return slot, ev
@@ -240,7 +246,7 @@ class Server:
             if self._builder is None:
                 with single_thread_blas():
                     self._activate(_Slot(req, self.conv_tol, self.max_cycle, self.host_diag_max,
-                                         self.low_precision))
+                                         self.low_precision, self.purify_min))
             else:
                 self._building.append(self._builder.submit(self._build, req))
         # requests whose build finished join the batch; when nothing is

@@ -37,15 +37,15 @@ def test_footprint_and_streams():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('schedule', ['async', 'lockstep'])
+@pytest.mark.parametrize('schedule,purify_min', [('async', None), ('lockstep', None), ('async', 60)])
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
-def test_alone_equals_mix(precision, schedule):
+def test_alone_equals_mix(precision, schedule, purify_min):
     from electrolites_amd import serve
     from electrolites_amd.scf import RHF
     mols = _mols()
     # in the mix: a new request joins every other J/K launch while earlier
     # ones run (built at admission, so it is active at once)
-    srv = serve.Server(conv_tol=1e-10, schedule=schedule, async_build=False)
+    srv = serve.Server(conv_tol=1e-10, schedule=schedule, async_build=False, purify_min=purify_min)
     reqs = [serve.Request(m, rid=i, k_precision=precision) for i, m in enumerate(mols)]
     pending = list(reqs)
     peak = 0
@@ -60,12 +60,12 @@ def test_alone_equals_mix(precision, schedule):
     assert peak >= 3, 'requests did not overlap'
     for r in reqs:
         assert r.converged
-        alone = serve.Server(conv_tol=1e-10, schedule=schedule)
+        alone = serve.Server(conv_tol=1e-10, schedule=schedule, purify_min=purify_min)
         a = serve.Request(r.mol, k_precision=precision)
         alone.serve([a])
         # the stand-alone driver with the server's diagonalisation choice
         mf = RHF(r.mol, conv_tol=1e-10, k_precision=precision, host_diag_max=srv.host_diag_max,
-                 low_precision=srv.low_precision)
+                 low_precision=srv.low_precision, purify_min=srv.purify_min)
         e = mf.kernel()
         assert abs(r.e_tot - a.e_tot) <= 1e-12, (r.rid, r.e_tot, a.e_tot)
         assert abs(r.e_tot - e) <= 1e-12, (r.rid, r.e_tot, e)
