@@ -234,7 +234,7 @@ def test_config5_serving_stream():
     assert sum(m.nao > 400 for m in mols) >= 3
     arr = serve.poisson_arrivals(len(mols), 8., seed=2)
     reqs = [serve.Request(m, arr[i], rid=i, k_precision='mixed') for i, m in enumerate(mols)]
-    srv = serve.Server(conv_tol=1e-10, host_diag_max=400)
+    srv = serve.Server(conv_tol=1e-10, host_diag_max=400, purify_min=None)
     print(f'config 5: {len(mols)} requests nao {sorted(m.nao for m in mols)}, '
           f'device diagonalisation above {srv.host_diag_max}', flush=True)
     stats = srv.serve(reqs)
@@ -242,7 +242,7 @@ def test_config5_serving_stream():
     assert max(tr[1] for tr in srv.trace) >= 3, 'requests did not overlap'
     assert stats['p99_s'] >= stats['p50_s'] > 0
     for r in sorted(reqs, key=lambda r: r.rid):
-        alone = serve.Server(conv_tol=1e-10, host_diag_max=400)
+        alone = serve.Server(conv_tol=1e-10, host_diag_max=400, purify_min=None)
         a = serve.Request(r.mol, k_precision='mixed')
         alone.serve([a])
         assert abs(r.e_tot - a.e_tot) <= 1e-12, (r.rid, r.e_tot, a.e_tot)
