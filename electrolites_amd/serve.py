@@ -208,6 +208,8 @@ class Server:
         self.done = []
         self.resident = 0.
         self._plans = {}
+        self._dead_plans = []          # plans of retired requests, closed at close()
+        self._serial = 0
         self.ticks = 0
         self.t0 = time.perf_counter()
         self.trace = []               # (time, n_active, n_queued, resident bytes)
@@ -266,13 +268,18 @@ class Server:
     def _activate(self, slot):
         """Join the batch, in the lighter of the two pipeline groups (work
         weighed as nao^3)."""
+        slot.serial = self._serial
+        self._serial += 1
         slot.weight = float(slot.nao) ** 3
         slot.group = 0 if self._weight[0] <= self._weight[1] else 1
         self._weight[slot.group] += slot.weight
         self.active.append(slot)
 
     def _plan(self, slots, prec):
-        key = (prec,) + tuple(sorted(id(s) for s in slots))
+        # keyed by the slots' serial numbers, which are never reused (an
+        # object's id() is: a new request could otherwise meet a plan cached
+        # for the one that held its address before, and read freed buffers)
+        key = (prec,) + tuple(sorted(s.serial for s in slots))
         g = self._plans.get(key)
         if g is None:
             # plans run one after another on one stream: one shared workspace
@@ -288,6 +295,18 @@ class Server:
         return key, g
 
     def _retire(self, s):
+        # the plans this request was part of are never run again; their
+        # references to its buffers go now (so its bytes are freed), their
+        # small device tables when the server closes (freeing device memory
+        # synchronises the device)
+        for key in [k for k in self._plans if s.serial in k[1:]]:
+            g = self._plans.pop(key)
+            self._plan_tick.pop(key, None)
+            g.entries = None
+            g._arr = None
+            g.graph = None
+            g.work = None
+            self._dead_plans.append(g)
         r, scf = s.req, s.scf
         r.t_done = self.now()
         r.e_tot, r.cycles, r.converged = scf.e_tot, scf.cycles, scf.converged
@@ -467,6 +486,9 @@ class Server:
         return summary(self.done)
 
     def close(self):
+        for g in self._dead_plans:
+            g.close()
+        self._dead_plans = []
         for g in self._plans.values():
             g.close()
         self._plans.clear()

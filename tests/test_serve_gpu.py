@@ -92,3 +92,33 @@ def test_admission_budget():
     assert srv.resident == 0
     for r in reqs:
         assert r.t_admit >= r.arrival and r.t_done > r.t_admit
+
+
+@pytest.mark.gpu
+def test_plans_never_outlive_their_requests():
+    """Grouped plans are keyed by never-reused slot serials and dropped when
+    a member retires: while requests retire and others join (the pattern
+    that, keyed by object ids, once reran a retired request's plan on a new
+    request), every cached plan holds active requests only, and every
+    energy equals the request's energy alone."""
+    from electrolites_amd import serve
+    mols = _mols() * 2
+    srv = serve.Server(conv_tol=1e-9, async_build=False)
+    reqs = [serve.Request(m, rid=i) for i, m in enumerate(mols)]
+    pending = list(reqs)
+    while pending or srv.busy:
+        if pending and len(srv.active) < 3:
+            srv.submit(pending.pop(0))
+        srv.step()
+        live = {s.serial for s in srv.active}
+        for key in srv._plans:
+            assert set(key[1:]) <= live, (key, live)
+        assert srv.ticks < 5000
+    srv.close()
+    alone = {}
+    for r in reqs:
+        if id(r.mol) not in alone:
+            a = serve.Request(r.mol)
+            serve.Server(conv_tol=1e-9).serve([a])
+            alone[id(r.mol)] = a.e_tot
+        assert abs(r.e_tot - alone[id(r.mol)]) <= 1e-12, (r.rid, r.e_tot, alone[id(r.mol)])
