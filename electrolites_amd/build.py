@@ -19,6 +19,11 @@ HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=' + ARCH, '-O3', '-fPIC', '-std=c++17',
          # packed-FP32 instructions stay off (compiler-generated v_pk_*_f32)
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+         # MFMA accumulators in VGPRs, never AGPRs: a register-only loop of
+         # v_mfma_f64_16x16x4 runs at 77.6 TF/s with VGPR accumulators and
+         # 47 TF/s with AGPR ones on gfx950, and the K step-1 tile of 160
+         # columns (AGPR accumulators by default) ran 65.5 -> 48.9 ms per pass
+         '-mllvm', '-amdgpu-mfma-vgpr-form',
          '-Wno-unused-command-line-argument', '-Wno-unknown-warning-option',
          '-I' + os.path.join(HERE, '..', 'include')]
 

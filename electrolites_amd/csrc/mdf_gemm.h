@@ -180,9 +180,21 @@ struct RowsLoader {
         const int t = threadIdx.x;
         const int row = r0 + t / (G_BK / N);
         const int kb = k0 + kt * G_BK + (t % (G_BK / N)) * N;
-        const Src *src = base + (long)row * ld + kb;
+        if (row < nrows && kb + N <= kmax) {
+            // the whole chunk is in range: plain loads, no per-element select
+            const Src *src = base + (long)row * ld + kb;
 #pragma unroll
-        for (int v = 0; v < N; v++) r[v] = (row < nrows && kb + v < kmax) ? src[v] : (Src)0;
+            for (int v = 0; v < N; v++) r[v] = src[v];
+            return;
+        }
+        // edge: unconditional loads at clamped (in-bounds) addresses, the
+        // out-of-range elements selected to zero
+        const Src *src = base + (long)min(row, nrows - 1) * ld;
+#pragma unroll
+        for (int v = 0; v < N; v++) {
+            const Src x = src[min(kb + v, kmax - 1)];
+            r[v] = (row < nrows && kb + v < kmax) ? x : (Src)0;
+        }
     }
     __device__ __forceinline__ void store(const Src (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;
@@ -209,9 +221,21 @@ struct ColsLoader {
         const int kk = t / TPR;
         const int cb = c0 + (t % TPR) * N;
         const int k = k0 + kt * G_BK + kk;
-        const Src *src = base + (long)k * ld + cb;
+        if (kk < G_BK && k < kmax && cb + N <= ncols) {
+            // the whole chunk is in range: plain loads, no per-element select
+            const Src *src = base + (long)k * ld + cb;
 #pragma unroll
-        for (int v = 0; v < N; v++) r[v] = (kk < G_BK && k < kmax && cb + v < ncols) ? src[v] : (Src)0;
+            for (int v = 0; v < N; v++) r[v] = src[v];
+            return;
+        }
+        // edge: unconditional loads at clamped (in-bounds) addresses, the
+        // out-of-range elements selected to zero
+        const Src *src = base + (long)min(k, kmax - 1) * ld;
+#pragma unroll
+        for (int v = 0; v < N; v++) {
+            const Src x = src[min(cb + v, ncols - 1)];
+            r[v] = (kk < G_BK && k < kmax && cb + v < ncols) ? x : (Src)0;
+        }
     }
     __device__ __forceinline__ void store(const Src (&r)[N], T *lds, int pitch) const {
         const int t = threadIdx.x;

@@ -252,6 +252,12 @@ __device__ __forceinline__ void finalize_range(const double *__restrict__ part, 
 // go to the wider tile
 inline int pick_bn(int nocc) {
     const int cand[6] = {64, 128, 160, 192, 256, 320};
+    // MDF_K_STEP1_BN forces one of the widths (for tuning runs)
+    if (const char *e = getenv("MDF_K_STEP1_BN")) {
+        const int v = atoi(e);
+        for (int b : cand)
+            if (b == v) return b;
+    }
     int best = 64;
     long best_cost = 1L << 40;
     for (int b : cand) {
