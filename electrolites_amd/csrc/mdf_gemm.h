@@ -168,11 +168,11 @@ __device__ __forceinline__ void gemm_epilogue(const Acc<TileShape<BM, BN>::MI, T
 // Row-major matrix M[row][k] (k contiguous, element type S) as the A or B
 // operand: tile rows r0 .. r0+R-1, K-step kt covers k = kt*BK .. +BK-1.
 // Out-of-range reads 0.
-template <int R, typename T = double, typename Src = double>
+template <int R, typename T = double, typename Src = double, int THREADS = G_THREADS>
 struct RowsLoader {
     using reg_t = Src;
-    static constexpr int N = R * G_BK / G_THREADS;
-    static_assert(N >= 1 && (G_THREADS / (G_BK / N)) == R, "RowsLoader shape");
+    static constexpr int N = R * G_BK / THREADS;
+    static_assert(N >= 1 && G_BK % N == 0 && (THREADS / (G_BK / N)) == R, "RowsLoader shape");
     const Src *base;      // &M[0][0]
     long ld;              // row stride
     int r0, nrows, k0, kmax;

@@ -25,6 +25,15 @@ namespace {
 
 constexpr int LB = 128;   // panel / tile width
 
+// Lower 128-tile x of a square result: (ti, tj) with tj <= ti, row-major
+// over the lower triangle of tiles.
+__device__ __forceinline__ void lower_tile(int x, int &ti, int &tj) {
+    ti = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+    while (ti * (ti + 1) / 2 > x) ti--;
+    while ((ti + 1) * (ti + 2) / 2 <= x) ti++;
+    tj = x - ti * (ti + 1) / 2;
+}
+
 // TRI: 0 = A general; 1 = A lower triangular (row block m reads k < m0 + 128);
 // 2 = A upper triangular (row block m reads k >= m0).  blockIdx.z is the
 // batch index (operand strides sA, sB, sC).
@@ -39,11 +48,7 @@ __global__ __launch_bounds__(G_THREADS, 2) void gemm_f64_kernel(
     C += (long)blockIdx.z * sC;
     int ti, tj;
     if (LOWER_C) {
-        const int x = blockIdx.x;
-        ti = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
-        while (ti * (ti + 1) / 2 > x) ti--;
-        while ((ti + 1) * (ti + 2) / 2 <= x) ti++;
-        tj = x - ti * (ti + 1) / 2;
+        lower_tile(blockIdx.x, ti, tj);
     } else {
         ti = blockIdx.y;
         tj = blockIdx.x;
@@ -71,6 +76,70 @@ __global__ __launch_bounds__(G_THREADS, 2) void gemm_f64_kernel(
     });
 }
 
+// C = alpha A B + beta C with 128 x 320 tiles on 2 x 4 wavefronts (B not
+// transposed, A general): for N in (256, 320] one column tile covers C's
+// width, where 128-wide tiles would compute 384 columns and read every A
+// tile three times (the gradient's F_r^T R_L, N = nocc).
+__global__ __launch_bounds__(512) void gemm_f64_wide(int M, int N, int K, double alpha, const double *__restrict__ A,
+                                                     long lda, long sA, const double *__restrict__ B, long ldb,
+                                                     long sB, double beta, double *C, long ldc, long sC) {
+    using S = TileShape<128, 320, 2, 4>;
+    __shared__ double lds[S::LDS_DOUBLES];
+    A += (long)blockIdx.z * sA;
+    B += (long)blockIdx.z * sB;
+    C += (long)blockIdx.z * sC;
+    const int m0 = blockIdx.y * 128;
+    RowsLoader<128, double, double, 512> la{A, lda, m0, M, 0, K};
+    ColsLoader<320, double, double, 512> lb{B, ldb, 0, N, 0, K};
+    Acc<S::MI, S::NI> acc;
+    acc.zero();
+    gemm_mainloop<128, 320, decltype(la), decltype(lb), double, 2, 4>(la, lb, (K + G_BK - 1) / G_BK, lds, acc);
+    auto emit = [&](int r, int c, double v) {
+        const int m = m0 + r;
+        if (m < M && c < N) {
+            double *dst = C + (long)m * ldc + c;
+            *dst = beta == 0. ? alpha * v : alpha * v + beta * *dst;
+        }
+    };
+    gemm_epilogue_layout<128, 320, double, decltype(acc), decltype(emit) &, 2, 4>(acc, emit);
+}
+
+// The lower 128-tiles of P_z = A_z op(B_z) (n x n), added at the stored
+// pairs: W[z ldw + pcol[i (i + 1) / 2 + j]] += alpha P_z[i][j] for
+// n > i >= j with pcol >= 0.  No P_z is written.
+template <bool BT>
+__global__ __launch_bounds__(G_THREADS, 2) void gemm_pairs_kernel(int n, int K, double alpha,
+                                                                  const double *__restrict__ A, long lda, long sA,
+                                                                  const double *__restrict__ B, long ldb, long sB,
+                                                                  const int *__restrict__ pcol, double *W, long ldw) {
+    using S = TileShape<128, 128>;
+    __shared__ double lds[S::LDS_DOUBLES];
+    A += (long)blockIdx.z * sA;
+    B += (long)blockIdx.z * sB;
+    W += (long)blockIdx.z * ldw;
+    int ti, tj;
+    lower_tile(blockIdx.x, ti, tj);
+    const int m0 = ti * 128, n0 = tj * 128;
+    RowsLoader<128> la{A, lda, m0, n, 0, K};
+    Acc<S::MI, S::NI> acc;
+    acc.zero();
+    const int nk = (K + G_BK - 1) / G_BK;
+    if (BT) {
+        RowsLoader<128> lb{B, ldb, n0, n, 0, K};
+        gemm_mainloop<128, 128>(la, lb, nk, lds, acc);
+    } else {
+        ColsLoader<128> lb{B, ldb, n0, n, 0, K};
+        gemm_mainloop<128, 128>(la, lb, nk, lds, acc);
+    }
+    gemm_epilogue<128, 128>(acc, [&](int r, int c, double v) {
+        const int i = m0 + r, j = n0 + c;
+        if (i < n && j <= i) {
+            const int col = pcol[(long)i * (i + 1) / 2 + j];
+            if (col >= 0) W[col] += alpha * v;
+        }
+    });
+}
+
 constexpr long MAX_GRID_Z = 65535;
 
 template <int BN, bool BT, int TRI, bool LOWER_C>
@@ -89,6 +158,18 @@ int launch_gemm(int M, int N, int K, double alpha, const double *A, long lda, lo
         hipLaunchKernelGGL((gemm_f64_kernel<BN, BT, TRI, LOWER_C>), g, dim3(G_THREADS), 0, st, M, N, K, alpha,
                            A + b0 * sA, lda, sA, B + b0 * sB, ldb, sB, beta, C + b0 * sC, ldc, sC);
         MDF_LAUNCHED("gemm_f64_kernel");
+    }
+    return MDF_OK;
+}
+
+int launch_wide(int M, int N, int K, double alpha, const double *A, long lda, long sA, const double *B, long ldb,
+                long sB, double beta, double *C, long ldc, long sC, long batch, hipStream_t st) {
+    if (M <= 0 || N <= 0 || batch <= 0) return MDF_OK;
+    for (long b0 = 0; b0 < batch; b0 += MAX_GRID_Z) {
+        const unsigned nz = (unsigned)(batch - b0 < MAX_GRID_Z ? batch - b0 : MAX_GRID_Z);
+        hipLaunchKernelGGL(gemm_f64_wide, dim3(1, (unsigned)((M + 127) / 128), nz), dim3(512), 0, st, M, N, K, alpha,
+                           A + b0 * sA, lda, sA, B + b0 * sB, ldb, sB, beta, C + b0 * sC, ldc, sC);
+        MDF_LAUNCHED("gemm_f64_wide");
     }
     return MDF_OK;
 }
@@ -186,6 +267,8 @@ extern "C" int mdf_gemm_batched(int transb, int tri_a, int lower_c, long M, long
     MDF_CHECK_ARG(strideA >= 0 && strideB >= 0 && strideC >= 0, "mdf_gemm: negative batch stride");
     hipStream_t st = as_stream(stream);
     const int m = (int)M, n = (int)N, k = (int)K;
+    if (!transb && tri_a == 0 && !lower_c && N > 256 && N <= 320)
+        return launch_wide(m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, batch, st);
     if (lower_c)
         return transb ? gemm_tri<true, true>(tri_a, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
                                              strideC, batch, st)
@@ -195,6 +278,33 @@ extern "C" int mdf_gemm_batched(int transb, int tri_a, int lower_c, long M, long
                                           strideC, batch, st)
                   : gemm_tri<false, false>(tri_a, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
                                            strideC, batch, st);
+}
+
+extern "C" int mdf_gemm_pairs_batched(int transb, long n, long K, double alpha, const double *A, long lda,
+                                      long strideA, const double *B, long ldb, long strideB, const int *pair_col,
+                                      double *W, long ldw, long batch, void *stream) {
+    MDF_CHECK_ARG(n >= 0 && K >= 0 && n < (1L << 31) && K < (1L << 31) && batch >= 0 &&
+                  n * (n + 1) / 2 < (1L << 31), "mdf_gemm_pairs_batched: bad sizes");
+    MDF_CHECK_ARG((A && B && pair_col && W) || batch == 0 || n == 0, "mdf_gemm_pairs_batched: null pointer");
+    MDF_CHECK_ARG(lda >= K && ldb >= (transb ? K : n) && strideA >= 0 && strideB >= 0 && ldw >= 0,
+                  "mdf_gemm_pairs_batched: leading dimension or stride");
+    if (n == 0 || batch == 0) return MDF_OK;
+    hipStream_t st = as_stream(stream);
+    const int t = (int)((n + 127) / 128);
+    for (long b0 = 0; b0 < batch; b0 += MAX_GRID_Z) {
+        const unsigned nz = (unsigned)(batch - b0 < MAX_GRID_Z ? batch - b0 : MAX_GRID_Z);
+        const dim3 g((unsigned)(t * (t + 1) / 2), 1, nz);
+        if (transb)
+            hipLaunchKernelGGL(gemm_pairs_kernel<true>, g, dim3(G_THREADS), 0, st, (int)n, (int)K, alpha,
+                               A + b0 * strideA, lda, strideA, B + b0 * strideB, ldb, strideB, pair_col, W + b0 * ldw,
+                               ldw);
+        else
+            hipLaunchKernelGGL(gemm_pairs_kernel<false>, g, dim3(G_THREADS), 0, st, (int)n, (int)K, alpha,
+                               A + b0 * strideA, lda, strideA, B + b0 * strideB, ldb, strideB, pair_col, W + b0 * ldw,
+                               ldw);
+        MDF_LAUNCHED("gemm_pairs_kernel");
+    }
+    return MDF_OK;
 }
 
 extern "C" int mdf_gemm(int transb, int tri_a, int lower_c, long M, long N, long K, double alpha,

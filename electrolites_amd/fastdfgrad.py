@@ -156,7 +156,6 @@ def _pair_tables(dfobj, mol):
         'ncol': len(addr), 'class_off': class_off,
         'pairs': i32(pairs.reshape(-1)) if len(pairs) else t.zeros(2, dtype=t.int32, device=_dev.device()),
         'cmap': i32(cmap), 'addr': _dev.to_device(addr.astype(np.int64), dtype=t.int64),
-        'lowidx': _lower_index(addr, mol.nao),
         'ao_atom': i32(mol.ao.shl_atom), 'aux_atom': i32(mol.aux.shl_atom),
     }
     fastdf._cache_put(dfobj, '_mdf_grad_pairs', key, tabs)
@@ -274,6 +273,43 @@ def _dmoo(A, lower, At, rhok, naux):
     return dmoo.reshape(naux, nocc, nocc)
 
 
+def _upper(nocc):
+    """Flat indices ``a nocc + b`` and ``b nocc + a`` of the upper triangle
+    (a <= b, row-major) of an nocc x nocc matrix, and the weights that turn
+    a dot product of two packed symmetric matrices into the full one
+    (1 on the diagonal, 2 off it), as device tensors."""
+    t = _dev.torch()
+    a, b = t.triu_indices(nocc, nocc, device=_dev.device())
+    w = t.where(a == b, 1., 2.).to(t.float64)
+    return (a * nocc + b, b * nocc + a), w
+
+
+def _dmoo_packed(A, lower, At, rhok, naux):
+    """``A^T rhok`` on the packed upper triangles of the symmetric rhok_L:
+    ``(naux, nocc (nocc + 1) / 2)`` and the index pair of ``_upper``."""
+    t = _dev.torch()
+    nrow, nocc = rhok.shape[0], rhok.shape[1]
+    (iu, il), w = _upper(nocc)
+    rp = rhok.reshape(nrow, nocc * nocc)[:, iu]
+    dmoo = t.empty((naux, rp.shape[1]), dtype=t.float64, device=_dev.device())
+    _gemm(At, rp, dmoo, tri_a=2 if lower else 0)
+    return dmoo, (iu, il)
+
+
+def _trace_products_packed(dmoo, nocc):
+    """``tr(dmoo_P dmoo_Q)`` = sum_ab dmoo_P[a, b] dmoo_Q[a, b] (symmetric
+    dmoo) from the packed upper triangles, the off-diagonal terms weighted
+    by sqrt(2) on both sides: the lower 128-tiles of one product, mirrored."""
+    t = _dev.torch()
+    naux = dmoo.shape[0]
+    _, w = _upper(nocc)
+    xs = dmoo * w.sqrt()
+    out = t.empty((naux, naux), dtype=t.float64, device=_dev.device())
+    _gemm(xs, xs, out, transb=True, lower_c=True)
+    out = t.tril(out)
+    return out + t.tril(out, -1).T
+
+
 def _trace_products(dmoo, naux):
     """``tr(dmoo_P dmoo_Q)`` for all P, Q: the lower 128-tiles of one
     product, mirrored."""
@@ -284,14 +320,6 @@ def _trace_products(dmoo, naux):
     _gemm(dm_flat, dmT_flat, out, transb=True, lower_c=True)
     out = t.tril(out)
     return out + t.tril(out, -1).T
-
-
-def _lower_index(addr, nao):
-    """``hi * nao + lo`` for every stored pair (hi >= lo): where its value
-    sits in the lower triangle of an nao x nao matrix."""
-    t = _dev.torch()
-    pi, pj = np.divmod(np.asarray(addr, dtype=np.int64), nao)
-    return _dev.to_device(np.maximum(pi, pj) * nao + np.minimum(pi, pj), dtype=t.int64)
 
 
 def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_response=True, timings=None):
@@ -349,28 +377,43 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
         P1 = min(naux, P0 + jb)
         W[P0:P1] = (j_factor * c[P0:P1, None]) * Dflat[None, :]
     if k_factor != 0:
-        lowidx = tabs['lowidx']
         for si, (fl, fr) in enumerate(facs):
             nocc = fl.shape[1]
-            dmoo = _dmoo(A, lower, At, rhoks[si], naux)
-            rhoks[si] = None
-            Z += (wk * k_factor) * _trace_products(dmoo, naux)
+            # one factor (F_l = F_r): every rhok_L and dmoo_P is symmetric,
+            # so A^T rhok and tr(dmoo_P dmoo_Q) run on the packed upper
+            # triangles (half the flops of both products)
+            sym = fl.data_ptr() == fr.data_ptr() and fl.shape == fr.shape and fl.stride() == fr.stride()
+            if sym:
+                dmoo, iu = _dmoo_packed(A, lower, At, rhoks[si], naux)
+                rhoks[si] = None
+                Z += (wk * k_factor) * _trace_products_packed(dmoo, nocc)
+            else:
+                dmoo = _dmoo(A, lower, At, rhoks[si], naux).reshape(naux, nocc * nocc)
+                rhoks[si] = None
+                Z += (wk * k_factor) * _trace_products(dmoo.reshape(naux, nocc, nocc), naux)
             # the exchange part at the stored pairs: G_P = F_l dmoo_P F_r^T
             # as H_P = F_r dmoo_P^T for a block of P (one product), then
-            # the lower 128-tiles of F_l H_P^T per P (one batched product),
-            # gathered into the cderi's compact columns
-            blk = max(1, min(naux, WBLOCK_BYTES // max(8 * nao * nao, 1)))
+            # the lower 128-tiles of F_l H_P^T per P added straight into
+            # the cderi's compact columns (one batched product; no G_P is
+            # written)
+            blk = max(1, min(naux, WBLOCK_BYTES // max(8 * nao * nocc, 1)))
             H = t.empty((nao, blk * nocc), dtype=t.float64, device=_dev.device())
-            G = t.empty((blk, nao, nao), dtype=t.float64, device=_dev.device())
+            full = t.empty((blk, nocc * nocc), dtype=t.float64, device=_dev.device()) if sym else None
             for P0 in range(0, naux, blk):
                 P1 = min(naux, P0 + blk)
                 nP = P1 - P0
                 Hb = H[:, :nP * nocc]
-                _gemm(fr, dmoo[P0:P1].reshape(nP * nocc, nocc), Hb, transb=True)
-                _gemm_batched(nao, nao, nocc, fl.data_ptr(), nocc, 0, Hb.data_ptr(), Hb.stride(0), nocc,
-                              G.data_ptr(), nao, nao * nao, nP, transb=True, lower_c=True)
-                W[P0:P1] -= (2 * wk * k_factor) * G[:nP].reshape(nP, nao * nao)[:, lowidx]
-            del dmoo, H, G
+                if sym:
+                    dm_blk = full[:nP]
+                    dm_blk[:, iu[0]] = dmoo[P0:P1]
+                    dm_blk[:, iu[1]] = dmoo[P0:P1]
+                else:
+                    dm_blk = dmoo[P0:P1]
+                _gemm(fr, dm_blk.reshape(nP * nocc, nocc), Hb, transb=True)
+                _lib.call('mdf_gemm_pairs_batched', 1, nao, nocc, float(-2 * wk * k_factor), ctypes.c_void_p(fl.data_ptr()),
+                          nocc, 0, ctypes.c_void_p(Hb.data_ptr()), Hb.stride(0), nocc, _dev.ptr(tabs['cmap']),
+                          _dev.ptr(W[P0]), W.stride(0), nP, _dev.stream_ptr())
+            del dmoo, H, full
     del rhoks
     grad = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())
     grad_aux = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())

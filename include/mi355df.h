@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 8
+#define MDF_ABI_VERSION 9
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
@@ -307,6 +307,18 @@ int mdf_gemm(int transb, int tri_a, int lower_c, long M, long N, long K, double 
 int mdf_gemm_batched(int transb, int tri_a, int lower_c, long M, long N, long K, double alpha,
                      const double *A, long lda, long strideA, const double *B, long ldb, long strideB,
                      double beta, double *C, long ldc, long strideC, long batch, void *stream);
+/* mdf_gemm_pairs_batched: for each z < batch, P_z = A_z op(B_z) (n x n;
+ *   A_z = A + z strideA is n x K, op(B_z) is K x n) is added at the stored
+ *   pairs of a packed cderi without being written itself:
+ *     W[z ldw + pair_col[i (i + 1) / 2 + j]] += alpha P_z[i][j]
+ *   for n > i >= j where pair_col (device int32, n (n + 1) / 2 entries) is
+ *   >= 0; only the lower 128-tiles of P_z are computed.  The DF gradient's
+ *   exchange weights F_l dmoo_P F_r^T at the cderi's compact columns
+ *   (electrolites/fastdfgrad.py:870-1182, there a dense cuTENSOR product
+ *   and a gather). */
+int mdf_gemm_pairs_batched(int transb, long n, long K, double alpha, const double *A, long lda,
+                           long strideA, const double *B, long ldb, long strideB, const int *pair_col,
+                           double *W, long ldw, long batch, void *stream);
 size_t mdf_potrf_workspace(long n);
 int mdf_potrf(double *A, long n, long lda, int *info, void *work, size_t work_bytes, void *stream);
 size_t mdf_trtri_workspace(long n);

@@ -239,3 +239,38 @@ def test_gemm_batched(transb, tri_a, lower_c):
         else:
             assert rel(g, ref) < 1e-13
         assert np.array_equal(got[i, :, N:], c[i, :, N:])          # the padding column untouched
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('transb', [0, 1])
+def test_gemm_pairs_batched(transb):
+    """mdf_gemm_pairs_batched against numpy: each P_z = A_z op(B) is added
+    at the stored pairs of row z of W only (a ragged n over three 128-tiles,
+    a column map with 60 % of the pairs screened out and the stored ones in
+    a shuffled column order, a shared B, padded W rows); every other entry
+    of W keeps its value."""
+    import torch
+    from electrolites_amd import _dev, _lib
+    rng = np.random.default_rng(20 + transb)
+    nb, n, K = 4, 261, 37
+    a = rng.standard_normal((nb, n, K + 1))
+    b = rng.standard_normal((n, K + 2) if transb else (K, n + 2))
+    npk = n * (n + 1) // 2
+    keep = rng.random(npk) < .4
+    ncol = int(keep.sum())
+    col = np.full(npk, -1, dtype=np.int32)
+    col[keep] = rng.permutation(ncol)
+    w = rng.standard_normal((nb, ncol + 5))
+    A, B, W = (torch.from_numpy(x.copy()).cuda() for x in (a, b, w))
+    C = torch.from_numpy(col).cuda()
+    _lib.call('mdf_gemm_pairs_batched', transb, n, K, ctypes.c_double(-.6), _dev.ptr(A), K + 1, n * (K + 1),
+              _dev.ptr(B), b.shape[1], 0, _dev.ptr(C), _dev.ptr(W), ncol + 5, nb, None)
+    bb = b[:, :K].T if transb else b[:K, :n]
+    i, j = np.tril_indices(n)
+    pk = i * (i + 1) // 2 + j
+    sel = col[pk] >= 0
+    ref = w.copy()
+    for z in range(nb):
+        p = a[z, :, :K] @ bb
+        ref[z, col[pk[sel]]] += -.6 * p[i[sel], j[sel]]
+    assert rel(W.cpu().numpy(), ref) < 1e-13
