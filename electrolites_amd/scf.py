@@ -263,14 +263,14 @@ class RHF:
         coef = t.from_numpy(coef).to(F.device)
         return (coef.view(-1, 1, 1) * self._fh[:n]).sum(dim=0)
 
-    def _occupied(self, F):
+    def _occupied(self, A):
         """The next cycle's occupied AO coefficients (nao x nocc) from the
-        DIIS-extrapolated Fock matrix: purified (``purify_min``) or from a
-        diagonalisation; the occupied orbitals in the orthogonal basis are
-        kept for the next purification."""
+        DIIS-extrapolated Fock matrix in the orthogonal basis: purified
+        (``purify_min``) or from a diagonalisation; the occupied orbitals in
+        the orthogonal basis are kept (the next purification and DIIS error
+        start from them)."""
         t = _dev.torch()
         if self.purify_min is not None and self.X.shape[1] > self.purify_min:
-            A = self.X.T @ F @ self.X
             A = .5 * (A + A.T)
             q = self._purify_occ(A)
             if q is not None:
@@ -279,7 +279,7 @@ class RHF:
                 self.mo_energy = None
                 self.mo_coeff = self.X @ q
                 return self.mo_coeff
-        e, C, c = self._diag(F, orth=True)
+        e, C, c = self._diag_orth(A)
         self._q_occ = c[:, :self.nocc].contiguous()
         self.mo_energy, self.mo_coeff = e, C
         return C
@@ -290,8 +290,13 @@ class RHF:
         (small symmetric eigenproblems are latency-bound on the GPU; a serving
         loop runs many of them side by side on host cores).  Either way the
         choice depends on the molecule only."""
+        e, C, c = self._diag_orth(self.X.T @ F @ self.X)
+        return (e, C, c) if orth else (e, C)
+
+    def _diag_orth(self, A):
+        """``(e, X c, c)`` with ``(e, c)`` the eigenpairs of ``A``, a Fock
+        matrix in the orthogonal basis."""
         t = _dev.torch()
-        A = self.X.T @ F @ self.X
         if A.shape[0] <= self.host_diag_max:
             e, c = host_eigh(A.cpu().numpy(), self.concurrent_diag)
             e = t.from_numpy(e).to(A.device)
@@ -301,9 +306,7 @@ class RHF:
             # calls from a serving loop's worker threads crashed the process;
             # there, the requests diagonalising at once share one batched call
             e, c = eigsolve.eigh(A)
-        if orth:
-            return e, self.X @ c, c
-        return e, self.X @ c
+        return e, self.X @ c, c
 
     # ------------------------------------------------------------- loop
     def kernel(self, dm0=None):
@@ -348,6 +351,8 @@ class RHF:
             _, C, c = self._diag(self.H, orth=True)
             self._q_occ = c[:, :self.nocc].contiguous()
             dm0 = self._density(C)
+        else:
+            self._q_occ = None              # dm0 is not 2 X q q^T X^T of a known q
         self.dm = dm0
         self._diis_reset()
         self._e_old = 0.
@@ -373,11 +378,20 @@ class RHF:
             return
         self.cycles = cycle = self.cycles + 1
         prec = self.prec
-        err = F @ self.dm @ self.S
-        err = self.X.T @ (err - err.T) @ self.X
-        F = self._diis(F, err)
+        # the Fock matrix and the DIIS error in the orthogonal basis (X^T S X
+        # = I); with D = 2 X q q^T X^T, X^T (F D S - S D F) X is
+        # 2 (Fo q q^T - q q^T Fo): two thin products instead of four square
+        Fo = self.X.T @ F @ self.X
+        q = self._q_occ
+        if q is not None:
+            M = (Fo @ q) @ q.T
+            err = 2. * (M - M.T)
+        else:
+            err = F @ self.dm @ self.S
+            err = self.X.T @ (err - err.T) @ self.X
+        Fo = self._diis(Fo, err)
         # diagonalise first; the scalars come back in one transfer afterwards
-        self._occupied(F)
+        self._occupied(Fo)
         e_el, err_max = t.stack([e_el, err.abs().max()]).tolist()
         e_tot = e_el + self.enuc
         de = e_tot - self._e_old
