@@ -452,9 +452,10 @@ def bench_batch(args):
 def bench_serve(args):
     """Continuous-batching DF-SCF serving over the molecule mix (as
     ``bench_batch``): for each Poisson rate, ``--requests`` energy requests
-    served with mixed precision (FP32 K until the DIIS error is below 1e-5,
-    FP64 finish), then the same stream solved one request at a time with the
-    plain FP64 driver.  Reports solves/hr and p50/p99 latency per rate."""
+    served with mixed precision (split-BF16 K until the DIIS error is below
+    the switch threshold, FP64 finish), then the same stream solved one
+    request at a time with the plain FP64 driver.  Reports solves/hr and
+    p50/p99 latency per rate."""
     import torch
     from electrolites_amd import _dev, serve
     _dev.require_gpu()
@@ -488,7 +489,7 @@ def bench_serve(args):
         'value': top['serving']['solves_per_hr'], 'unit': 'solves/hr',
         'n_gpus': 1, 'steps': len(mols), 'warmup': 1,
         'ms_per_step': 3.6e6 / top['serving']['solves_per_hr'],
-        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64+f32',
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64+bf16x3',
         'data': 'synthetic geometries (seeded water clusters, Li+ on half); Poisson arrivals',
         'config': {'workload': f'{len(mols)} requests, (H2O)3-40 (+Li+ on half), rates {rates} req/s',
                    'nao': [m.nao for m in mols]},
