@@ -380,6 +380,16 @@ constexpr int S2_FLUSH = 64;
 
 __host__ __device__ inline long pad32(long n) { return (n + 31) / 32 * 32; }
 
+// Element (i, k) of one aux row's split R plane (nao rows, ldr = pad32(nocc)
+// columns): stored as ldr / 32 chunks of 32 columns, each chunk nao rows of
+// 32 contiguous values, so the 128 rows x 32 columns a step-2 K-step reads
+// are one contiguous 8 KB run per plane (row-major rows would give 128 runs
+// of 64 bytes, half a cache line each).  The plane's size is nao * ldr, as
+// row-major.
+__host__ __device__ inline long r_index(int nao, int i, int k) {
+    return ((long)(k >> 5) * nao + i) * 32 + (k & 31);
+}
+
 // A operand: rows i0..i0+127 of the unpacked A_L; K-step kt covers
 // j = 32 kt .. 32 kt + 31.  THREADS threads, N = 4096 / THREADS values each.
 template <bool MAPPED, int THREADS, typename Src = double>
@@ -534,14 +544,15 @@ __device__ __forceinline__ void step1_tile(const Src *__restrict__ row, const in
             const int r = u / CH, c = (u % CH) * 8;
             const int i = i0 + r, k = k0 + c;
             if (i < nao && k < ldr)
-                *reinterpret_cast<uint4 *>(R + (long)i * ldr + k) = *reinterpret_cast<const uint4 *>(lds + r * P + c);
+                *reinterpret_cast<uint4 *>(R + r_index(nao, i, k)) = *reinterpret_cast<const uint4 *>(lds + r * P + c);
         }
         __syncthreads();
     }
 }
 
 // Rows r0..r0+127 of the R planes; K-step kt of this split is
-// (L, kc) = divmod(s0 + kt, ldr / 32), advanced in place by load().
+// (L, kc) = divmod(s0 + kt, ldr / 32), advanced in place by load().  A
+// K-step's 128 rows x 32 k of one plane are 8 KB in one piece (r_index).
 struct RLoader {
     using reg_t = uint4;
     static constexpr int N = b3::PlaneRows128::N;
@@ -553,7 +564,7 @@ struct RLoader {
     __device__ __forceinline__ void load(int, uint4 (&r)[N]) {
         const int t = threadIdx.x, row = r0 + (t >> 1);
         const bool ok = row < nao;
-        const long off = L * strideL + (long)row * ldr + kc * b3::BK + (t & 1) * 16;
+        const long off = L * strideL + r_index(nao, row, kc * b3::BK) + (t & 1) * 16;
         b3::PlaneRows128::fetch(ok ? hi + off : hi, ok ? lo + off : lo, ok, r);
         if (++kc == kchunks) {
             kc = 0;
