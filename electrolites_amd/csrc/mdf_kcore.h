@@ -366,10 +366,12 @@ inline KPlan plan_k(long naux, int nao, int nocc, bool two, long max_rows, size_
 
 // ------------------------------------------------- split-BF16 (prec 2)
 // The same two steps on the three-term split-BF16 engine (mdf_b3.h).  The
-// factors are pre-split into transposed planes FT[k][j] (j contiguous,
-// zero-padded to ldft = a multiple of 32 >= nao); R is stored split, as two
-// planes of [L][i][k] bfloat16 with k zero-padded to ldr = a multiple of 32
-// >= nocc, so step 2 copies it into LDS without conversion.
+// factors are pre-split into transposed planes FT[k][j] (j zero-padded to
+// ldft = a multiple of 32 >= nao); R is stored split, as two planes of
+// [L][i][k] bfloat16 with k zero-padded to ldr = a multiple of 32 >= nocc,
+// so step 2 copies it into LDS without conversion.  Both are stored in
+// chunks of 32 along their contiguous index (FT: (j / 32, k, j % 32); R:
+// r_index), so each K-step reads its tile of a plane as one contiguous run.
 namespace kb3 {
 
 using b3::u16;
@@ -483,8 +485,10 @@ struct FTLoader {
     const u16 *hi, *lo;
     long ld;
     int k0, nocc;
+    // the planes are chunk-major (split_ft_range): K-step kt is chunk kt,
+    // nocc rows of 32 values
     __device__ __forceinline__ void load(int kt, uint4 (&r)[N]) const {
-        P::fetch(hi, lo, ld, (long)kt * b3::BK, k0, nocc, r);
+        P::fetch(hi, lo, b3::BK, (long)kt * b3::BK * nocc, k0, nocc, r);
     }
     __device__ __forceinline__ void store(const uint4 (&r)[N], u16 *h, u16 *l) const { P::put(r, h, l); }
 };
@@ -606,11 +610,16 @@ __device__ __forceinline__ void step2_tile(const u16 *lhi, const u16 *llo, const
     } while (s < s1);
 }
 
-// FT[k][j] = F[j][k] split into two planes, j zero-padded to ldft.
+// FT[k][j] = F[j][k] split into two planes, j zero-padded to ldft, stored
+// chunk-major: (j / 32, k, j % 32).
 __device__ __forceinline__ void split_ft_range(const double *__restrict__ F, long ldf, int nao, int nocc, u16 *hi,
                                                u16 *lo, long ldft, long e0, long e1) {
+    // e runs over the planes in storage order: chunks of 32 j, each nocc
+    // rows k of 32 contiguous j (a step-1 K-step then reads BN rows x 32 j
+    // of a plane as one contiguous run)
     for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        const int k = (int)(e / ldft), j = (int)(e % ldft);
+        const long chunk = e / ((long)nocc * 32), rem = e % ((long)nocc * 32);
+        const int k = (int)(rem / 32), j = (int)(chunk * 32 + rem % 32);
         const double x = j < nao ? F[(long)j * ldf + k] : 0.;
         u16 h, l;
         b3::split((float)x, h, l);
