@@ -9,7 +9,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, 'libmi355df.so')
-ABI_VERSION = 9
+ABI_VERSION = 10
 DTYPE_F64, DTYPE_F32 = 0, 1
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -44,6 +44,7 @@ _SIGS = {
     'mdf_int3c2e': (I, [P, P, P, P, P, P, L, P, P, L, P]),
     'mdf_int2c2e': (I, [P, P, P, P, L, P]),
     'mdf_k_step1': (I, [P, I, L, L, L, P, I, P, L, I, P, L, L, P]),
+    'mdf_k_step1_lower': (I, [P, I, L, L, L, P, I, P, L, I, P, L, L, P]),
     'mdf_int3c2e_ip': (I, [P, P, P, P, P, P, P, L, P, P, P, I, P, P, P]),
     'mdf_int2c2e_ip': (I, [P, P, P, P, L, P, I, P, P]),
     'mdf_int1e': (I, [P, I, P, P, I, P, P, P, P, P, P]),

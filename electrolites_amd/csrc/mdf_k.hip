@@ -33,7 +33,7 @@ namespace {
 // 1-D grid of round_xcd(nkt * nit * nL) workgroups in XCD order; tile
 // (k-tile fastest, then i-tile, then aux row) so the k-tiles of one A tile
 // share an L2.
-template <int BN, bool MAPPED, typename T, typename Src>
+template <int BN, bool MAPPED, typename T, typename Src, bool TRI = false>
 __global__ __launch_bounds__(Step1Shape<BN>::THREADS) void k_step1(
     const Src *__restrict__ cderi, long ld, const int *__restrict__ pmap, int nao,
     const double *__restrict__ F, long ldf, int nocc, T *__restrict__ R, long ldr,
@@ -43,8 +43,8 @@ __global__ __launch_bounds__(Step1Shape<BN>::THREADS) void k_step1(
     if (x >= ntiles) return;
     const int kt = (int)(x % nkt), it = (int)((x / nkt) % nit);
     const long L = x / ((long)nkt * nit);
-    step1_tile<BN, MAPPED, T, Src>(cderi + L * ld, pmap, nao, F, ldf, nocc, R + L * strideL, ldr, it * 128, kt * BN,
-                              lds);
+    step1_tile<BN, MAPPED, T, Src, TRI>(cderi + L * ld, pmap, nao, F, ldf, nocc, R + L * strideL, ldr, it * 128,
+                                        kt * BN, lds);
 }
 
 // 1-D grid of ntiles_grid * ksplit workgroups, (tile, split) by step2_coords.
@@ -133,30 +133,30 @@ __global__ __launch_bounds__(G_THREADS, 2) void k_step2_b3(
                     part + (long)split * part_stride, beta, lds);
 }
 
-template <int BN, typename T, typename Src>
+template <int BN, typename T, typename Src, bool TRI = false>
 void launch_step1(int nL, hipStream_t st, const Src *blk, long ld, const int *pmap, int nao,
                   const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
     const int nkt = (int)cdiv(nocc, BN), nit = (int)cdiv(nao, 128);
     const long ntiles = (long)nkt * nit * nL;
     const dim3 g((unsigned)round_xcd(ntiles)), b(Step1Shape<BN>::THREADS);
     if (pmap)
-        hipLaunchKernelGGL((k_step1<BN, true, T, Src>), g, b, 0, st, blk, ld, pmap, nao, f, ldf,
+        hipLaunchKernelGGL((k_step1<BN, true, T, Src, TRI>), g, b, 0, st, blk, ld, pmap, nao, f, ldf,
                            nocc, r, ldr, strideL, nkt, nit, ntiles);
     else
-        hipLaunchKernelGGL((k_step1<BN, false, T, Src>), g, b, 0, st, blk, ld, pmap, nao, f, ldf,
+        hipLaunchKernelGGL((k_step1<BN, false, T, Src, TRI>), g, b, 0, st, blk, ld, pmap, nao, f, ldf,
                            nocc, r, ldr, strideL, nkt, nit, ntiles);
 }
 
-template <typename T, typename Src>
+template <typename T, typename Src, bool TRI = false>
 int step1(const KPlan &p, int nL, hipStream_t st, const Src *blk, long ld, const int *pmap, int nao,
           const double *f, long ldf, int nocc, T *r, long ldr, long strideL) {
     switch (p.bn) {
-    case 320: launch_step1<320, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    case 256: launch_step1<256, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    case 192: launch_step1<192, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    case 160: launch_step1<160, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    case 128: launch_step1<128, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
-    default: launch_step1<64, T, Src>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 320: launch_step1<320, T, Src, TRI>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 256: launch_step1<256, T, Src, TRI>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 192: launch_step1<192, T, Src, TRI>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 160: launch_step1<160, T, Src, TRI>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    case 128: launch_step1<128, T, Src, TRI>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
+    default: launch_step1<64, T, Src, TRI>(nL, st, blk, ld, pmap, nao, f, ldf, nocc, r, ldr, strideL); break;
     }
     MDF_LAUNCHED("k_step1");
     return MDF_OK;
@@ -356,25 +356,42 @@ extern "C" int mdf_get_k(const void *cderi, int cderi_dtype, long naux, long npa
 // of one block (A_L the unpacked symmetric row L), R[L * strideL + i * ldr +
 // k] for i < nao, k < nocc.  The DF gradient's rhok = F_r^T A_L F_l takes its
 // inner product from here.
-extern "C" int mdf_k_step1(const void *cderi, int cderi_dtype, long nL, long npairs, long ld, const int *pair_map,
-                           int nao, const double *f, long ldf, int nocc, double *R, long ldr, long strideL,
-                           void *stream) {
-    MDF_CHECK_ARG(nL >= 0 && nao >= 0 && nocc >= 0 && nL < (1L << 31), "mdf_k_step1: bad sizes");
-    MDF_CHECK_ARG(cderi && f && R, "mdf_k_step1: null pointer");
+namespace {
+template <bool TRI>
+int k_step1_entry(const char *name, const void *cderi, int cderi_dtype, long nL, long npairs, long ld,
+                  const int *pair_map, int nao, const double *f, long ldf, int nocc, double *R, long ldr,
+                  long strideL, void *stream) {
+    MDF_CHECK_ARG(nL >= 0 && nao >= 0 && nocc >= 0 && nL < (1L << 31), "%s: bad sizes", name);
+    MDF_CHECK_ARG(cderi && f && R, "%s: null pointer", name);
     MDF_CHECK_ARG(ld >= npairs && ldf >= nocc && ldr >= nocc && strideL >= (long)nao * ldr,
-                  "mdf_k_step1: leading dimension too small");
-    MDF_CHECK_ARG(cderi_dtype == MDF_DTYPE_F64 || cderi_dtype == MDF_DTYPE_F32, "mdf_k_step1: cderi dtype %d",
+                  "%s: leading dimension too small", name);
+    MDF_CHECK_ARG(cderi_dtype == MDF_DTYPE_F64 || cderi_dtype == MDF_DTYPE_F32, "%s: cderi dtype %d", name,
                   cderi_dtype);
     MDF_CHECK_ARG(pair_map || npairs == (long)nao * (nao + 1) / 2,
-                  "mdf_k_step1: without a pair map the cderi must hold the full packed triangle "
-                  "(npairs=%ld, nao=%d)", npairs, nao);
+                  "%s: without a pair map the cderi must hold the full packed triangle "
+                  "(npairs=%ld, nao=%d)", name, npairs, nao);
     if (nL == 0 || nao == 0 || nocc == 0) return MDF_OK;
     KPlan p{};
     p.bn = pick_bn(nocc);
     hipStream_t st = as_stream(stream);
     if (cderi_dtype == MDF_DTYPE_F32)
-        return step1<double, float>(p, (int)nL, st, (const float *)cderi, ld, pair_map, nao, f, ldf, nocc, R, ldr,
-                                    strideL);
-    return step1<double, double>(p, (int)nL, st, (const double *)cderi, ld, pair_map, nao, f, ldf, nocc, R, ldr,
-                                 strideL);
+        return step1<double, float, TRI>(p, (int)nL, st, (const float *)cderi, ld, pair_map, nao, f, ldf, nocc, R,
+                                         ldr, strideL);
+    return step1<double, double, TRI>(p, (int)nL, st, (const double *)cderi, ld, pair_map, nao, f, ldf, nocc, R,
+                                      ldr, strideL);
+}
+}  // namespace
+
+extern "C" int mdf_k_step1_lower(const void *cderi, int cderi_dtype, long nL, long npairs, long ld,
+                                 const int *pair_map, int nao, const double *f, long ldf, int nocc, double *R,
+                                 long ldr, long strideL, void *stream) {
+    return k_step1_entry<true>("mdf_k_step1_lower", cderi, cderi_dtype, nL, npairs, ld, pair_map, nao, f, ldf,
+                               nocc, R, ldr, strideL, stream);
+}
+
+extern "C" int mdf_k_step1(const void *cderi, int cderi_dtype, long nL, long npairs, long ld, const int *pair_map,
+                           int nao, const double *f, long ldf, int nocc, double *R, long ldr, long strideL,
+                           void *stream) {
+    return k_step1_entry<false>("mdf_k_step1", cderi, cderi_dtype, nL, npairs, ld, pair_map, nao, f, ldf, nocc, R,
+                                ldr, strideL, stream);
 }

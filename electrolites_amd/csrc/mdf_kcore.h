@@ -19,8 +19,9 @@ __device__ __forceinline__ long tri(long i) { return i * (i + 1) / 2; }
 
 // ---------------------------------------------------------------- step 1
 // A operand of step 1: rows i0..i0+127 of the unpacked A_L, K-step kt covers
-// j = 16 kt .. 16 kt + 15.
-template <bool MAPPED, typename T = double, typename Src = double, int THREADS = G_THREADS>
+// j = 16 kt .. 16 kt + 15.  TRI: the lower triangle of A_L with its
+// diagonal halved (A_L = T + T^T), zero above the diagonal.
+template <bool MAPPED, typename T = double, typename Src = double, int THREADS = G_THREADS, bool TRI = false>
 struct PackedALoader {
     using reg_t = Src;                                   // the cderi storage type
     static constexpr int N = 128 * G_BK / THREADS;       // 8 (256 threads) or 4 (512)
@@ -36,7 +37,8 @@ struct PackedALoader {
     __device__ __forceinline__ void map_step(int kt, int (&p)[N]) const {
         const int t = threadIdx.x, i = i0 + t / TPR, jb = kt * G_BK + (t % TPR) * N;
 #pragma unroll
-        for (int v = 0; v < N; v++) p[v] = (i < nao && jb + v < nao) ? pmap[(long)i * nao + jb + v] : -1;
+        for (int v = 0; v < N; v++)
+            p[v] = (i < nao && jb + v < nao && (!TRI || jb + v <= i)) ? pmap[(long)i * nao + jb + v] : -1;
     }
     __device__ __forceinline__ void load(int kt, Src (&r)[N]) {
         const int t = threadIdx.x, j0 = kt * G_BK;
@@ -47,7 +49,21 @@ struct PackedALoader {
             if (kt == 0) map_step(0, pm);
 #pragma unroll
             for (int v = 0; v < N; v++) r[v] = pm[v] >= 0 ? row[pm[v]] : (Src)0;
+            if (TRI) {
+                const int i = i0 + t / TPR, jb = j0 + (t % TPR) * N;
+#pragma unroll
+                for (int v = 0; v < N; v++)
+                    if (jb + v == i) r[v] *= (Src)0.5;
+            }
             map_step(kt + 1, pm);
+            return;
+        }
+        if (TRI && j0 >= i0 + 127) {
+            // above the diagonal: zero (the K-loop of a TRI tile stops at
+            // the diagonal block, so this is only a guard)
+            upper = 0;
+#pragma unroll
+            for (int v = 0; v < N; v++) r[v] = (Src)0;
             return;
         }
         if (!MAPPED && j0 >= i0 + 127) {
@@ -76,6 +92,8 @@ struct PackedALoader {
                 if (MAPPED) {
                     const int p = pmap[(long)i * nao + j];
                     x = p >= 0 ? row[p] : 0.;
+                } else if (TRI) {
+                    x = i > j ? row[tri(i) + j] : i == j ? (Src)0.5 * row[tri(i) + i] : (Src)0;
                 } else {
                     x = row[i >= j ? tri(i) + j : tri(j) + i];
                 }
@@ -110,17 +128,21 @@ struct Step1Shape {
 
 // One 128 x BN tile of step 1 for aux row `row` (the cderi row L): rows
 // i0.., columns (occupied index) k0.. of R_L.
-template <int BN, bool MAPPED, typename T, typename Src = double>
+// TRI: A_L replaced by its lower triangle with the diagonal halved, and the
+// K loop stops after the tile's diagonal block (about half the work; for a
+// symmetric bilinear form F^T A_L F = G + G^T with G = F^T (TRI product)).
+template <int BN, bool MAPPED, typename T, typename Src = double, bool TRI = false>
 __device__ __forceinline__ void step1_tile(const Src *__restrict__ row, const int *__restrict__ pmap, int nao,
                                   const double *__restrict__ F, long ldf, int nocc, T *__restrict__ Rl, long ldr,
                                   int i0, int k0, T *lds) {
     using C = Step1Shape<BN>;
     using S = typename C::S;
-    PackedALoader<MAPPED, T, Src, C::THREADS> la{row, pmap, nao, i0, 0};
+    PackedALoader<MAPPED, T, Src, C::THREADS, TRI> la{row, pmap, nao, i0, 0};
     ColsLoader<BN, T, double, C::THREADS> lb{F, ldf, k0, nocc, 0, nao};
     Acc<S::MI, S::NI, T> acc;
     acc.zero();
-    gemm_mainloop<128, BN, decltype(la), decltype(lb), T, 2, C::WGN>(la, lb, (nao + G_BK - 1) / G_BK, lds, acc);
+    const int jend = TRI ? min(nao, i0 + 128) : nao;
+    gemm_mainloop<128, BN, decltype(la), decltype(lb), T, 2, C::WGN>(la, lb, (jend + G_BK - 1) / G_BK, lds, acc);
     auto emit = [&](int r, int c, T v) {
         const int i = i0 + r, k = k0 + c;
         if (i < nao && k < nocc) Rl[(long)i * ldr + k] = v;

@@ -227,29 +227,41 @@ def _rho_packed(dfobj, cderi, D, nao, nrow):
               _dev.ptr(d), 1, npairs, _dev.ptr(rho), nrow, 0, _dev.ptr(work), nbytes, _dev.stream_ptr())
     return rho
 
+def _same_factor(fl, fr):
+    return fl.data_ptr() == fr.data_ptr() and fl.shape == fr.shape and fl.stride() == fr.stride()
+
+
 def _rhok(dfobj, fl, fr, nao, nrow):
-    """``rhok[L] = F_r^T A_L F_l`` for every cderi row L."""
+    """``rhok[L] = F_r^T A_L F_l`` for every cderi row L.  With one factor
+    (F_l = F_r = F) the form is symmetric: A_L = T_L + T_L^T with T_L its
+    lower triangle, diagonal halved, so rhok[L] = G + G^T with G = F^T T_L F,
+    and T_L F costs about half of A_L F (``mdf_k_step1_lower``)."""
     t = _dev.torch()
     nocc = fl.shape[1]
     rhok = t.empty((nrow, nocc, nocc), dtype=t.float64, device=_dev.device())
     frT = fr.T.contiguous()
     cderi = fastdf._cderi_of(dfobj)
     if _on_device(cderi):
-        # R_L = A_L F_l by the K engine's first step, straight from the packed rows
+        # R_L = A_L F_l (or T_L F) by the K engine's first step, straight
+        # from the packed rows
         if cderi.shape[0] != nrow:
             raise ValueError(f'jk_energy_per_atom: the cderi has {cderi.shape[0]} rows, the metric factor {nrow}')
+        sym = _same_factor(fl, fr)
         lay = fastdf._layout(dfobj, cderi, nao)
         nb = max(1, min(nrow, RBLOCK_BYTES // max(8 * nao * nocc, 1)))
         R = t.empty((nb, nao, nocc), dtype=t.float64, device=_dev.device())
         code = fastdf._dtype_code(cderi)
         for L0 in range(0, nrow, nb):
             nL = min(nb, nrow - L0)
-            _lib.call('mdf_k_step1', _dev.ptr(cderi[L0]), code, nL, cderi.shape[1], cderi.stride(0),
-                      _dev.ptr(lay.map), nao, _dev.ptr(fl), fl.stride(0), nocc, _dev.ptr(R), nocc, nao * nocc,
-                      _dev.stream_ptr())
-            # rhok[L] = F_r^T R_L, one batched product over the block's rows
+            _lib.call('mdf_k_step1_lower' if sym else 'mdf_k_step1', _dev.ptr(cderi[L0]), code, nL,
+                      cderi.shape[1], cderi.stride(0), _dev.ptr(lay.map), nao, _dev.ptr(fl), fl.stride(0), nocc,
+                      _dev.ptr(R), nocc, nao * nocc, _dev.stream_ptr())
+            # F_r^T R_L, one batched product over the block's rows
             _gemm_batched(nocc, nocc, nao, frT.data_ptr(), nao, 0, R.data_ptr(), nocc, nao * nocc,
                           rhok[L0].data_ptr(), nocc, nocc * nocc, nL)
+            if sym:
+                blk = rhok[L0:L0 + nL]
+                blk += blk.transpose(1, 2).clone()
         return rhok
     L0 = 0
     for dense, _ in dfobj.loop():
@@ -382,7 +394,7 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
             # one factor (F_l = F_r): every rhok_L and dmoo_P is symmetric,
             # so A^T rhok and tr(dmoo_P dmoo_Q) run on the packed upper
             # triangles (half the flops of both products)
-            sym = fl.data_ptr() == fr.data_ptr() and fl.shape == fr.shape and fl.stride() == fr.stride()
+            sym = _same_factor(fl, fr)
             if sym:
                 dmoo, iu = _dmoo_packed(A, lower, At, rhoks[si], naux)
                 rhoks[si] = None

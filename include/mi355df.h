@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 9
+#define MDF_ABI_VERSION 10
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
@@ -245,6 +245,15 @@ int mdf_int2c2e(const mdf_basis *aux, const int *aux_sorted, const long *aux_l_o
  * ``rhok`` from the cderi, :665-714) takes its inner product from R. */
 int mdf_k_step1(const void *cderi, int cderi_dtype, long nL, long npairs, long ld, const int *pair_map,
                 int nao, const double *f, long ldf, int nocc, double *R, long ldr, long strideL, void *stream);
+/* mdf_k_step1_lower: as mdf_k_step1 with A_L replaced by its lower triangle,
+ *   diagonal halved (T_L with A_L = T_L + T_L^T): R_L = T_L F, about half
+ *   the work (each 128-row tile's K loop stops at its diagonal block).  For
+ *   the symmetric bilinear form of the DF gradient, F^T A_L F = G + G^T with
+ *   G = F^T R_L (electrolites/fastdfgrad.py:870-1182, rhok there from the
+ *   full product). */
+int mdf_k_step1_lower(const void *cderi, int cderi_dtype, long nL, long npairs, long ld, const int *pair_map,
+                      int nao, const double *f, long ldf, int nocc, double *R, long ldr, long strideL,
+                      void *stream);
 
 /* Nuclear derivative of the DF gradient's three-centre term:
  *   grad[3a + x]     += sum_{P,i,j} W[P, (i,j)] d(ij|P)/dR_{a,x}  over the

@@ -184,7 +184,9 @@ def test_patched_method_runs_the_hip_path(mol):
 def test_k_step1_entry(nao, naux, nocc, nL, screened, dtype):
     """mdf_k_step1 (K's first step alone, the gradient's R_L = A_L F) equals
     the unpacked rows times F (numpy) on ragged shapes, full-triangle and
-    screened pair lists, FP64 and FP32 storage (widened exactly)."""
+    screened pair lists, FP64 and FP32 storage (widened exactly); and
+    mdf_k_step1_lower equals T_L F with T_L the lower triangle of A_L,
+    diagonal halved (A_L = T_L + T_L^T)."""
     import torch
     from electrolites_amd import _dev, _lib, fastdf
     rng = np.random.default_rng(nao + naux)
@@ -202,13 +204,17 @@ def test_k_step1_entry(nao, naux, nocc, nL, screened, dtype):
     c = DFo._cderi[0]
     lay = fastdf._layout(DFo(), c, nao)
     fd = torch.from_numpy(f).cuda()
-    R = torch.full((nL, nao, nocc + 3), np.nan, dtype=torch.float64, device='cuda')
-    _lib.call('mdf_k_step1', _dev.ptr(c), fastdf._dtype_code(c), nL, c.shape[1], c.stride(0),
-              _dev.ptr(lay.map), nao, _dev.ptr(fd), fd.stride(0), nocc, _dev.ptr(R), nocc + 3,
-              nao * (nocc + 3), _dev.stream_ptr())
-    got = R.cpu().numpy()
-    assert np.all(np.isnan(got[..., nocc:]))              # nothing written past nocc
-    assert rel(got[..., :nocc], ref) < 1e-13
+    A = O.unpack(cderi.astype(np.float64), pa, nao)[:nL]
+    T = np.tril(A, -1) + .5 * A * np.eye(nao)
+    assert np.allclose(T + T.transpose(0, 2, 1), A)
+    for entry, want in (('mdf_k_step1', ref), ('mdf_k_step1_lower', np.matmul(T, f))):
+        R = torch.full((nL, nao, nocc + 3), np.nan, dtype=torch.float64, device='cuda')
+        _lib.call(entry, _dev.ptr(c), fastdf._dtype_code(c), nL, c.shape[1], c.stride(0),
+                  _dev.ptr(lay.map), nao, _dev.ptr(fd), fd.stride(0), nocc, _dev.ptr(R), nocc + 3,
+                  nao * (nocc + 3), _dev.stream_ptr())
+        got = R.cpu().numpy()
+        assert np.all(np.isnan(got[..., nocc:]))              # nothing written past nocc
+        assert rel(got[..., :nocc], want) < 1e-13, entry
 
 
 @pytest.mark.gpu
