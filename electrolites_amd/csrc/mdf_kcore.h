@@ -36,6 +36,25 @@ struct PackedALoader {
     int pm[N];
     __device__ __forceinline__ void map_step(int kt, int (&p)[N]) const {
         const int t = threadIdx.x, i = i0 + t / TPR, jb = kt * G_BK + (t % TPR) * N;
+        if ((nao & 3) == 0 && i < nao && jb + N <= nao) {
+            // a thread's N map entries as 16-byte loads (N is 4 or 8, jb a
+            // multiple of 4, so the row start i nao + jb is one too)
+            const int4 *src = reinterpret_cast<const int4 *>(pmap + (long)i * nao + jb);
+#pragma unroll
+            for (int q = 0; q < N / 4; q++) {
+                const int4 m = src[q];
+                p[4 * q] = m.x;
+                p[4 * q + 1] = m.y;
+                p[4 * q + 2] = m.z;
+                p[4 * q + 3] = m.w;
+            }
+            if (TRI) {
+#pragma unroll
+                for (int v = 0; v < N; v++)
+                    if (jb + v > i) p[v] = -1;
+            }
+            return;
+        }
 #pragma unroll
         for (int v = 0; v < N; v++)
             p[v] = (i < nao && jb + v < nao && (!TRI || jb + v <= i)) ? pmap[(long)i * nao + jb + v] : -1;
@@ -440,6 +459,19 @@ struct PackedA32 {
             for (int v = 0; v < N; v++) p[v] = (i < nao && jb + v < nao) ? pmap[(long)(jb + v) * nao + i] : -1;
         } else {
             const int i = i0 + t / TPR, jb = j0 + (t % TPR) * N;
+            if ((nao & 3) == 0 && i < nao && jb + N <= nao) {
+                // N (a multiple of 4) map entries as 16-byte loads
+                const int4 *src = reinterpret_cast<const int4 *>(pmap + (long)i * nao + jb);
+#pragma unroll
+                for (int q = 0; q < N / 4; q++) {
+                    const int4 m = src[q];
+                    p[4 * q] = m.x;
+                    p[4 * q + 1] = m.y;
+                    p[4 * q + 2] = m.z;
+                    p[4 * q + 3] = m.w;
+                }
+                return;
+            }
 #pragma unroll
             for (int v = 0; v < N; v++) p[v] = (i < nao && jb + v < nao) ? pmap[(long)i * nao + jb + v] : -1;
         }
