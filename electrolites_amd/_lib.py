@@ -9,7 +9,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, 'libmi355df.so')
-ABI_VERSION = 10
+ABI_VERSION = 11
 DTYPE_F64, DTYPE_F32 = 0, 1
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -50,7 +50,9 @@ _SIGS = {
     'mdf_int1e': (I, [P, I, P, P, I, P, P, P, P, P, P]),
     'mdf_gemm': (I, [I, I, I, L, L, L, D, P, L, P, L, D, P, L, P]),
     'mdf_gemm_batched': (I, [I, I, I, L, L, L, D, P, L, L, P, L, L, D, P, L, L, L, P]),
-    'mdf_gemm_pairs_batched': (I, [I, L, L, D, P, L, L, P, L, L, P, P, L, L, P]),
+    'mdf_gemm_pairs_batched': (I, [I, L, L, D, P, L, L, P, L, L, P, P, L, P, P, L, P]),
+    'mdf_sym_pack': (I, [L, P, L, P, L, D, D, L, P]),
+    'mdf_sym_unpack': (I, [L, P, L, P, L, D, D, L, P]),
     'mdf_potrf_workspace': (SZ, [L]),
     'mdf_potrf': (I, [P, L, L, P, P, SZ, P]),
     'mdf_trtri_workspace': (SZ, [L]),

@@ -104,14 +104,20 @@ __global__ __launch_bounds__(512) void gemm_f64_wide(int M, int N, int K, double
     gemm_epilogue_layout<128, 320, double, decltype(acc), decltype(emit) &, 2, 4>(acc, emit);
 }
 
-// The lower 128-tiles of P_z = A_z op(B_z) (n x n), added at the stored
-// pairs: W[z ldw + pcol[i (i + 1) / 2 + j]] += alpha P_z[i][j] for
-// n > i >= j with pcol >= 0.  No P_z is written.
+// The lower 128-tiles of P_z = A_z op(B_z) (n x n), put at the stored
+// pairs: W[z ldw + c] = base + alpha P_z[i][j] with c = pcol[i (i + 1) / 2 + j]
+// for n > i >= j with c >= 0, base = iscale[z] ivals[c] when iscale is given
+// (W need not hold anything), else W's own value.  No P_z is written.  Every
+// column of the tile is looked up first, then every base value is loaded
+// (all loads in flight together: the columns of one tile are distinct, so
+// no load depends on a store of the same tile), then every store issues.
 template <bool BT>
 __global__ __launch_bounds__(G_THREADS, 2) void gemm_pairs_kernel(int n, int K, double alpha,
                                                                   const double *__restrict__ A, long lda, long sA,
                                                                   const double *__restrict__ B, long ldb, long sB,
-                                                                  const int *__restrict__ pcol, double *W, long ldw) {
+                                                                  const int *__restrict__ pcol, double *W, long ldw,
+                                                                  const double *__restrict__ iscale,
+                                                                  const double *__restrict__ ivals) {
     using S = TileShape<128, 128>;
     __shared__ double lds[S::LDS_DOUBLES];
     A += (long)blockIdx.z * sA;
@@ -131,13 +137,119 @@ __global__ __launch_bounds__(G_THREADS, 2) void gemm_pairs_kernel(int n, int K, 
         ColsLoader<128> lb{B, ldb, n0, n, 0, K};
         gemm_mainloop<128, 128>(la, lb, nk, lds, acc);
     }
-    gemm_epilogue<128, 128>(acc, [&](int r, int c, double v) {
-        const int i = m0 + r, j = n0 + c;
-        if (i < n && j <= i) {
-            const int col = pcol[(long)i * (i + 1) / 2 + j];
-            if (col >= 0) W[col] += alpha * v;
+    // per row group of 16 x 64 elements: every column's byte offset
+    // (32-bit, so the loads and stores take the uniform row base plus one
+    // register each; ~0 for pairs not stored), then every base value (the
+    // loads in flight together: the columns of one tile are distinct), then
+    // every store
+    constexpr unsigned NONE = ~0u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave % 2, wn = wave / 2;
+    const double s = iscale ? iscale[blockIdx.z] : 0.;
+    const char *vb = (const char *)ivals;
+    char *wb = (char *)W;
+#pragma unroll
+    for (int mi = 0; mi < S::MI; mi++) {
+        unsigned off[S::NI][4];
+#pragma unroll
+        for (int ni = 0; ni < S::NI; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = m0 + wm * S::WM + mi * 16 + Mfma<double>::row(lane, v);
+                const int j = n0 + wn * S::WN + ni * 16 + (lane & 15);
+                // the packed index's byte offset fits 32 bits (checked at entry)
+                const unsigned po = (unsigned)(i * (i + 1) / 2 + j) * 4u;
+                const int c = (i < n && j <= i) ? *(const int *)((const char *)pcol + po) : -1;
+                off[ni][v] = c >= 0 ? (unsigned)c * 8u : NONE;
+            }
+#pragma unroll
+        for (int ni = 0; ni < S::NI; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const unsigned o = off[ni][v];
+                double base = 0.;
+                if (o != NONE) base = iscale ? s * *(const double *)(vb + o) : *(const double *)(wb + o);
+                acc.v[mi][ni][v] = base + alpha * acc.v[mi][ni][v];
+            }
+#pragma unroll
+        for (int ni = 0; ni < S::NI; ni++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (off[ni][v] != NONE) *(double *)(wb + off[ni][v]) = acc.v[mi][ni][v];
+    }
+}
+
+// Packed upper triangles (row-major over a <= b: the offset of (a, b) is
+// a n - a (a - 1) / 2 + b - a) of n x n matrices, by 32 x 32 tile pairs
+// (ta <= tb) staged in LDS so that both the rows and the columns of G are
+// read along rows.  blockIdx.x is the tile pair, blockIdx.y strides the
+// batch.
+constexpr int SY_T = 32;
+
+__device__ __forceinline__ long upper_offset(long n, long a, long b) { return a * n - a * (a - 1) / 2 + (b - a); }
+
+__device__ __forceinline__ void upper_tile(int x, int &ta, int &tb) {
+    lower_tile(x, tb, ta);   // ta <= tb
+}
+
+// P_z[(a, b)] = s_ab (G_z[a][b] + G_z[b][a]), s = dscale on the diagonal,
+// oscale off it.
+__global__ __launch_bounds__(256) void sym_pack_kernel(int n, const double *__restrict__ G, long sG,
+                                                       double *__restrict__ P, long sP, long batch, double dscale,
+                                                       double oscale) {
+    __shared__ double tab[SY_T][SY_T + 1], tba[SY_T][SY_T + 1];
+    int ta, tb;
+    upper_tile(blockIdx.x, ta, tb);
+    const int a0 = ta * SY_T, b0 = tb * SY_T;
+    for (long z = blockIdx.y; z < batch; z += gridDim.y) {
+        const double *g = G + z * sG;
+        double *p = P + z * sP;
+#pragma unroll
+        for (int e = threadIdx.x; e < SY_T * SY_T; e += 256) {
+            const int r = e / SY_T, c = e % SY_T;
+            tab[r][c] = (a0 + r < n && b0 + c < n) ? g[(long)(a0 + r) * n + b0 + c] : 0.;
+            tba[r][c] = (b0 + r < n && a0 + c < n) ? g[(long)(b0 + r) * n + a0 + c] : 0.;
         }
-    });
+        __syncthreads();
+#pragma unroll
+        for (int e = threadIdx.x; e < SY_T * SY_T; e += 256) {
+            const int r = e / SY_T, c = e % SY_T;
+            const int a = a0 + r, b = b0 + c;
+            if (a <= b && b < n) p[upper_offset(n, a, b)] = (a == b ? dscale : oscale) * (tab[r][c] + tba[c][r]);
+        }
+        __syncthreads();
+    }
+}
+
+// G_z[a][b] = G_z[b][a] = s_ab P_z[(a, b)] for a <= b.
+__global__ __launch_bounds__(256) void sym_unpack_kernel(int n, const double *__restrict__ P, long sP,
+                                                         double *__restrict__ G, long sG, long batch, double dscale,
+                                                         double oscale) {
+    __shared__ double t[SY_T][SY_T + 1];
+    int ta, tb;
+    upper_tile(blockIdx.x, ta, tb);
+    const int a0 = ta * SY_T, b0 = tb * SY_T;
+    for (long z = blockIdx.y; z < batch; z += gridDim.y) {
+        const double *p = P + z * sP;
+        double *g = G + z * sG;
+#pragma unroll
+        for (int e = threadIdx.x; e < SY_T * SY_T; e += 256) {
+            const int r = e / SY_T, c = e % SY_T;
+            const int a = a0 + r, b = b0 + c;
+            t[r][c] = (a <= b && b < n) ? (a == b ? dscale : oscale) * p[upper_offset(n, a, b)] : 0.;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = threadIdx.x; e < SY_T * SY_T; e += 256) {
+            const int r = e / SY_T, c = e % SY_T;
+            // rows of the a-block: (a0 + r, b0 + c), from the tile or, on a
+            // diagonal tile below its diagonal, from the mirrored entry
+            if (a0 + r < n && b0 + c < n) g[(long)(a0 + r) * n + b0 + c] = (ta < tb || r <= c) ? t[r][c] : t[c][r];
+            // rows of the b-block: (b0 + r, a0 + c) = (a0 + c, b0 + r) mirrored
+            if (ta < tb && b0 + r < n && a0 + c < n) g[(long)(b0 + r) * n + a0 + c] = t[c][r];
+        }
+        __syncthreads();
+    }
 }
 
 constexpr long MAX_GRID_Z = 65535;
@@ -282,10 +394,12 @@ extern "C" int mdf_gemm_batched(int transb, int tri_a, int lower_c, long M, long
 
 extern "C" int mdf_gemm_pairs_batched(int transb, long n, long K, double alpha, const double *A, long lda,
                                       long strideA, const double *B, long ldb, long strideB, const int *pair_col,
-                                      double *W, long ldw, long batch, void *stream) {
+                                      double *W, long ldw, const double *init_scale, const double *init_vals,
+                                      long batch, void *stream) {
     MDF_CHECK_ARG(n >= 0 && K >= 0 && n < (1L << 31) && K < (1L << 31) && batch >= 0 &&
-                  n * (n + 1) / 2 < (1L << 31), "mdf_gemm_pairs_batched: bad sizes");
+                  n * (n + 1) / 2 < (1L << 30) && ldw < (1L << 29), "mdf_gemm_pairs_batched: bad sizes");
     MDF_CHECK_ARG((A && B && pair_col && W) || batch == 0 || n == 0, "mdf_gemm_pairs_batched: null pointer");
+    MDF_CHECK_ARG(!init_scale == !init_vals, "mdf_gemm_pairs_batched: init_scale and init_vals go together");
     MDF_CHECK_ARG(lda >= K && ldb >= (transb ? K : n) && strideA >= 0 && strideB >= 0 && ldw >= 0,
                   "mdf_gemm_pairs_batched: leading dimension or stride");
     if (n == 0 || batch == 0) return MDF_OK;
@@ -294,16 +408,56 @@ extern "C" int mdf_gemm_pairs_batched(int transb, long n, long K, double alpha, 
     for (long b0 = 0; b0 < batch; b0 += MAX_GRID_Z) {
         const unsigned nz = (unsigned)(batch - b0 < MAX_GRID_Z ? batch - b0 : MAX_GRID_Z);
         const dim3 g((unsigned)(t * (t + 1) / 2), 1, nz);
+        const double *isc = init_scale ? init_scale + b0 : nullptr;
         if (transb)
             hipLaunchKernelGGL(gemm_pairs_kernel<true>, g, dim3(G_THREADS), 0, st, (int)n, (int)K, alpha,
                                A + b0 * strideA, lda, strideA, B + b0 * strideB, ldb, strideB, pair_col, W + b0 * ldw,
-                               ldw);
+                               ldw, isc, init_vals);
         else
             hipLaunchKernelGGL(gemm_pairs_kernel<false>, g, dim3(G_THREADS), 0, st, (int)n, (int)K, alpha,
                                A + b0 * strideA, lda, strideA, B + b0 * strideB, ldb, strideB, pair_col, W + b0 * ldw,
-                               ldw);
+                               ldw, isc, init_vals);
         MDF_LAUNCHED("gemm_pairs_kernel");
     }
+    return MDF_OK;
+}
+
+// Sizes of mdf_sym_pack / mdf_sym_unpack: n x n matrices of src_min / dst_min
+// elements, one per batch stride.
+static int sym_check(const char *name, long n, const void *src, long s_src, long src_min, const void *dst,
+                     long s_dst, long dst_min, long batch) {
+    MDF_CHECK_ARG(n >= 0 && batch >= 0 && n < (1L << 20), "%s: bad sizes", name);
+    if (n == 0 || batch == 0) return MDF_OK;
+    MDF_CHECK_ARG(src && dst, "%s: null pointer", name);
+    MDF_CHECK_ARG(s_src >= 0 && s_dst >= 0 && (batch == 1 || (s_src >= src_min && s_dst >= dst_min)),
+                  "%s: batch stride below one matrix", name);
+    return MDF_OK;
+}
+
+static dim3 sym_grid(long n, long batch) {
+    const long t = (n + SY_T - 1) / SY_T;
+    return dim3((unsigned)(t * (t + 1) / 2), (unsigned)(batch < 65535 ? batch : 65535));
+}
+
+extern "C" int mdf_sym_pack(long n, const double *G, long strideG, double *P, long strideP, double diag_scale,
+                            double off_scale, long batch, void *stream) {
+    const long np = n * (n + 1) / 2;
+    int rc = sym_check("mdf_sym_pack", n, G, strideG, n * n, P, strideP, np, batch);
+    if (rc || n == 0 || batch == 0) return rc;
+    hipLaunchKernelGGL(sym_pack_kernel, sym_grid(n, batch), dim3(256), 0, as_stream(stream), (int)n, G, strideG, P,
+                       strideP, batch, diag_scale, off_scale);
+    MDF_LAUNCHED("sym_pack_kernel");
+    return MDF_OK;
+}
+
+extern "C" int mdf_sym_unpack(long n, const double *P, long strideP, double *G, long strideG, double diag_scale,
+                              double off_scale, long batch, void *stream) {
+    const long np = n * (n + 1) / 2;
+    int rc = sym_check("mdf_sym_unpack", n, P, strideP, np, G, strideG, n * n, batch);
+    if (rc || n == 0 || batch == 0) return rc;
+    hipLaunchKernelGGL(sym_unpack_kernel, sym_grid(n, batch), dim3(256), 0, as_stream(stream), (int)n, P, strideP, G,
+                       strideG, batch, diag_scale, off_scale);
+    MDF_LAUNCHED("sym_unpack_kernel");
     return MDF_OK;
 }
 

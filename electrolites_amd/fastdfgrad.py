@@ -33,11 +33,17 @@ held in the cderi's own compact column layout (one value per stored pair).
 
 R_L = A_L F_l comes from the K engine's first step (``mdf_k_step1``, the
 packed rows read through the pair map) for a device-resident cderi, and from
-the unpacked blocks of :meth:`DF.loop` otherwise; the remaining small dense
-contractions (F_r^T R_L, the triangular products, W's blocks and Z) run as
-torch FP64 matrix products on the device.
+the unpacked blocks of :meth:`DF.loop` otherwise; the remaining contractions
+(F_r^T R_L, the triangular products, W's blocks and Z) run on the package's
+FP64 matrix-core GEMM (``mdf_gemm``, ``mdf_gemm_batched``,
+``mdf_gemm_pairs_batched``).  With one factor (F_l = F_r) every rhok_L and
+dmoo_P is symmetric and is kept as its packed upper triangle
+(``mdf_sym_pack`` / ``mdf_sym_unpack``) with the off-diagonal entries times
+sqrt 2, so that the dot product of two packed rows is the full trace
+product.
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -56,6 +62,8 @@ RBLOCK_BYTES = 1 << 31
 WBLOCK_BYTES = 1 << 30
 #: aux rows per block of the triangular metric products (host-stored path)
 TBLOCK = 1024
+#: off-diagonal weight of a packed symmetric block
+SQRT2 = math.sqrt(2.)
 
 
 def stats():
@@ -231,14 +239,40 @@ def _same_factor(fl, fr):
     return fl.data_ptr() == fr.data_ptr() and fl.shape == fr.shape and fl.stride() == fr.stride()
 
 
+def _sym_pack(G, P, n, batch, diag_scale, off_scale):
+    """``P[z] = packed upper triangle of s (G[z] + G[z]^T)`` (``mdf_sym_pack``)
+    for the rows z < batch of the 2-D P and the contiguous n x n blocks of G."""
+    assert P.dim() == 2 and P.shape[0] >= batch
+    _lib.call('mdf_sym_pack', n, _dev.ptr(G), n * n, _dev.ptr(P), P.stride(0), float(diag_scale),
+              float(off_scale), batch, _dev.stream_ptr())
+
+
+def _sym_unpack(P, G, n, batch, diag_scale, off_scale):
+    """``G[z] = G[z]^T = s P[z]`` unpacked (``mdf_sym_unpack``) from the rows
+    z < batch of the 2-D P into contiguous n x n blocks of G."""
+    assert P.dim() == 2 and P.shape[0] >= batch
+    _lib.call('mdf_sym_unpack', n, _dev.ptr(P), P.stride(0), _dev.ptr(G), n * n, float(diag_scale),
+              float(off_scale), batch, _dev.stream_ptr())
+
+
+def _diag_positions(n):
+    """Offsets of the diagonal entries in a packed upper triangle of order n."""
+    a = np.arange(n, dtype=np.int64)
+    return a * n - a * (a - 1) // 2
+
+
 def _rhok(dfobj, fl, fr, nao, nrow):
-    """``rhok[L] = F_r^T A_L F_l`` for every cderi row L.  With one factor
-    (F_l = F_r = F) the form is symmetric: A_L = T_L + T_L^T with T_L its
-    lower triangle, diagonal halved, so rhok[L] = G + G^T with G = F^T T_L F,
-    and T_L F costs about half of A_L F (``mdf_k_step1_lower``)."""
+    """``rhok[L] = F_r^T A_L F_l`` for every cderi row L, and whether it is
+    packed.  With one factor (F_l = F_r = F) the form is symmetric: A_L =
+    T_L + T_L^T with T_L its lower triangle, diagonal halved, so rhok[L] =
+    G + G^T with G = F^T T_L F, and T_L F costs about half of A_L F
+    (``mdf_k_step1_lower``); rhok then comes back as ``(nrow, nocc (nocc +
+    1) / 2)`` packed upper triangles, off-diagonal entries times sqrt 2."""
     t = _dev.torch()
     nocc = fl.shape[1]
-    rhok = t.empty((nrow, nocc, nocc), dtype=t.float64, device=_dev.device())
+    sym = _same_factor(fl, fr)
+    shape = (nrow, nocc * (nocc + 1) // 2) if sym else (nrow, nocc, nocc)
+    rhok = t.empty(shape, dtype=t.float64, device=_dev.device())
     frT = fr.T.contiguous()
     cderi = fastdf._cderi_of(dfobj)
     if _on_device(cderi):
@@ -246,10 +280,10 @@ def _rhok(dfobj, fl, fr, nao, nrow):
         # from the packed rows
         if cderi.shape[0] != nrow:
             raise ValueError(f'jk_energy_per_atom: the cderi has {cderi.shape[0]} rows, the metric factor {nrow}')
-        sym = _same_factor(fl, fr)
         lay = fastdf._layout(dfobj, cderi, nao)
         nb = max(1, min(nrow, RBLOCK_BYTES // max(8 * nao * nocc, 1)))
         R = t.empty((nb, nao, nocc), dtype=t.float64, device=_dev.device())
+        G = t.empty((nb, nocc, nocc), dtype=t.float64, device=_dev.device()) if sym else None
         code = fastdf._dtype_code(cderi)
         for L0 in range(0, nrow, nb):
             nL = min(nb, nrow - L0)
@@ -257,67 +291,48 @@ def _rhok(dfobj, fl, fr, nao, nrow):
                       cderi.shape[1], cderi.stride(0), _dev.ptr(lay.map), nao, _dev.ptr(fl), fl.stride(0), nocc,
                       _dev.ptr(R), nocc, nao * nocc, _dev.stream_ptr())
             # F_r^T R_L, one batched product over the block's rows
+            out = G if sym else rhok[L0]
             _gemm_batched(nocc, nocc, nao, frT.data_ptr(), nao, 0, R.data_ptr(), nocc, nao * nocc,
-                          rhok[L0].data_ptr(), nocc, nocc * nocc, nL)
+                          out.data_ptr(), nocc, nocc * nocc, nL)
             if sym:
-                blk = rhok[L0:L0 + nL]
-                blk += blk.transpose(1, 2).clone()
-        return rhok
+                _sym_pack(G, rhok[L0:L0 + nL], nocc, nL, 1., SQRT2)
+        return rhok, sym
     L0 = 0
     for dense, _ in dfobj.loop():
         nL = dense.shape[0]
-        rhok[L0:L0 + nL] = t.matmul(frT, t.matmul(dense, fl))
+        blk = t.matmul(frT, t.matmul(dense, fl))
+        if sym:
+            # the full symmetric rhok_L: pack (rhok + rhok^T) / 2
+            _sym_pack(blk.contiguous(), rhok[L0:L0 + nL], nocc, nL, .5, .5 * SQRT2)
+        else:
+            rhok[L0:L0 + nL] = blk
         L0 += nL
     if L0 != nrow:
         raise ValueError(f'jk_energy_per_atom: the cderi has {L0} rows, the metric factor {nrow}')
-    return rhok
+    return rhok, sym
 
 
 def _dmoo(A, lower, At, rhok, naux):
     """``dmoo[P] = sum_L A[L, P] rhok[L]`` = A^T rhok as one product; A^T is
     upper triangular on the Cholesky route (only L >= P contributes), and
-    the product skips the zero blocks."""
+    the product skips the zero blocks.  Packed rhok rows give packed dmoo
+    rows (the product is linear in each column)."""
     t = _dev.torch()
-    nrow, nocc = rhok.shape[0], rhok.shape[1]
-    rk = rhok.reshape(nrow, nocc * nocc)
-    dmoo = t.empty((naux, nocc * nocc), dtype=t.float64, device=_dev.device())
+    nrow = rhok.shape[0]
+    rk = rhok.reshape(nrow, -1)
+    dmoo = t.empty((naux, rk.shape[1]), dtype=t.float64, device=_dev.device())
     _gemm(At, rk, dmoo, tri_a=2 if lower else 0)
-    return dmoo.reshape(naux, nocc, nocc)
+    return dmoo
 
 
-def _upper(nocc):
-    """Flat indices ``a nocc + b`` and ``b nocc + a`` of the upper triangle
-    (a <= b, row-major) of an nocc x nocc matrix, and the weights that turn
-    a dot product of two packed symmetric matrices into the full one
-    (1 on the diagonal, 2 off it), as device tensors."""
-    t = _dev.torch()
-    a, b = t.triu_indices(nocc, nocc, device=_dev.device())
-    w = t.where(a == b, 1., 2.).to(t.float64)
-    return (a * nocc + b, b * nocc + a), w
-
-
-def _dmoo_packed(A, lower, At, rhok, naux):
-    """``A^T rhok`` on the packed upper triangles of the symmetric rhok_L:
-    ``(naux, nocc (nocc + 1) / 2)`` and the index pair of ``_upper``."""
-    t = _dev.torch()
-    nrow, nocc = rhok.shape[0], rhok.shape[1]
-    (iu, il), w = _upper(nocc)
-    rp = rhok.reshape(nrow, nocc * nocc)[:, iu]
-    dmoo = t.empty((naux, rp.shape[1]), dtype=t.float64, device=_dev.device())
-    _gemm(At, rp, dmoo, tri_a=2 if lower else 0)
-    return dmoo, (iu, il)
-
-
-def _trace_products_packed(dmoo, nocc):
-    """``tr(dmoo_P dmoo_Q)`` = sum_ab dmoo_P[a, b] dmoo_Q[a, b] (symmetric
-    dmoo) from the packed upper triangles, the off-diagonal terms weighted
-    by sqrt(2) on both sides: the lower 128-tiles of one product, mirrored."""
+def _trace_products_packed(dmoo):
+    """``tr(dmoo_P dmoo_Q)`` for symmetric dmoo from the packed rows (the
+    off-diagonal entries already times sqrt 2): the lower 128-tiles of one
+    product, mirrored."""
     t = _dev.torch()
     naux = dmoo.shape[0]
-    _, w = _upper(nocc)
-    xs = dmoo * w.sqrt()
     out = t.empty((naux, naux), dtype=t.float64, device=_dev.device())
-    _gemm(xs, xs, out, transb=True, lower_c=True)
+    _gemm(dmoo, dmoo, out, transb=True, lower_c=True)
     out = t.tril(out)
     return out + t.tril(out, -1).T
 
@@ -373,41 +388,48 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
     else:
         # rhok_s from the cderi's blocks; rho_B[L] = tr(A_L D) = sum_s tr(rhok_s[L])
         rhoks = [_rhok(dfobj, fl, fr, nao, nrow) for fl, fr in facs]
-        rho = sum(t.diagonal(r, dim1=1, dim2=2).sum(dim=1) for r in rhoks)
+        rho = 0
+        for (r, packed), (fl, _) in zip(rhoks, facs):
+            if packed:
+                dpos = t.as_tensor(_diag_positions(fl.shape[1]), device=_dev.device())
+                rho = rho + r[:, dpos].sum(dim=1)
+            else:
+                rho = rho + t.diagonal(r, dim1=1, dim2=2).sum(dim=1)
     tm.mark('weights')
     c = t.empty((1, naux), dtype=t.float64, device=_dev.device())
     _gemm(rho.reshape(1, -1), A, c)                             # c = A^T rho
     c = c.reshape(-1)
     Z = (-.5 * j_factor) * t.outer(c, c)
-    # W in the cderi's compact columns
+    # W in the cderi's compact columns: W[P, c] = j c_P D_c, the exchange
+    # part added by the pair products (the first of which writes the sum)
     tabs = _pair_tables(dfobj, mol)
     ncol = tabs['ncol']
     W = t.empty((naux, ncol), dtype=t.float64, device=_dev.device())
-    Dflat = D.reshape(-1)[tabs['addr']]
-    jb = max(1, min(naux, WBLOCK_BYTES // max(8 * ncol, 1)))
-    for P0 in range(0, naux, jb):
-        P1 = min(naux, P0 + jb)
-        W[P0:P1] = (j_factor * c[P0:P1, None]) * Dflat[None, :]
-    if k_factor != 0:
+    Dflat = D.reshape(-1)[tabs['addr']].contiguous()
+    jc = (j_factor * c).contiguous()
+    if k_factor == 0:
+        jb = max(1, min(naux, WBLOCK_BYTES // max(8 * ncol, 1)))
+        for P0 in range(0, naux, jb):
+            P1 = min(naux, P0 + jb)
+            W[P0:P1] = jc[P0:P1, None] * Dflat[None, :]
+    else:
         for si, (fl, fr) in enumerate(facs):
             nocc = fl.shape[1]
-            # one factor (F_l = F_r): every rhok_L and dmoo_P is symmetric,
-            # so A^T rhok and tr(dmoo_P dmoo_Q) run on the packed upper
-            # triangles (half the flops of both products)
-            sym = _same_factor(fl, fr)
+            rk, sym = rhoks[si]
+            rhoks[si] = None
+            dmoo = _dmoo(A, lower, At, rk, naux)
+            del rk
             if sym:
-                dmoo, iu = _dmoo_packed(A, lower, At, rhoks[si], naux)
-                rhoks[si] = None
-                Z += (wk * k_factor) * _trace_products_packed(dmoo, nocc)
+                # packed rows, off-diagonal entries times sqrt 2: the trace
+                # products are plain dot products of the rows
+                Z += (wk * k_factor) * _trace_products_packed(dmoo)
             else:
-                dmoo = _dmoo(A, lower, At, rhoks[si], naux).reshape(naux, nocc * nocc)
-                rhoks[si] = None
                 Z += (wk * k_factor) * _trace_products(dmoo.reshape(naux, nocc, nocc), naux)
             # the exchange part at the stored pairs: G_P = F_l dmoo_P F_r^T
             # as H_P = F_r dmoo_P^T for a block of P (one product), then
-            # the lower 128-tiles of F_l H_P^T per P added straight into
-            # the cderi's compact columns (one batched product; no G_P is
-            # written)
+            # the lower 128-tiles of F_l H_P^T per P put straight into the
+            # cderi's compact columns (one batched product; no G_P is
+            # written), over the Coulomb part on the first factor set
             blk = max(1, min(naux, WBLOCK_BYTES // max(8 * nao * nocc, 1)))
             H = t.empty((nao, blk * nocc), dtype=t.float64, device=_dev.device())
             full = t.empty((blk, nocc * nocc), dtype=t.float64, device=_dev.device()) if sym else None
@@ -417,14 +439,14 @@ def jk_energy_per_atom(dfobj, dm, j_factor=1., k_factor=1., hermi=1, auxbasis_re
                 Hb = H[:, :nP * nocc]
                 if sym:
                     dm_blk = full[:nP]
-                    dm_blk[:, iu[0]] = dmoo[P0:P1]
-                    dm_blk[:, iu[1]] = dmoo[P0:P1]
+                    _sym_unpack(dmoo[P0:P1], dm_blk, nocc, nP, 1., 1. / SQRT2)
                 else:
                     dm_blk = dmoo[P0:P1]
                 _gemm(fr, dm_blk.reshape(nP * nocc, nocc), Hb, transb=True)
+                init = (_dev.ptr(jc[P0:P1]), _dev.ptr(Dflat)) if si == 0 else (None, None)
                 _lib.call('mdf_gemm_pairs_batched', 1, nao, nocc, float(-2 * wk * k_factor), ctypes.c_void_p(fl.data_ptr()),
                           nocc, 0, ctypes.c_void_p(Hb.data_ptr()), Hb.stride(0), nocc, _dev.ptr(tabs['cmap']),
-                          _dev.ptr(W[P0]), W.stride(0), nP, _dev.stream_ptr())
+                          _dev.ptr(W[P0]), W.stride(0), init[0], init[1], nP, _dev.stream_ptr())
             del dmoo, H, full
     del rhoks
     grad = t.zeros((natm, 3), dtype=t.float64, device=_dev.device())

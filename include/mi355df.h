@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 10
+#define MDF_ABI_VERSION 11
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
@@ -317,17 +317,34 @@ int mdf_gemm_batched(int transb, int tri_a, int lower_c, long M, long N, long K,
                      const double *A, long lda, long strideA, const double *B, long ldb, long strideB,
                      double beta, double *C, long ldc, long strideC, long batch, void *stream);
 /* mdf_gemm_pairs_batched: for each z < batch, P_z = A_z op(B_z) (n x n;
- *   A_z = A + z strideA is n x K, op(B_z) is K x n) is added at the stored
+ *   A_z = A + z strideA is n x K, op(B_z) is K x n) is put at the stored
  *   pairs of a packed cderi without being written itself:
- *     W[z ldw + pair_col[i (i + 1) / 2 + j]] += alpha P_z[i][j]
+ *     W[z ldw + c] = base + alpha P_z[i][j],  c = pair_col[i (i + 1) / 2 + j],
  *   for n > i >= j where pair_col (device int32, n (n + 1) / 2 entries) is
- *   >= 0; only the lower 128-tiles of P_z are computed.  The DF gradient's
- *   exchange weights F_l dmoo_P F_r^T at the cderi's compact columns
- *   (electrolites/fastdfgrad.py:870-1182, there a dense cuTENSOR product
- *   and a gather). */
+ *   >= 0; base = init_scale[z] init_vals[c] when init_scale (batch device
+ *   doubles) and init_vals (device doubles, one per column) are given (W's
+ *   old contents are then not read), else W's own value (W += alpha P_z).
+ *   pair_col must not name one column twice.  Only the lower 128-tiles of
+ *   P_z are computed.  The DF gradient's exchange weights F_l dmoo_P F_r^T
+ *   at the cderi's compact columns, with its Coulomb weights c_P D_c as the
+ *   base (electrolites/fastdfgrad.py:870-1182, there a dense cuTENSOR
+ *   product, a gather and a separate sum). */
 int mdf_gemm_pairs_batched(int transb, long n, long K, double alpha, const double *A, long lda,
                            long strideA, const double *B, long ldb, long strideB, const int *pair_col,
-                           double *W, long ldw, long batch, void *stream);
+                           double *W, long ldw, const double *init_scale, const double *init_vals,
+                           long batch, void *stream);
+/* mdf_sym_pack: for each z < batch, the packed upper triangle (row-major
+ *   over a <= b; (a, b) at a n - a (a - 1) / 2 + b - a) of G_z + G_z^T,
+ *   scaled by diag_scale on the diagonal and off_scale off it:
+ *     P[z strideP + (a, b)] = s_ab (G[z strideG + a n + b] + G[z strideG + b n + a]).
+ * mdf_sym_unpack: the inverse layout, G_z[a][b] = G_z[b][a] = s_ab P_z[(a, b)].
+ *   The DF gradient's occupied-space blocks rhok_L = G_L + G_L^T and dmoo_P,
+ *   symmetric for one factor, kept packed between its products
+ *   (electrolites/fastdfgrad.py:870-1182, there full nocc x nocc blocks). */
+int mdf_sym_pack(long n, const double *G, long strideG, double *P, long strideP, double diag_scale,
+                 double off_scale, long batch, void *stream);
+int mdf_sym_unpack(long n, const double *P, long strideP, double *G, long strideG, double diag_scale,
+                   double off_scale, long batch, void *stream);
 size_t mdf_potrf_workspace(long n);
 int mdf_potrf(double *A, long n, long lda, int *info, void *work, size_t work_bytes, void *stream);
 size_t mdf_trtri_workspace(long n);

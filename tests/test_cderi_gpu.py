@@ -243,15 +243,17 @@ def test_gemm_batched(transb, tri_a, lower_c):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('transb', [0, 1])
-def test_gemm_pairs_batched(transb):
-    """mdf_gemm_pairs_batched against numpy: each P_z = A_z op(B) is added
-    at the stored pairs of row z of W only (a ragged n over three 128-tiles,
-    a column map with 60 % of the pairs screened out and the stored ones in
-    a shuffled column order, a shared B, padded W rows); every other entry
-    of W keeps its value."""
+@pytest.mark.parametrize('init', [False, True])
+def test_gemm_pairs_batched(transb, init):
+    """mdf_gemm_pairs_batched against numpy: each P_z = A_z op(B) is put at
+    the stored pairs of row z of W only (a ragged n over three 128-tiles, a
+    column map with 60 % of the pairs screened out and the stored ones in a
+    shuffled column order, a shared B, padded W rows): added to W's value,
+    or over init_scale[z] init_vals[c] with W's old value not read (W
+    starts as NaN there); every other entry of W keeps its value."""
     import torch
     from electrolites_amd import _dev, _lib
-    rng = np.random.default_rng(20 + transb)
+    rng = np.random.default_rng(20 + transb + 2 * init)
     nb, n, K = 4, 261, 37
     a = rng.standard_normal((nb, n, K + 1))
     b = rng.standard_normal((n, K + 2) if transb else (K, n + 2))
@@ -261,16 +263,64 @@ def test_gemm_pairs_batched(transb):
     col = np.full(npk, -1, dtype=np.int32)
     col[keep] = rng.permutation(ncol)
     w = rng.standard_normal((nb, ncol + 5))
-    A, B, W = (torch.from_numpy(x.copy()).cuda() for x in (a, b, w))
+    if init:
+        w[:, :ncol] = np.nan
+    sc, vals = rng.standard_normal(nb), rng.standard_normal(ncol)
+    A, B, W, S, V = (torch.from_numpy(x.copy()).cuda() for x in (a, b, w, sc, vals))
     C = torch.from_numpy(col).cuda()
     _lib.call('mdf_gemm_pairs_batched', transb, n, K, ctypes.c_double(-.6), _dev.ptr(A), K + 1, n * (K + 1),
-              _dev.ptr(B), b.shape[1], 0, _dev.ptr(C), _dev.ptr(W), ncol + 5, nb, None)
+              _dev.ptr(B), b.shape[1], 0, _dev.ptr(C), _dev.ptr(W), ncol + 5, _dev.ptr(S) if init else None,
+              _dev.ptr(V) if init else None, nb, None)
     bb = b[:, :K].T if transb else b[:K, :n]
     i, j = np.tril_indices(n)
     pk = i * (i + 1) // 2 + j
     sel = col[pk] >= 0
     ref = w.copy()
+    if init:
+        ref[:, :ncol] = sc[:, None] * vals[None, :]
     for z in range(nb):
         p = a[z, :, :K] @ bb
         ref[z, col[pk[sel]]] += -.6 * p[i[sel], j[sel]]
     assert rel(W.cpu().numpy(), ref) < 1e-13
+    with pytest.raises(RuntimeError, match='go together'):
+        _lib.call('mdf_gemm_pairs_batched', transb, n, K, ctypes.c_double(-.6), _dev.ptr(A), K + 1, n * (K + 1),
+                  _dev.ptr(B), b.shape[1], 0, _dev.ptr(C), _dev.ptr(W), ncol + 5, _dev.ptr(S), None, nb, None)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('n,batch', [(1, 1), (31, 3), (32, 2), (33, 5), (320, 4), (5, 70000)])
+def test_sym_pack_unpack(n, batch):
+    """mdf_sym_pack / mdf_sym_unpack against numpy's upper-triangle order
+    (row-major over a <= b) with diagonal and off-diagonal scales, at sizes
+    on, below and above the 32-tiles and a batch beyond the grid's y limit;
+    padded batch strides leave the padding untouched."""
+    import torch
+    from electrolites_amd import _dev, _lib
+    rng = np.random.default_rng(n + batch)
+    if batch > 1000:
+        batch = 66000      # beyond 65535 blocks in y: the kernel strides the batch
+    npk = n * (n + 1) // 2
+    g = rng.standard_normal((batch, n * n + 3))
+    p = rng.standard_normal((batch, npk + 2))
+    G, P = torch.from_numpy(g.copy()).cuda(), torch.from_numpy(p.copy()).cuda()
+    _lib.call('mdf_sym_pack', n, _dev.ptr(G), n * n + 3, _dev.ptr(P), npk + 2, ctypes.c_double(.5),
+              ctypes.c_double(2 ** .5), batch, None)
+    gm = g[:, :n * n].reshape(batch, n, n)
+    a, b2 = np.triu_indices(n)
+    s = np.where(a == b2, .5, 2 ** .5)
+    ref = p.copy()
+    ref[:, :npk] = s * (gm[:, a, b2] + gm[:, b2, a])
+    assert np.array_equal(P.cpu().numpy(), ref)
+    G2 = torch.from_numpy(g.copy()).cuda()
+    _lib.call('mdf_sym_unpack', n, _dev.ptr(P), npk + 2, _dev.ptr(G2), n * n + 3, ctypes.c_double(2.),
+              ctypes.c_double(2 ** -.5), batch, None)
+    full = np.empty((batch, n, n))
+    u = np.where(a == b2, 2., 2 ** -.5) * ref[:, :npk]
+    full[:, a, b2] = u
+    full[:, b2, a] = u
+    out = g.copy()
+    out[:, :n * n] = full.reshape(batch, -1)
+    assert np.array_equal(G2.cpu().numpy(), out)
+    with pytest.raises(RuntimeError, match='batch stride'):
+        _lib.call('mdf_sym_pack', n, _dev.ptr(G), n * n - 1, _dev.ptr(P), npk + 2, ctypes.c_double(1.),
+                  ctypes.c_double(1.), 2, None)
