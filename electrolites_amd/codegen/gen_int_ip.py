@@ -5,8 +5,10 @@ One kernel per angular class, as the integral kernels of gen_int.py, each
 contracting its derivative integrals with a weight on the fly and adding the
 result to a per-atom gradient:
 
-* ``int3c_ip_<la><lb><lc>``: one thread per (shell pair (I >= J), aux shell
-  K, aux spherical component m).  The derivative of a primitive Cartesian
+* ``int3c_ip_<la><lb><lc>`` (``_m<m>``: one kernel per aux spherical
+  component m, launched together, in the classes where a thread does not
+  take all of them): one thread per (shell pair (I >= J), aux shell K, aux
+  spherical component m).  The derivative of a primitive Cartesian
   Gaussian about its centre is ``d/dA_x x_A^a e^(-alpha r_A^2) = 2 alpha
   x_A^(a+1) e - a x_A^(a-1) e``, so with the Rys factors of gen_int.py
   ``dI/dA_x = 2 alpha I_x(a+1, b) - a I_x(a-1, b)`` (and the same about B with
@@ -40,8 +42,11 @@ OUT = os.path.join(HERE, '..', 'csrc', 'mdf_int_ip_gen.hip')
 
 def _vrr_hrr(w, lab_max, lc, lb_max, indent):
     """Rys 2-D integrals g(e, f) for e <= lab_max, f <= lc, then the
-    horizontal transfer I(a, b, f) for b <= lb_max, a + b <= lab_max."""
+    horizontal transfer I(a, b, f) for b <= lb_max, a + b <= lab_max.  lc is
+    one limit for all three directions or a dict of one per direction."""
+    lcd = lc if isinstance(lc, dict) else {d: lc for d in 'xyz'}
     for d in 'xyz':
+        lc = lcd[d]
         init = 'rw[r] * pref' if d == 'z' else '1.0'
         w(f'{indent}const double g{d}_0_0 = {init};')
         for e in range(0, lab_max + 1):
@@ -60,7 +65,7 @@ def _vrr_hrr(w, lab_max, lc, lb_max, indent):
                         terms.append(f'{e}.0 * B00 * g{d}_{e-1}_{f-1}')
                 w(f'{indent}const double g{d}_{e}_{f} = ' + ' + '.join(terms) + ';')
     for d in 'xyz':
-        for f in range(lc + 1):
+        for f in range(lcd[d] + 1):
             for a in range(lab_max + 1):
                 w(f'{indent}const double I{d}_{a}_0_{f} = g{d}_{a}_{f};')
             for b in range(1, lb_max + 1):
@@ -115,8 +120,21 @@ def _tail(w, indent='    '):
     w(indent + '}')
 
 
-def gen3(la, lb, lc):
-    name = f'int3c_ip_{la}{lb}{lc}'
+def _component_limits(lc, mi):
+    """Highest power per direction among the Cartesian functions of the
+    spherical aux component mi (its Rys recurrences stop there)."""
+    cc, Cc = cart_powers(lc), cart2sph(lc)
+    sup = [cc[k] for k in range(len(cc)) if abs(Cc[mi, k]) > 1e-15]
+    return {d: max(p[i] for p in sup) for i, d in enumerate('xyz')}
+
+
+def gen3(la, lb, lc, m_fixed=None):
+    """One three-centre derivative kernel: every aux component per thread
+    (``allm_ip``), or the component ``m_fixed`` alone (one kernel per
+    component, each with the recurrences that component needs: a kernel
+    that switched on m kept every component's Rys factors live and spilled
+    in the larger classes)."""
+    name = f'int3c_ip_{la}{lb}{lc}' + ('' if m_fixed is None else f'_m{m_fixed}')
     lab = la + lb
     nroots = (lab + 1 + lc) // 2 + 1
     ca, cb, cc = cart_powers(la), cart_powers(lb), cart_powers(lc)
@@ -133,17 +151,14 @@ def gen3(la, lb, lc):
     w('    const int nslot = 6;')
     _head(w)
     am = allm_ip(la, lb, lc)
+    assert am == (m_fixed is None)
     ms = list(range(2 * lc + 1)) if am else [None]
     w('    const int nkc = (nk + kch - 1) / kch;')
     w(f'    const long tid = (long)blockIdx.x * {THREADS} + threadIdx.x;')
-    if am:
-        w('    if (tid < (long)np * nkc) {')
-        w('    const int ip = (int)(tid % np), kc = (int)(tid / np);')
-    else:
-        w(f'    if (tid < (long)np * nkc * {2 * lc + 1}) {{')
-        w('    const int ip = (int)(tid % np);')
-        w('    const long rest = tid / np;')
-        w('    const int kc = (int)(rest % nkc), m = (int)(rest / nkc);')
+    w('    if (tid < (long)np * nkc) {')
+    w('    const int ip = (int)(tid % np), kc = (int)(tid / np);')
+    if not am:
+        w(f'    const int m = {m_fixed};')
     w('    const int ish = pairs[2 * ip], jsh = pairs[2 * ip + 1];')
     w('    const double Ax = ao.xyz[3 * ish], Ay = ao.xyz[3 * ish + 1], Az = ao.xyz[3 * ish + 2];')
     w('    const double Bx = ao.xyz[3 * jsh], By = ao.xyz[3 * jsh + 1], Bz = ao.xyz[3 * jsh + 2];')
@@ -215,7 +230,7 @@ def gen3(la, lb, lc):
     for d in 'xyz':
         w(f'                    const double C00{d} = PA{d} - qf * PC{d}, C0p{d} = pf * PC{d};')
     ind = '                    '
-    _vrr_hrr(w, lab + 1, lc, lb, ind)
+    _vrr_hrr(w, lab + 1, lc if am else _component_limits(lc, m_fixed), lb, ind)
     def products(mi, wname, indent):
         sup = [(k, Cc[mi, k]) for k in range(len(cc)) if abs(Cc[mi, k]) > 1e-15]
         for a_i, (ax, ay, az) in enumerate(ca):
@@ -235,13 +250,7 @@ def gen3(la, lb, lc):
         for mi in range(2 * lc + 1):
             products(mi, f'wc_{mi}_', ind)
     else:
-        w(ind + 'switch (m) {')
-        for mi in range(2 * lc + 1):
-            w(ind + f'case {mi}: {{')
-            products(mi, 'wc_', ind + '    ')
-            w(ind + '    break;')
-            w(ind + '}')
-        w(ind + '}')
+        products(m_fixed, 'wc_', ind)
     w('                }')
     w('            }')
     w('        }')
@@ -387,9 +396,16 @@ def main():
     for la in range(LMAX_AO + 1):
         for lb in range(LMAX_AO + 1):
             for lc in range(LMAX_AUX + 1):
-                n, src = gen3(la, lb, lc)
-                n3[(la, lb, lc)] = n
-                parts += [src, '']
+                if allm_ip(la, lb, lc):
+                    n, src = gen3(la, lb, lc)
+                    n3[(la, lb, lc)] = [n]
+                    parts += [src, '']
+                else:
+                    n3[(la, lb, lc)] = []
+                    for mi in range(2 * lc + 1):
+                        n, src = gen3(la, lb, lc, mi)
+                        n3[(la, lb, lc)].append(n)
+                        parts += [src, '']
     for la in range(LMAX_AUX + 1):
         for lc in range(LMAX_AUX + 1):
             n, src = gen2(la, lc)
@@ -404,14 +420,15 @@ def main():
     parts.append('    const size_t shm = use_lds ? 6 * (size_t)natm * sizeof(double) : 0;')
     parts.append('    if ((long)np * nkc == 0) return MDF_OK;')
     parts.append(f'    const dim3 b({THREADS});')
+    parts.append(f'    const dim3 g((unsigned)(((long)np * nkc + {THREADS - 1}) / {THREADS}));')
     parts.append(f'    switch ((la * {LMAX_AO + 1} + lb) * {LMAX_AUX + 1} + lc) {{')
-    for (la, lb, lc), n in n3.items():
+    for (la, lb, lc), names in n3.items():
         key = (la * (LMAX_AO + 1) + lb) * (LMAX_AUX + 1) + lc
-        mult = 1 if allm_ip(la, lb, lc) else 2 * lc + 1
-        parts.append(f'    case {key}: {{ const long n = (long)np * nkc * {mult}; '
-                     f'const dim3 g((unsigned)((n + {THREADS - 1}) / {THREADS})); '
-                     f'hipLaunchKernelGGL({n}, g, b, shm, st, ao, aux, pairs, np, kl, nk, W, ldw, '
-                     'cmap, ao_atom, aux_atom, natm, use_lds, kch, grad, grad_aux); } break;')
+        parts.append(f'    case {key}:')
+        for n in names:
+            parts.append(f'        hipLaunchKernelGGL({n}, g, b, shm, st, ao, aux, pairs, np, kl, nk, W, ldw, '
+                         'cmap, ao_atom, aux_atom, natm, use_lds, kch, grad, grad_aux);')
+        parts.append('        break;')
     parts.append('    default: set_error("int3c2e_ip: class (%d %d|%d) not generated", la, lb, lc); return MDF_EINVAL;')
     parts.append('    }')
     parts.append('    MDF_LAUNCHED("int3c_ip");')
@@ -438,7 +455,7 @@ def main():
     parts.append('}  // namespace mdf')
     with open(OUT, 'w') as f:
         f.write('\n'.join(parts) + '\n')
-    print('wrote', OUT, len(n3), '3c and', len(n2), '2c derivative kernels')
+    print('wrote', OUT, sum(len(v) for v in n3.values()), '3c and', len(n2), '2c derivative kernels')
 
 
 if __name__ == '__main__':

@@ -105,11 +105,16 @@ def _cholesky_inverse(M):
 class RHF:
     def __init__(self, mol, conv_tol=1e-10, conv_tol_grad=None, max_cycle=100, diis_space=8,
                  df=None, verbose=0, k_precision='fp64', switch_err=1e-5, low_precision='fp32',
-                 host_diag_max=0, precision_report=False, purify_min=None):
+                 host_diag_max=0, precision_report=False, purify_min=None, conv_check=False):
         if k_precision not in ('fp64', 'fp32', 'bf16x3', 'mixed'):
             raise ValueError(f'k_precision {k_precision!r}')
         if low_precision not in ('fp32', 'bf16x3'):
             raise ValueError(f'low_precision {low_precision!r}')
+        # after convergence on an FP64 J/K, one more FP64 J/K of the new
+        # density gives the energy (as PySCF's conv_check re-runs a cycle);
+        # off, the converged cycle's FP64 energy is the result.  A run that
+        # converged in FP32 or split-BF16 always takes the extra FP64 J/K.
+        self.conv_check = conv_check
         self.k_precision = k_precision
         self.low_precision = low_precision
         # diagnostic: every low-precision cycle also builds the FP64 J/K of
@@ -366,8 +371,10 @@ class RHF:
 
     def absorb(self, vj, vk):
         """One SCF cycle from the J/K of ``self.dm``: Fock, energy, DIIS,
-        diagonalisation and the next density.  After convergence (or
-        ``max_cycle``) one more FP64 J/K gives the energy of the final
+        diagonalisation and the next density.  Converged on an FP64 J/K, the
+        cycle's energy is the result (``conv_check``: one more FP64 J/K of
+        the new density gives it); converged in a narrower precision or at
+        ``max_cycle``, one more FP64 J/K gives the energy of the final
         density."""
         t = _dev.torch()
         F = self.H + vj - .5 * vk
@@ -406,6 +413,10 @@ class RHF:
             self._diis_reset()
         elif abs(de) < self.conv_tol and err_max < self.conv_tol_grad:
             self.converged = True           # (an all-FP32 run converges in FP32)
+            if prec == 'fp64' and not self.conv_check:
+                self.e_tot = e_tot
+                self.done = True
+                return
             self._final = True
         if cycle >= self.max_cycle:
             self._final = True

@@ -73,3 +73,29 @@ def test_purified_scf_matches_diagonalised(mol, precision):
     assert float((mf.dm - ref.dm).abs().max()) < 1e-6
     q = mf._q_occ
     assert float((q.T @ q - torch.eye(q.shape[1], dtype=q.dtype, device=q.device)).abs().max()) < 1e-12
+
+
+@pytest.mark.gpu
+def test_conv_check(mol):
+    """Converged on an FP64 J/K, the cycle's energy is the result; with
+    conv_check one more FP64 J/K of the new density gives it.  The two agree
+    to 1e-9 Eh (the energy is stationary at convergence), and conv_check
+    costs exactly one more J/K."""
+    from electrolites_amd.scf import RHF
+    out = []
+    for check in (False, True):
+        mf = RHF(mol, conv_tol=1e-10, conv_check=check)
+        calls = [0]
+        get_jk = mf.with_df.get_jk
+
+        def counted(*a, **k):
+            calls[0] += 1
+            return get_jk(*a, **k)
+        mf.with_df.get_jk = counted
+        e = mf.kernel()
+        assert mf.converged
+        out.append((e, mf.cycles, calls[0], mf.history[-1]['e_tot']))
+    (ea, ca, na, ha), (eb, cb, nb, hb) = out
+    assert ca == cb and na == ca and nb == cb + 1
+    assert ea == ha
+    assert abs(ea - eb) < 1e-9, (ea, eb)
