@@ -9,7 +9,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, 'libmi355df.so')
-ABI_VERSION = 11
+ABI_VERSION = 12
 DTYPE_F64, DTYPE_F32 = 0, 1
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -53,6 +53,9 @@ _SIGS = {
     'mdf_gemm_pairs_batched': (I, [I, L, L, D, P, L, L, P, L, L, P, P, L, P, P, L, P]),
     'mdf_sym_pack': (I, [L, P, L, P, L, D, D, L, P]),
     'mdf_sym_unpack': (I, [L, P, L, P, L, D, D, L, P]),
+    'mdf_sp2_tiles': (L, [L]),
+    'mdf_sp2_begin': (I, [L, P, L, P, P]),
+    'mdf_sp2_step': (I, [L, D, P, L, P, P, L, P, P]),
     'mdf_potrf_workspace': (SZ, [L]),
     'mdf_potrf': (I, [P, L, L, P, P, SZ, P]),
     'mdf_trtri_workspace': (SZ, [L]),

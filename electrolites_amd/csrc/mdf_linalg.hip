@@ -252,6 +252,121 @@ __global__ __launch_bounds__(256) void sym_unpack_kernel(int n, const double *__
     }
 }
 
+// ------------------------------------------------------------------ SP2
+// One step of SP2 purification of a symmetric n x n X on 64-tiles of its
+// lower triangle: Y = X^2 or Y = 2X - X^2, whichever trace lies closer to
+// nocc, with tr(X^2) = ||X||_F^2 (X symmetric) known before the product, so
+// the choice, the product, the update and the next step's sums are one
+// kernel.  Each lower tile writes its block of Y and the mirrored block, and
+// its share of tr(Y) and ||Y||_F^2 into part_out[2 tile]; the next step's
+// blocks all sum part_in in one fixed order (no atomics: every block takes
+// the same branch and the sums are reproducible).
+constexpr int SP2_T = 64;
+
+// sum of part[2 t + q] over t < ntiles, the same order in every block
+__device__ __forceinline__ double2 sp2_sums(const double *part, int ntiles, double *red) {
+    double a = 0., b = 0.;
+    for (int t = threadIdx.x; t < ntiles; t += G_THREADS) {
+        a += part[2 * t];
+        b += part[2 * t + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * w] = a;
+        red[2 * w + 1] = b;
+    }
+    __syncthreads();
+    double2 r{0., 0.};
+    for (int i = 0; i < G_THREADS / 64; i++) {
+        r.x += red[2 * i];
+        r.y += red[2 * i + 1];
+    }
+    __syncthreads();
+    return r;
+}
+
+// the block's (tr, ||.||_F^2 share) into part[2 tile]
+__device__ __forceinline__ void sp2_put(double tr, double fro, double *part, int tile, double *red) {
+    for (int off = 32; off > 0; off >>= 1) {
+        tr += __shfl_xor(tr, off, 64);
+        fro += __shfl_xor(fro, off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * w] = tr;
+        red[2 * w + 1] = fro;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0., b = 0.;
+        for (int i = 0; i < G_THREADS / 64; i++) {
+            a += red[2 * i];
+            b += red[2 * i + 1];
+        }
+        part[2 * tile] = a;
+        part[2 * tile + 1] = b;
+    }
+}
+
+// the partial sums of X itself (the first step's input)
+__global__ __launch_bounds__(G_THREADS) void sp2_norms_kernel(int n, const double *__restrict__ X, long ldx,
+                                                              double *__restrict__ part) {
+    __shared__ double red[2 * G_THREADS / 64];
+    int ti, tj;
+    lower_tile(blockIdx.x, ti, tj);
+    const int m0 = ti * SP2_T, n0 = tj * SP2_T;
+    const double wgt = ti == tj ? 1. : 2.;
+    double tr = 0., fro = 0.;
+    for (int e = threadIdx.x; e < SP2_T * SP2_T; e += G_THREADS) {
+        const int i = m0 + e / SP2_T, j = n0 + e % SP2_T;
+        if (i < n && j < n) {
+            const double x = X[(long)i * ldx + j];
+            fro += wgt * x * x;
+            if (i == j) tr += x;
+        }
+    }
+    sp2_put(tr, fro, part, blockIdx.x, red);
+}
+
+__global__ __launch_bounds__(G_THREADS, 2) void sp2_step_kernel(int n, double nocc, const double *__restrict__ X,
+                                                                long ldx, const double *__restrict__ part_in,
+                                                                double *__restrict__ Y, long ldy,
+                                                                double *__restrict__ part_out, int ntiles) {
+    using S = TileShape<SP2_T, SP2_T>;
+    __shared__ double lds[S::LDS_DOUBLES];
+    int ti, tj;
+    lower_tile(blockIdx.x, ti, tj);
+    const int m0 = ti * SP2_T, n0 = tj * SP2_T;
+    const double2 sums = sp2_sums(part_in, ntiles, lds);
+    const double trX = sums.x, trX2 = sums.y;
+    const bool up = fabs(trX2 - nocc) >= fabs(2. * trX - trX2 - nocc);
+    // X X = X X^T: both operands are rows of X
+    RowsLoader<SP2_T> la{X, ldx, m0, n, 0, n};
+    RowsLoader<SP2_T> lb{X, ldx, n0, n, 0, n};
+    Acc<S::MI, S::NI> acc;
+    acc.zero();
+    gemm_mainloop<SP2_T, SP2_T>(la, lb, (n + G_BK - 1) / G_BK, lds, acc);
+    const double wgt = ti == tj ? 1. : 2.;
+    double tr = 0., fro = 0.;
+    gemm_epilogue<SP2_T, SP2_T>(acc, [&](int r, int c, double v) {
+        const int i = m0 + r, j = n0 + c;
+        if (i < n && j < n) {
+            const double y = up ? 2. * X[(long)i * ldx + j] - v : v;
+            Y[(long)i * ldy + j] = y;
+            if (ti != tj) Y[(long)j * ldy + i] = y;
+            fro += wgt * y * y;
+            if (i == j) tr += y;
+        }
+    });
+    sp2_put(tr, fro, part_out, blockIdx.x, lds);
+}
+
 constexpr long MAX_GRID_Z = 65535;
 
 template <int BN, bool BT, int TRI, bool LOWER_C>
@@ -458,6 +573,34 @@ extern "C" int mdf_sym_unpack(long n, const double *P, long strideP, double *G, 
     hipLaunchKernelGGL(sym_unpack_kernel, sym_grid(n, batch), dim3(256), 0, as_stream(stream), (int)n, P, strideP, G,
                        strideG, batch, diag_scale, off_scale);
     MDF_LAUNCHED("sym_unpack_kernel");
+    return MDF_OK;
+}
+
+extern "C" long mdf_sp2_tiles(long n) {
+    const long t = (n + SP2_T - 1) / SP2_T;
+    return n > 0 ? t * (t + 1) / 2 : 0;
+}
+
+extern "C" int mdf_sp2_begin(long n, const double *X, long ldx, double *part, void *stream) {
+    MDF_CHECK_ARG(n >= 0 && n < (1L << 20) && ldx >= n, "mdf_sp2_begin: bad sizes");
+    if (n == 0) return MDF_OK;
+    MDF_CHECK_ARG(X && part, "mdf_sp2_begin: null pointer");
+    hipLaunchKernelGGL(sp2_norms_kernel, dim3((unsigned)mdf_sp2_tiles(n)), dim3(G_THREADS), 0, as_stream(stream),
+                       (int)n, X, ldx, part);
+    MDF_LAUNCHED("sp2_norms_kernel");
+    return MDF_OK;
+}
+
+extern "C" int mdf_sp2_step(long n, double nocc, const double *X, long ldx, const double *part_in, double *Y,
+                            long ldy, double *part_out, void *stream) {
+    MDF_CHECK_ARG(n >= 0 && n < (1L << 20) && ldx >= n && ldy >= n, "mdf_sp2_step: bad sizes");
+    if (n == 0) return MDF_OK;
+    MDF_CHECK_ARG(X && Y && part_in && part_out, "mdf_sp2_step: null pointer");
+    MDF_CHECK_ARG(X != Y && part_in != part_out, "mdf_sp2_step: Y and part_out must not alias X and part_in");
+    const long nt = mdf_sp2_tiles(n);
+    hipLaunchKernelGGL(sp2_step_kernel, dim3((unsigned)nt), dim3(G_THREADS), 0, as_stream(stream), (int)n, nocc, X,
+                       ldx, part_in, Y, ldy, part_out, (int)nt);
+    MDF_LAUNCHED("sp2_step_kernel");
     return MDF_OK;
 }
 

@@ -31,6 +31,7 @@ import numpy as np
 from threadpoolctl import threadpool_limits
 
 from . import _dev
+from . import _lib
 from . import eigsolve
 from . import fastdf
 from . import ints
@@ -179,7 +180,8 @@ class RHF:
         - A) / (e_max - e_min); each step takes X^2 or 2X - X^2, whichever
         trace lies closer to nocc, and drives the eigenvalues to 1 (the nocc
         lowest of A) or 0, until X is the spectral projector P (FP64
-        products only).  The basis is
+        products only; tr(X^2) = ||X||_F^2 for the symmetric X, so each step
+        is one kernel).  The basis is
         P R (R^T P R)^-1/2 by a Cholesky factor (``mdf_potrf`` /
         ``mdf_trtri``), R the previous cycle's occupied orbitals: no solver
         library is called.  None (the caller diagonalises) when the steps do
@@ -194,26 +196,33 @@ class RHF:
         emin, emax = t.stack([(d - off).min(), (d + off).max()]).tolist()
         if not emax > emin:
             return None
-        X = A * (-1. / (emax - emin))
-        X.diagonal().add_(emax / (emax - emin))
-        # the step choice is made on the device (no host round trip per
-        # step); convergence is read back once per PURIFY_BLOCK steps, and
-        # the steps run until a block ends with tr(X - X^2) below 1e-12
+        # two iterates and their per-tile (tr, ||.||_F^2) sums, ping-pong
+        Xs = t.empty((2, n, n), dtype=t.float64, device=A.device)
+        part = t.empty((2, _lib.query('mdf_sp2_tiles', n), 2), dtype=t.float64, device=A.device)
+        t.mul(A, -1. / (emax - emin), out=Xs[0])
+        Xs[0].diagonal().add_(emax / (emax - emin))
+        st = _dev.stream_ptr()
+        _lib.call('mdf_sp2_begin', n, _dev.ptr(Xs[0]), n, _dev.ptr(part[0]), st)
+        # one kernel a step (``mdf_sp2_step``: the step choice from the
+        # previous sums, the lower tiles of the product, the update and the
+        # next sums); convergence is read back once per PURIFY_BLOCK steps,
+        # and the steps run until a block ends with tr(X - X^2) below 1e-12
         # (further steps leave a converged projector as it is)
-        done = False
+        done, cur = False, 0
         for _ in range(0, self.PURIFY_MAXIT, self.PURIFY_BLOCK):
             for _ in range(self.PURIFY_BLOCK):
-                X2 = X @ X
-                trX, trX2 = X.diagonal().sum(), X2.diagonal().sum()
-                up = (trX2 - nocc).abs() >= (2. * trX - trX2 - nocc).abs()
-                X = t.where(up, 2. * X - X2, X2)
-            if abs(float(trX - trX2)) < 1e-12:
+                _lib.call('mdf_sp2_step', n, float(nocc), _dev.ptr(Xs[cur]), n, _dev.ptr(part[cur]),
+                          _dev.ptr(Xs[1 - cur]), n, _dev.ptr(part[1 - cur]), st)
+                cur = 1 - cur
+            trX, trX2 = part[cur].sum(dim=0).tolist()
+            if abs(trX - trX2) < 1e-12:
                 done = True
                 break
         if not done:
             return None
-        if abs(float(X.diagonal().sum()) - nocc) > 1e-6:
+        if abs(trX - nocc) > 1e-6:
             return None
+        X = Xs[cur]
         PR = X @ R
         M = (R.T @ PR).contiguous()
         L, ok = _cholesky_inverse(M)

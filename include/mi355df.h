@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MDF_ABI_VERSION 11
+#define MDF_ABI_VERSION 12
 
 /* arithmetic precision of a contraction */
 #define MDF_PREC_FP64 0
@@ -345,6 +345,20 @@ int mdf_sym_pack(long n, const double *G, long strideG, double *P, long strideP,
                  double off_scale, long batch, void *stream);
 int mdf_sym_unpack(long n, const double *P, long strideP, double *G, long strideG, double diag_scale,
                    double off_scale, long batch, void *stream);
+/* SP2 purification of the serving loop's SCF (no counterpart in the
+ * reference, whose SCF driver is GPU4PySCF's): for a symmetric X (n x n,
+ * row stride ldx) one step writes Y = X^2 when |tr(X^2) - nocc| >=
+ * |2 tr(X) - tr(X^2) - nocc|, else Y = 2X - X^2, computing only the lower
+ * 64-tiles of the product and mirroring them (Y symmetric to the bit).
+ * tr(X) and tr(X^2) = ||X||_F^2 come from part_in, mdf_sp2_tiles(n) pairs of
+ * partial sums in a fixed order, written by mdf_sp2_begin (for X itself) or
+ * by the previous step's part_out (for its Y); the sums are taken in the
+ * same order by every workgroup, so a step is reproducible.  Y and part_out
+ * must not alias X and part_in. */
+long mdf_sp2_tiles(long n);
+int mdf_sp2_begin(long n, const double *X, long ldx, double *part, void *stream);
+int mdf_sp2_step(long n, double nocc, const double *X, long ldx, const double *part_in, double *Y, long ldy,
+                 double *part_out, void *stream);
 size_t mdf_potrf_workspace(long n);
 int mdf_potrf(double *A, long n, long lda, int *info, void *work, size_t work_bytes, void *stream);
 size_t mdf_trtri_workspace(long n);

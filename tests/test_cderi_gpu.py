@@ -324,3 +324,55 @@ def test_sym_pack_unpack(n, batch):
     with pytest.raises(RuntimeError, match='batch stride'):
         _lib.call('mdf_sym_pack', n, _dev.ptr(G), n * n - 1, _dev.ptr(P), npk + 2, ctypes.c_double(1.),
                   ctypes.c_double(1.), 2, None)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('n', [1, 63, 64, 150, 257])
+def test_sp2_step(n):
+    """mdf_sp2_begin / mdf_sp2_step against numpy: from a symmetric X with
+    its spectrum in [0, 1], each step takes X^2 or 2X - X^2 by the trace rule
+    (|tr X^2 - nocc| >= |2 tr X - tr X^2 - nocc| -> 2X - X^2), Y is
+    symmetric to the bit (padded row strides; the padding untouched), the
+    per-tile sums add up to tr(Y) and ||Y||_F^2, and 60 steps from a
+    Gershgorin-scaled Fock-like matrix give the spectral projector onto its
+    nocc lowest eigenvectors (1e-10)."""
+    import torch
+    from electrolites_amd import _dev, _lib
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n))
+    a = a + a.T + np.diag(np.arange(n) * 3.)
+    nocc = max(1, n // 3)
+    d = np.diag(a)
+    off = np.abs(a).sum(1) - np.abs(d)
+    emin, emax = (d - off).min(), (d + off).max()
+    # (one function: the bounds meet, and any value in (0, 1) is the input)
+    x = (emax * np.eye(n) - a) / (emax - emin) if n > 1 else np.full((1, 1), .6)
+    nt = _lib.query('mdf_sp2_tiles', n)
+    ld = n + 3
+    X = torch.full((2, n, ld), 7., dtype=torch.float64).cuda()
+    X[0, :, :n] = torch.from_numpy(x)
+    part = torch.zeros((2, nt, 2), dtype=torch.float64).cuda()
+    _lib.call('mdf_sp2_begin', n, _dev.ptr(X[0]), ld, _dev.ptr(part[0]), None)
+    s = part[0].sum(0).cpu().numpy()
+    assert abs(s[0] - np.trace(x)) < 1e-10 * n and abs(s[1] - (x * x).sum()) < 1e-10 * n
+    ref, cur = x.copy(), 0
+    for step in range(60):
+        _lib.call('mdf_sp2_step', n, ctypes.c_double(nocc), _dev.ptr(X[cur]), ld, _dev.ptr(part[cur]),
+                  _dev.ptr(X[1 - cur]), ld, _dev.ptr(part[1 - cur]), None)
+        cur = 1 - cur
+        x2 = ref @ ref
+        up = abs(np.trace(x2) - nocc) >= abs(2 * np.trace(ref) - np.trace(x2) - nocc)
+        ref = 2 * ref - x2 if up else x2
+        if step < 3:
+            got = X[cur].cpu().numpy()
+            assert np.array_equal(got[:, :n], got[:, :n].T)
+            assert np.all(got[:, n:] == 7.)
+            assert rel(got[:, :n], ref) < 1e-12
+            s = part[cur].sum(0).cpu().numpy()
+            assert abs(s[0] - np.trace(ref)) < 1e-10 * n and abs(s[1] - (ref * ref).sum()) < 1e-10 * n
+    w, v = np.linalg.eigh(a)
+    proj = v[:, :nocc] @ v[:, :nocc].T
+    assert np.abs(X[cur].cpu().numpy()[:, :n] - proj).max() < 1e-10
+    with pytest.raises(RuntimeError, match='alias'):
+        _lib.call('mdf_sp2_step', n, ctypes.c_double(nocc), _dev.ptr(X[0]), ld, _dev.ptr(part[0]),
+                  _dev.ptr(X[0]), ld, _dev.ptr(part[1]), None)
