@@ -315,8 +315,10 @@ class Server:
         self.done.append(r)
 
     def _launch(self, slots):
-        """Grouped J/K of the slots' current densities on the current stream;
-        returns the event that marks its end."""
+        """Grouped J/K of the slots' current densities on the current stream,
+        one launch per precision group; returns each slot's event, recorded
+        right after its own group's launch (a request of the first group
+        starts its absorb while the next group's J/K still runs)."""
         t = _dev.torch()
         main = t.cuda.current_stream()
         groups = collections.defaultdict(list)
@@ -330,12 +332,15 @@ class Server:
         for prec, members in sorted(groups.items()):
             key, g = self._plan(members, prec)
             self._plan_tick[key] = self.ticks
-            plans.append(g)
-        for g in plans:
+            plans.append((g, members))
+        done = {}
+        for g, members in plans:
             g.run()
-        ev = t.cuda.Event()
-        ev.record(main)
-        return ev
+            ev = t.cuda.Event()
+            ev.record(main)
+            for s in members:
+                done[s.serial] = ev
+        return done
 
     def _absorb_all(self, slots):
         """Each slot absorbs its J/K on its own stream from a worker thread;
@@ -411,9 +416,9 @@ class Server:
         ready = [s for s in self.active if s.future is None]
         if ready:
             self.ticks += 1
-            ev = self._launch(ready)
+            evs = self._launch(ready)
             for s in ready:
-                s.jk_done = ev
+                s.jk_done = evs[s.serial]
                 s.future = self._pool.submit(self._absorb_one, s)
             self.timing['launch_s'] += time.perf_counter() - t1
             self.trace.append((self.now(), len(self.active), len(self.queue), self.resident))
@@ -449,9 +454,9 @@ class Server:
         else:
             launch = absorb = [s for s in self.active if s.jk_done is None]
         if launch:
-            ev = self._launch(launch)
+            evs = self._launch(launch)
             for s in launch:
-                s.jk_done = ev
+                s.jk_done = evs[s.serial]
         finished = self._absorb_all(absorb) if absorb else []
         # plans unused for a while are freed (a free synchronises the device,
         # so not every tick)
