@@ -194,6 +194,15 @@ class RHF:
         d = A.diagonal()
         off = A.abs().sum(dim=1) - d.abs()
         emin, emax = t.stack([(d - off).min(), (d + off).max()]).tolist()
+        if self._spectrum is not None:
+            # the last diagonalised spectrum, widened by PURIFY_MARGIN of its
+            # width on each side, inside the Gershgorin bounds (those are
+            # often ten times wider, and every doubling of the width costs
+            # steps); a spectrum that has moved beyond it makes the steps
+            # fail to converge to trace nocc, and the caller diagonalises
+            lo, hi = self._spectrum
+            m = self.PURIFY_MARGIN * (hi - lo)
+            emin, emax = max(emin, lo - m), min(emax, hi + m)
         if not emax > emin:
             return None
         # two iterates and their per-tile (tr, ||.||_F^2) sums, ping-pong
@@ -233,7 +242,9 @@ class RHF:
     #: most SP2 steps before falling back to a diagonalisation
     PURIFY_MAXIT = 96
     #: SP2 steps between reads of the convergence measure
-    PURIFY_BLOCK = 8
+    PURIFY_BLOCK = 4
+    #: widening of the last diagonalised spectrum for the SP2 bounds
+    PURIFY_MARGIN = 0.05
 
     # ------------------------------------------------------------- DIIS
     def _diis_reset(self):
@@ -280,11 +291,13 @@ class RHF:
     def _occupied(self, A):
         """The next cycle's occupied AO coefficients (nao x nocc) from the
         DIIS-extrapolated Fock matrix in the orthogonal basis: purified
-        (``purify_min``) or from a diagonalisation; the occupied orbitals in
+        (``purify_min``) or from a diagonalisation, which also gives the
+        next purifications their spectral bounds; the occupied orbitals in
         the orthogonal basis are kept (the next purification and DIIS error
         start from them)."""
         t = _dev.torch()
-        if self.purify_min is not None and self.X.shape[1] > self.purify_min:
+        purify = self.purify_min is not None and self.X.shape[1] > self.purify_min
+        if purify:
             A = .5 * (A + A.T)
             q = self._purify_occ(A)
             if q is not None:
@@ -296,6 +309,8 @@ class RHF:
         e, C, c = self._diag_orth(A)
         self._q_occ = c[:, :self.nocc].contiguous()
         self.mo_energy, self.mo_coeff = e, C
+        if purify:
+            self._spectrum = tuple(t.stack([e[0], e[-1]]).tolist())
         return C
 
     def _diag(self, F, orth=False):
@@ -361,6 +376,8 @@ class RHF:
         grouped launch while each solve follows exactly this sequence."""
         if not hasattr(self, 'H'):
             self.build()
+        # the spectrum of the last diagonalised Fock matrix (SP2 bounds)
+        self._spectrum = None
         if dm0 is None:
             _, C, c = self._diag(self.H, orth=True)
             self._q_occ = c[:, :self.nocc].contiguous()
