@@ -28,6 +28,7 @@ import math
 import threading
 
 import numpy as np
+from scipy.linalg import lapack as scipy_lapack
 from threadpoolctl import threadpool_limits
 
 from . import _dev
@@ -183,9 +184,11 @@ class RHF:
         products only; tr(X^2) = ||X||_F^2 for the symmetric X, so each step
         is one kernel).  The basis is
         P R (R^T P R)^-1/2 by a Cholesky factor (``mdf_potrf`` /
-        ``mdf_trtri``), R the previous cycle's occupied orbitals: no solver
-        library is called.  None (the caller diagonalises) when the steps do
-        not converge (a vanishing gap at nocc) or P R loses rank."""
+        ``mdf_trtri``), R the previous cycle's occupied orbitals, or, when P R
+        loses rank, from P alone by a pivoted Cholesky (``_projector_basis``):
+        no device solver library is called.  None (the caller diagonalises)
+        when the steps do not converge (a vanishing gap at nocc) or P does
+        not have rank nocc."""
         t = _dev.torch()
         n, nocc = A.shape[0], self.nocc
         R = getattr(self, '_q_occ', None)
@@ -239,9 +242,37 @@ class RHF:
         PR = X @ R
         M = (R.T @ PR).contiguous()
         L, ok = _cholesky_inverse(M)
+        if ok:
+            return PR @ L.T
+        # R misses part of the occupied space (early in a solve it rotates by
+        # more than the previous orbitals cover): the basis from P alone
+        return self._projector_basis(X)
+
+    def _projector_basis(self, P):
+        """An orthonormal basis (n x nocc) of the range of the projector P,
+        or None: LAPACK's pivoted Cholesky (``dpstrf``, one host thread)
+        stops at rank nocc with P = W W^T, and for a projector W^T W = I;
+        one Cholesky-QR pass on the device (``mdf_potrf`` / ``mdf_trtri``)
+        removes the rounding left in W^T W."""
+        t = _dev.torch()
+        n, nocc = P.shape[0], self.nocc
+        a = P.cpu().numpy()
+        if self.concurrent_diag:
+            if not _single_blas:
+                raise RuntimeError('concurrent purification outside single_thread_blas')
+            c, piv, rank, info = scipy_lapack.dpstrf(a, tol=1e-8, lower=1)
+        else:
+            with _HOST_LAPACK, threadpool_limits(1, user_api='blas'):
+                c, piv, rank, info = scipy_lapack.dpstrf(a, tol=1e-8, lower=1)
+        if info < 0 or rank != nocc:
+            return None
+        W = np.zeros((n, nocc))
+        W[piv - 1] = np.tril(c)[:, :nocc]
+        W = t.from_numpy(W).to(P.device)
+        L, ok = _cholesky_inverse((W.T @ W).contiguous())
         if not ok:
             return None
-        return PR @ L.T
+        return W @ L.T
 
     #: most SP2 steps before falling back to a diagonalisation
     PURIFY_MAXIT = 96

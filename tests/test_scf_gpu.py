@@ -99,3 +99,26 @@ def test_conv_check(mol):
     assert ca == cb and na == ca and nb == cb + 1
     assert ea == ha
     assert abs(ea - eb) < 1e-9, (ea, eb)
+
+
+@pytest.mark.gpu
+def test_projector_basis():
+    """The purification's basis from the projector alone (pivoted Cholesky,
+    then one Cholesky-QR pass): for a random rank-nocc projector P on the
+    device, Q^T Q = I and Q Q^T = P to 1e-12; a matrix of another rank is
+    refused."""
+    import torch
+    from electrolites_amd.scf import RHF
+    rng = np.random.default_rng(5)
+    n, nocc = 230, 47
+    q, _ = np.linalg.qr(rng.standard_normal((n, nocc)))
+    P = torch.from_numpy(q @ q.T).cuda()
+    mf = RHF.__new__(RHF)
+    mf.nocc, mf.concurrent_diag = nocc, False
+    Q = mf._projector_basis(P)
+    Qh = Q.cpu().numpy()
+    assert Qh.shape == (n, nocc)
+    assert np.abs(Qh.T @ Qh - np.eye(nocc)).max() < 1e-12
+    assert np.abs(Qh @ Qh.T - q @ q.T).max() < 1e-12
+    mf.nocc = nocc + 1
+    assert mf._projector_basis(P) is None
