@@ -243,7 +243,11 @@ class RHF:
         M = (R.T @ PR).contiguous()
         L, ok = _cholesky_inverse(M)
         if ok:
-            return PR @ L.T
+            Q = PR @ L.T
+            # a nearly singular R^T P R passes the pivot test but leaves Q
+            # far from orthonormal: then the basis comes from P alone
+            if float((Q.T @ Q - t.eye(nocc, dtype=Q.dtype, device=Q.device)).abs().max()) < 1e-10:
+                return Q
         # R misses part of the occupied space (early in a solve it rotates by
         # more than the previous orbitals cover): the basis from P alone
         return self._projector_basis(X)
