@@ -82,6 +82,25 @@ def host_eigh(a, concurrent=False):
         return np.linalg.eigh(a)
 
 
+def projector_factor(a, rank, concurrent=False):
+    """W (n x rank) with a = W W^T for a symmetric projector a of that rank,
+    by LAPACK's pivoted Cholesky (``dpstrf``, stopped at pivots below 1e-8,
+    on one host thread as :func:`host_eigh`), or None when a's numerical
+    rank differs.  For a projector W^T W = I."""
+    if concurrent:
+        if not _single_blas:
+            raise RuntimeError('projector_factor(concurrent=True) outside single_thread_blas')
+        c, piv, r, info = scipy_lapack.dpstrf(a, tol=1e-8, lower=1)
+    else:
+        with _HOST_LAPACK, threadpool_limits(1, user_api='blas'):
+            c, piv, r, info = scipy_lapack.dpstrf(a, tol=1e-8, lower=1)
+    if info < 0 or r != rank:
+        return None
+    W = np.zeros((a.shape[0], rank))
+    W[piv - 1] = np.tril(c)[:, :rank]
+    return W
+
+
 def _cholesky_inverse(M):
     """``(L^-1, ok)`` for the SPD device matrix M = L L^T (``mdf_potrf`` and
     ``mdf_trtri``, the build's own kernels); ok False for a pivot below
@@ -259,19 +278,10 @@ class RHF:
         one Cholesky-QR pass on the device (``mdf_potrf`` / ``mdf_trtri``)
         removes the rounding left in W^T W."""
         t = _dev.torch()
-        n, nocc = P.shape[0], self.nocc
-        a = P.cpu().numpy()
-        if self.concurrent_diag:
-            if not _single_blas:
-                raise RuntimeError('concurrent purification outside single_thread_blas')
-            c, piv, rank, info = scipy_lapack.dpstrf(a, tol=1e-8, lower=1)
-        else:
-            with _HOST_LAPACK, threadpool_limits(1, user_api='blas'):
-                c, piv, rank, info = scipy_lapack.dpstrf(a, tol=1e-8, lower=1)
-        if info < 0 or rank != nocc:
+        nocc = self.nocc
+        W = projector_factor(P.cpu().numpy(), nocc, self.concurrent_diag)
+        if W is None:
             return None
-        W = np.zeros((n, nocc))
-        W[piv - 1] = np.tril(c)[:, :nocc]
         W = t.from_numpy(W).to(P.device)
         L, ok = _cholesky_inverse((W.T @ W).contiguous())
         if not ok:
