@@ -274,3 +274,20 @@ def test_unrestricted_gradient_matches_finite_differences(mol, tagged):
     gu = fastdfgrad.jk_energy_per_atom(df, np.stack([D / 2, D / 2]), 1., 1.)
     gr = fastdfgrad.jk_energy_per_atom(df, D, 1., 1.)
     assert np.abs(gu - gr).max() < 1e-10
+
+
+@pytest.mark.parametrize('n', [1, 2, 7, 320])
+def test_packed_diagonal_positions(n):
+    """The diagonal offsets the gradient reads rho_B from in the packed
+    upper triangles (row-major over a <= b, the layout of mdf_sym_pack) are
+    where numpy's triu_indices order puts a == b, and a packed symmetric
+    matrix with its off-diagonal entries times sqrt 2 has the Frobenius dot
+    product of the full ones (the trace products tr(dmoo_P dmoo_Q))."""
+    from electrolites_amd.fastdfgrad import SQRT2, _diag_positions
+    a, b = np.triu_indices(n)
+    assert np.array_equal(_diag_positions(n), np.nonzero(a == b)[0])
+    rng = np.random.default_rng(n)
+    x, y = rng.standard_normal((2, n, n))
+    x, y = x + x.T, y + y.T
+    w = np.where(a == b, 1., SQRT2)
+    assert abs((w * x[a, b]) @ (w * y[a, b]) - (x * y).sum()) < 1e-12 * n * n
