@@ -61,7 +61,7 @@ def check_targets(names=None):
                 for part in attr.split('.'):
                     obj = getattr(obj, part)
             except (ImportError, AttributeError) as exc:
-                out[name] = f'{mod_path}.{attr} is missing ({exc})'
+                out[name] = f'cannot resolve {mod_path}.{attr}: {exc}'
                 break
     return out
 
@@ -109,12 +109,12 @@ def patch(*names, strict=True):
     names = [n for g in (names or (MODULES,)) for n in ((g,) if isinstance(g, str) else g)]
     unknown = [n for n in names if n not in _TARGETS]
     if unknown:
-        raise ValueError(f'unknown module(s) {unknown}; known: {list(MODULES)}')
+        raise ValueError(f'patch: {unknown} not among the patchable modules {list(MODULES)}')
     missing = {k: v for k, v in check_targets(names).items() if v}
     if missing:
         if strict:
-            raise RuntimeError(f'GPU4PySCF does not expose the entry point(s) '
-                               f'these modules replace: {missing}')
+            raise RuntimeError(f'patch(strict=True): the installed GPU4PySCF lacks targets '
+                               f'of {sorted(missing)}: {missing}')
         names = [n for n in names if n not in missing]
     for name in names:
         _install(name)
